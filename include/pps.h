@@ -1,0 +1,213 @@
+/*
+ * pps.h -- C ABI of the MI355X pressure-Poisson solver (libpps.so).
+ *
+ * Drop-in for the PARIS linear-solver entry points:
+ *   NewSolver(A,p,maxError,beta,maxit,it,ierr,norm)         newsolver.f90:34-57
+ *   NewSolverMG(A,p,maxError,beta,maxit,it,ierr,norm,tres2) MGsolver.f90:346-405
+ *   init_MG(nvar) / finalize_MG                             MGsolver.f90:34-82
+ * The module state those routines read implicitly (module_grid bounds and
+ * Ng, module_BC bdry_cond, the MultiGrid/nrelax/sym_MG flags,
+ * modules.f90:57-73) is passed explicitly in pps_grid / pps_options.
+ *
+ * Array conventions are the reference's (Fortran order, x fastest):
+ *   A : 8 planes of nx*ny*nz doubles; slots 1..6 = x-,x+,y-,y+,z-,z+
+ *       couplings, 7 = diagonal, 8 = right-hand side.  Never written.
+ *   p : (nx+2ng)*(ny+2ng)*(nz+2ng) doubles with ng ghost layers; initial
+ *       guess in, solution out.
+ *   equation: A7 p = A1 p(i-1) + A2 p(i+1) + A3 p(j-1) + A4 p(j+1)
+ *                    + A5 p(k-1) + A6 p(k+1) + A8
+ * Stopping rule (MGsolver.f90:385-396): tres2 = (sum |r|^norm / N)^(1/norm
+ * if norm==2), exit when tres2 < maxError; *it = maxit+1 if not converged.
+ *
+ * Ghost semantics: periodic directions wrap (the solver fills the first
+ * ghost layer of p with the wrapped solution); in wall directions the ghost
+ * values of p are fixed boundary data and are returned unchanged.
+ *
+ * All entry points are synchronous on return, thread-compatible, and not
+ * re-entrant per handle.  Device-pointer variants take a hipStream_t as
+ * void*; NULL means the handle's own stream.
+ */
+#ifndef PPS_H
+#define PPS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- return codes (0 = success) ------------------------------------ */
+enum pps_status {
+    PPS_OK = 0,
+    PPS_ERR_NAN_OR_TINY_DIAG = 1,  /* check_poisson_setup, newsolver.f90:295-310 */
+    PPS_ERR_VOLUME_SOURCE = 2,     /* mean(A8) > 1e-10, newsolver.f90:314-321 */
+    PPS_ERR_NOT_IN_KERNEL = 3,     /* row sums != 0, newsolver.f90:325-343 */
+    PPS_ERR_BAD_SIZE = 4,          /* init_MG power-of-two rule, MGsolver.f90:44-46 */
+    PPS_ERR_DIVERGED = 5,          /* catch_divergence, newsolver.f90:138-173 */
+    PPS_ERR_NOT_CONVERGED = 6,     /* it = maxit+1 (MGsolver.f90:402): warning */
+    PPS_ERR_NONSYMMETRIC = 7,      /* fast multigrid needs A2(i)=A1(i+1) etc. */
+    PPS_ERR_BAD_ARG = 8,
+    PPS_ERR_HIP = 9,               /* HIP runtime failure (out of memory, ...) */
+    PPS_ERR_NO_DEVICE = 10,
+    PPS_ERR_UNSUPPORTED = 11       /* e.g. pressure boundary codes 5/6 in assembly */
+};
+
+/* ---- solver methods --------------------------------------------------- */
+enum pps_method {
+    /* NewSolverMG restated on the GPU: RB-SOR(beta), nrelax sweeps, 8-slot
+     * child-average coarsening, piecewise-constant prolongation
+     * (MGsolver.f90:203-320).  FP64.  Same iteration history as the
+     * reference up to summation order. */
+    PPS_MG_REFERENCE = 0,
+    /* FP64 multigrid-preconditioned CG: red-black Gauss-Seidel, fused
+     * residual + full-weighting restriction, face-coefficient coarsening,
+     * trilinear prolongation, dense coarsest solve in one workgroup. */
+    PPS_MG_FP64 = 1,
+    /* Same V-cycle in FP32 (FP32 coefficients and accumulation) inside the
+     * FP64 outer iteration: residual, update and reductions stay FP64. */
+    PPS_MG_MIXED = 2,
+    /* NewSolver_std restated: plain RB-SOR to tolerance (MultiGrid = F,
+     * newsolver.f90:78-174). */
+    PPS_SOR = 3
+};
+
+enum pps_outer {
+    PPS_OUTER_CG = 0,      /* preconditioned conjugate gradients */
+    PPS_OUTER_REFINE = 1   /* plain iterative refinement p += V(b - A p) */
+};
+
+typedef struct pps_grid {
+    int nx, ny, nz;        /* interior cells (Nx, Ny, Nz) */
+    int ng;                /* ghost layers of p and the fields (Ng) */
+    int bdry_cond[6];      /* x-, y-, z-, x+, y+, z+ (modules.f90:870-877) */
+    double xlength, ylength, zlength;  /* box size, used by assembly only */
+} pps_grid;
+
+typedef struct pps_options {
+    int method;            /* enum pps_method */
+    int outer;             /* enum pps_outer (PPS_MG_FP64 / PPS_MG_MIXED) */
+    int nrelax;            /* reference nrelax (default 3) */
+    int sym_mg;            /* reference sym_MG (default 0) */
+    int nu_pre, nu_post;   /* fast V-cycle smoothing sweeps (default 2, 2) */
+    int check_setup;       /* run the check_poisson_setup tests first (default 1) */
+    int multigrid;         /* reference MultiGrid flag for pps_new_solver:
+                              1 -> method, 0 -> PPS_SOR (default 1) */
+} pps_options;
+
+typedef struct pps_handle pps_handle;
+
+/* Fill *opt with the defaults named above (method PPS_MG_FP64). */
+int pps_default_options(pps_options *opt);
+
+/* init_MG (MGsolver.f90:34-65): validate the grid, pick the device's
+ * current HIP device, allocate the level hierarchy lazily. */
+int pps_init(const pps_grid *grid, const pps_options *opt, pps_handle **out);
+/* finalize_MG (MGsolver.f90:67-82). */
+int pps_finalize(pps_handle *h);
+/* Change the method / options of an existing handle. */
+int pps_set_options(pps_handle *h, const pps_options *opt);
+
+/* NewSolver (newsolver.f90:34-57), host arrays.  Dispatches on
+ * opt.multigrid like the reference; *ierr = 0 on success (the reference's
+ * last MPI code), otherwise the pps_status value. */
+int pps_new_solver(pps_handle *h, const double *A, double *p, double maxError,
+                   double beta, int maxit, int *it, int *ierr, int norm);
+/* NewSolverMG (MGsolver.f90:346-405), host arrays, also returns tres2. */
+int pps_new_solver_mg(pps_handle *h, const double *A, double *p,
+                      double maxError, double beta, int maxit, int *it,
+                      int *ierr, int norm, double *tres2);
+
+/* Device-pointer solve: dA (8 planes) and dp (padded) already in HBM. */
+int pps_solve_device(pps_handle *h, const double *dA, double *dp,
+                     double maxError, double beta, int maxit, int norm,
+                     int *it, double *tres2, void *stream);
+
+/* Residual history of the last solve: tres2 after each outer iteration.
+ * Returns the number of entries written (<= cap). */
+int pps_last_history(pps_handle *h, double *hist, int cap);
+
+/* Adjoint of the last-solved system (no reference counterpart).  With
+ * M = diag(A7) - offdiag(A1..A6) symmetric, solves M lam = g (g projected
+ * mean-free when M is singular) on the same hierarchy, to relative residual
+ * rtol.  dg, dlam: padded like p. */
+int pps_adjoint_device(pps_handle *h, const double *dA, const double *dg,
+                       double *dlam, double rtol, int maxit, int *it,
+                       void *stream);
+/* Fused face kernel: grad_f = -w_f (lam_L - lam_R)(p_L - p_R) for the
+ * +x, +y, +z face of every cell (3 planes of nx*ny*nz), w_f = the
+ * geometric weight dt/(dx*dxh) of the face; a face with a closed mask
+ * (coefficient 0) gets 0.  wx/wy/wz: per-plane weights (length
+ * nx/ny/nz).  This is dJ/dc_f for c_f = 2/(rho_L+rho_R) = "1/rho at the
+ * face" (modules.f90:2737-2742). */
+int pps_face_gradient_device(const pps_grid *grid, const double *dA,
+                             const double *dp, const double *dlam,
+                             const double *wx, const double *wy,
+                             const double *wz, double *dgrad, void *stream);
+/* The linear map from face values to the 8-slot system that the face
+ * gradient differentiates: with c = 3 planes (+x, +y, +z face of every
+ * cell) and per-plane weights w, A2(i) = A1(i+1) = w_x(i) c_x(i) (wrapping
+ * in periodic directions, 0 through walls), likewise y and z,
+ * A7 = sum(A1..A6), A8 = b (interior, nx*ny*nz).  It is SetupPoisson's
+ * coefficient rule (modules.f90:2737-2743) with c_f = 2/(rho_L+rho_R) and
+ * w = dt/(dx*dxh) on a uniform grid. */
+int pps_faces_to_A_device(const pps_grid *grid, const double *cface,
+                          const double *wx, const double *wy,
+                          const double *wz, const double *b, double *A,
+                          void *stream);
+
+/* ---- producer / consumer of the system (modules.f90, paris.f90) ------ */
+/* SetPressureBC on all-ones masks (modules.f90:916-946), padded arrays. */
+int pps_pressure_masks_device(const pps_grid *grid, double *umask,
+                              double *vmask, double *wmask, void *stream);
+/* SetupPoisson + Poisson_BCs codes 0..4 + the solid-cell rule of
+ * check_and_debug_Poisson (modules.f90:2722-2953), padded fields in,
+ * 8-slot A out. */
+int pps_setup_poisson_device(const pps_grid *grid, const double *u,
+                             const double *v, const double *w,
+                             const double *umask, const double *vmask,
+                             const double *wmask, const double *rho, double dt,
+                             double volume_source, double *A, void *stream);
+/* check_poisson_setup (newsolver.f90:275-344); solid cells are those with
+ * six zero couplings. */
+int pps_check_setup_device(const pps_grid *grid, const double *A, void *stream);
+/* Subtract the mean of A8 (compatibility projection of the right-hand side). */
+int pps_remove_rhs_mean_device(const pps_grid *grid, double *A, void *stream);
+/* calcResiduals (modules.f90:3313-3384): out = {L1, L2, Linf}. */
+int pps_calc_residuals_device(const pps_grid *grid, const double *A,
+                              const double *p, double out[3], void *stream);
+/* project_velocity, staggered branch (paris.f90:898-908). */
+int pps_project_velocity_device(const pps_grid *grid, double *u, double *v,
+                                double *w, const double *umask,
+                                const double *vmask, const double *wmask,
+                                const double *rho, const double *p, double dt,
+                                void *stream);
+
+/* ---- synthetic inputs (benchmark / test fields, no reference code) ---- */
+/* Density from spherical droplets (tanh interface of width `width` cells,
+ * volume fraction capped at 1, rho = rho1 + (rho2-rho1) c) and iid
+ * U(-1,1) face velocities from a counter-based splitmix64 stream.  Wall
+ * faces get zero normal velocity; ghosts wrap (periodic) or copy the
+ * nearest interior cell (walls).  drops: ndrop x {cx, cy, cz, R}. */
+int pps_synth_fields_device(const pps_grid *grid, const double *drops,
+                            int ndrop, double rho1, double rho2, double width,
+                            unsigned long long seed, double *rho, double *u,
+                            double *v, double *w, void *stream);
+/* Droplet list: ndrop centres uniform in the box and radii uniform in
+ * [rmin, rmax] from splitmix64(seed); host memory, 4 doubles per drop. */
+int pps_synth_drops(const pps_grid *grid, int ndrop, double rmin, double rmax,
+                    unsigned long long seed, double *drops);
+
+/* Uniform-grid coordinates as paris.f90:2959-3102 builds them (xform=0):
+ * arrays of N+2ng entries, entry q = Fortran index q+1. */
+int pps_coords(int N, int ng, double length, double *x, double *xh,
+               double *dx, double *dxh);
+
+/* Human-readable name of a status code. */
+const char *pps_status_string(int status);
+
+/* Timing of the last solve on the handle's stream: milliseconds of the
+ * whole solve and of the setup (pack + coarsening) part. */
+int pps_last_timing(pps_handle *h, double *solve_ms, double *setup_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,818 @@
+/*
+ * pps_oracle.c -- TEST INFRASTRUCTURE, NOT PRODUCT CODE.
+ *
+ * Plain-C FP64 restatement of the reference PARIS pressure-Poisson path
+ * (MGsolver.f90, newsolver.f90, modules.f90, paris.f90).  Each function
+ * cites the reference lines it follows.  Only tests/, smoke() and the CPU
+ * baseline leg of bench.py use this file, as the checker; the product
+ * library never links it.
+ *
+ * Built with -ffp-contract=off so every product of a sum is rounded
+ * separately, as in the reference's plain Fortran expressions; sums run in
+ * the reference's left-to-right order and its k,j,i loop order.
+ *
+ * Single-rank semantics: MPI halo exchanges become a periodic wrap of one
+ * ghost layer in periodic directions, and no-ops at walls (MPI_PROC_NULL);
+ * MPI_ALLREDUCE over one rank is the identity.
+ */
+#include "pps_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------ */
+/* level geometry                                                      */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    int n[3];     /* interior cells */
+    int ng;       /* ghost layers */
+    int X, Y, Z;  /* padded extents */
+    int per[3];   /* periodic flags */
+} lvl_t;
+
+static lvl_t lvl_make(const int n[3], int ng, const int per[3]) {
+    lvl_t L;
+    for (int d = 0; d < 3; ++d) { L.n[d] = n[d]; L.per[d] = per[d]; }
+    L.ng = ng;
+    L.X = n[0] + 2 * ng; L.Y = n[1] + 2 * ng; L.Z = n[2] + 2 * ng;
+    return L;
+}
+static inline long ic(const lvl_t *L, int i, int j, int k) {
+    return (long)i + (long)L->n[0] * ((long)j + (long)L->n[1] * k);
+}
+static inline long ip(const lvl_t *L, int i, int j, int k) {
+    return (long)(i + L->ng) + (long)L->X * ((long)(j + L->ng) + (long)L->Y * (k + L->ng));
+}
+static inline long ncell(const lvl_t *L) { return (long)L->n[0] * L->n[1] * L->n[2]; }
+static inline long npad(const lvl_t *L) { return (long)L->X * L->Y * L->Z; }
+
+static lvl_t grid_lvl(const orc_grid *g) {
+    int n[3] = {g->nx, g->ny, g->nz};
+    int per[3] = {g->bc[0] == 1, g->bc[1] == 1, g->bc[2] == 1};
+    return lvl_make(n, g->ng, per);
+}
+
+/* ------------------------------------------------------------------ */
+/* coordinates: paris.f90:3045-3102 with xform = 0 and read_x = F       */
+/* ------------------------------------------------------------------ */
+void orc_coords(int N, int ng, double length, double *x, double *xh,
+                double *dx, double *dxh) {
+    int nt = N + 2 * ng;
+    /* Fortran index q = 1..nt maps to array entry q-1 */
+    for (int q = 1; q <= nt; ++q) {
+        double s = (double)(q - ng) / (double)N;
+        double form = 0.0 * s * (0.5 - s) * (1.0 - s);  /* xform = 0 */
+        xh[q - 1] = length * (form + s);
+    }
+    for (int q = 2; q <= nt; ++q) x[q - 1] = 0.5 * (xh[q - 1] + xh[q - 2]);
+    x[0] = 2.0 * xh[0] - x[1];
+    for (int q = 1; q <= nt - 1; ++q) dxh[q - 1] = x[q] - x[q - 1];
+    dxh[nt - 1] = dxh[nt - 2];
+    for (int q = 2; q <= nt; ++q) dx[q - 1] = xh[q - 1] - xh[q - 2];
+    dx[0] = dx[1];
+}
+
+typedef struct {
+    double *x, *xh, *dx, *dxh;
+} axis_t;
+
+static int axis_alloc(axis_t *a, int N, int ng, double length) {
+    int nt = N + 2 * ng;
+    a->x = malloc(sizeof(double) * nt * 4);
+    if (!a->x) return -1;
+    a->xh = a->x + nt; a->dx = a->xh + nt; a->dxh = a->dx + nt;
+    orc_coords(N, ng, length, a->x, a->xh, a->dx, a->dxh);
+    return 0;
+}
+static void axis_free(axis_t *a) { free(a->x); }
+
+/* ------------------------------------------------------------------ */
+/* SetPressureBC: modules.f90:916-946                                  */
+/* ------------------------------------------------------------------ */
+void orc_pressure_masks(const orc_grid *g, double *umask, double *vmask,
+                        double *wmask) {
+    lvl_t L = grid_lvl(g);
+    long np = npad(&L);
+    for (long q = 0; q < np; ++q) umask[q] = vmask[q] = wmask[q] = 1.0;
+    int nx = g->nx, ny = g->ny, nz = g->nz;
+    /* wall (0) or shear (2): zero the wall-normal face mask, over the
+     * interior +-1 cell band of the tangential directions */
+    for (int k = -1; k <= nz; ++k)
+        for (int j = -1; j <= ny; ++j) {
+            if (g->bc[0] == 0 || g->bc[0] == 2) umask[ip(&L, -1, j, k)] = 0.0;
+            if (g->bc[3] == 0 || g->bc[3] == 2) umask[ip(&L, nx - 1, j, k)] = 0.0;
+        }
+    for (int k = -1; k <= nz; ++k)
+        for (int i = -1; i <= nx; ++i) {
+            if (g->bc[1] == 0 || g->bc[1] == 2) vmask[ip(&L, i, -1, k)] = 0.0;
+            if (g->bc[4] == 0 || g->bc[4] == 2) vmask[ip(&L, i, ny - 1, k)] = 0.0;
+        }
+    for (int j = -1; j <= ny; ++j)
+        for (int i = -1; i <= nx; ++i) {
+            if (g->bc[2] == 0 || g->bc[2] == 2) wmask[ip(&L, i, j, -1)] = 0.0;
+            if (g->bc[5] == 0 || g->bc[5] == 2) wmask[ip(&L, i, j, nz - 1)] = 0.0;
+        }
+}
+
+/* ------------------------------------------------------------------ */
+/* SetupPoisson (2722-2753) + Poisson_BCs codes 3/4 (2755-2892)         */
+/* + check_and_debug_Poisson (2894-2953)                                */
+/* ------------------------------------------------------------------ */
+int orc_setup_poisson(const orc_grid *g, const double *u, const double *v,
+                      const double *w, const double *umask,
+                      const double *vmask, const double *wmask,
+                      const double *rho, double dt, double volume_source,
+                      double *A) {
+    for (int f = 0; f < 6; ++f)
+        if (g->bc[f] >= 5) return ORC_ERR_UNSUPPORTED;
+    lvl_t L = grid_lvl(g);
+    long n = ncell(&L);
+    axis_t ax, ay, az;
+    if (axis_alloc(&ax, g->nx, g->ng, g->xlength)) return ORC_ERR_NOMEM;
+    if (axis_alloc(&ay, g->ny, g->ng, g->ylength)) { axis_free(&ax); return ORC_ERR_NOMEM; }
+    if (axis_alloc(&az, g->nz, g->ng, g->zlength)) { axis_free(&ax); axis_free(&ay); return ORC_ERR_NOMEM; }
+    const int G = g->ng;
+    for (int k = 0; k < g->nz; ++k)
+        for (int j = 0; j < g->ny; ++j)
+            for (int i = 0; i < g->nx; ++i) {
+                long c = ic(&L, i, j, k);
+                long q = ip(&L, i, j, k);
+                /* coordinate arrays are indexed by padded index i+G */
+                int I = i + G, J = j + G, K = k + G;
+                double r0 = rho[q];
+                double a1 = 2.0 * dt * umask[ip(&L, i - 1, j, k)] /
+                            (ax.dx[I] * ax.dxh[I - 1] * (rho[ip(&L, i - 1, j, k)] + r0));
+                double a2 = 2.0 * dt * umask[q] /
+                            (ax.dx[I] * ax.dxh[I] * (rho[ip(&L, i + 1, j, k)] + r0));
+                double a3 = 2.0 * dt * vmask[ip(&L, i, j - 1, k)] /
+                            (ay.dx[J] * ay.dxh[J - 1] * (rho[ip(&L, i, j - 1, k)] + r0));
+                double a4 = 2.0 * dt * vmask[q] /
+                            (ay.dx[J] * ay.dxh[J] * (rho[ip(&L, i, j + 1, k)] + r0));
+                double a5 = 2.0 * dt * wmask[ip(&L, i, j, k - 1)] /
+                            (az.dx[K] * az.dxh[K - 1] * (rho[ip(&L, i, j, k - 1)] + r0));
+                double a6 = 2.0 * dt * wmask[q] /
+                            (az.dx[K] * az.dxh[K] * (rho[ip(&L, i, j, k + 1)] + r0));
+                A[0 * n + c] = a1; A[1 * n + c] = a2; A[2 * n + c] = a3;
+                A[3 * n + c] = a4; A[4 * n + c] = a5; A[5 * n + c] = a6;
+                A[6 * n + c] = ((((a1 + a2) + a3) + a4) + a5) + a6;
+                A[7 * n + c] = -(((volume_source + (u[q] - u[ip(&L, i - 1, j, k)]) / ax.dx[I]) +
+                                  (v[q] - v[ip(&L, i, j - 1, k)]) / ay.dx[J]) +
+                                 (w[q] - w[ip(&L, i, j, k - 1)]) / az.dx[K]);
+            }
+    axis_free(&ax); axis_free(&ay); axis_free(&az);
+
+    /* Poisson_BCs: flux conditions on a single rank (codes 3 and 4) */
+    for (int k = 0; k < g->nz; ++k)
+        for (int j = 0; j < g->ny; ++j) {
+            long c0 = ic(&L, 0, j, k), c1 = ic(&L, g->nx - 1, j, k);
+            if (g->bc[0] == 3) { A[6 * n + c0] = A[6 * n + c0] - A[0 * n + c0]; A[0 * n + c0] = 0.0; }
+            if (g->bc[3] == 4 || g->bc[3] == 3) { A[6 * n + c1] = A[6 * n + c1] - A[1 * n + c1]; A[1 * n + c1] = 0.0; }
+        }
+    for (int k = 0; k < g->nz; ++k)
+        for (int i = 0; i < g->nx; ++i) {
+            long c0 = ic(&L, i, 0, k), c1 = ic(&L, i, g->ny - 1, k);
+            if (g->bc[1] == 3) { A[6 * n + c0] = A[6 * n + c0] - A[2 * n + c0]; A[2 * n + c0] = 0.0; }
+            if (g->bc[4] == 3) { A[6 * n + c1] = A[6 * n + c1] - A[3 * n + c1]; A[3 * n + c1] = 0.0; }
+        }
+    for (int j = 0; j < g->ny; ++j)
+        for (int i = 0; i < g->nx; ++i) {
+            long c0 = ic(&L, i, j, 0), c1 = ic(&L, i, j, g->nz - 1);
+            if (g->bc[2] == 3) { A[6 * n + c0] = A[6 * n + c0] - A[4 * n + c0]; A[4 * n + c0] = 0.0; }
+            if (g->bc[5] == 3) { A[6 * n + c1] = A[6 * n + c1] - A[5 * n + c1]; A[5 * n + c1] = 0.0; }
+        }
+
+    /* check_and_debug_Poisson: a tiny diagonal is only legal in a solid
+     * (all six face masks closed), where A7 becomes 1 */
+    for (int k = 0; k < g->nz; ++k)
+        for (int j = 0; j < g->ny; ++j)
+            for (int i = 0; i < g->nx; ++i) {
+                long c = ic(&L, i, j, k);
+                if (!(A[6 * n + c] < 1e-50)) continue;
+                int solid = umask[ip(&L, i - 1, j, k)] < 0.5 && umask[ip(&L, i, j, k)] < 0.5 &&
+                            vmask[ip(&L, i, j - 1, k)] < 0.5 && vmask[ip(&L, i, j, k)] < 0.5 &&
+                            wmask[ip(&L, i, j, k - 1)] < 0.5 && wmask[ip(&L, i, j, k)] < 0.5;
+                if (!solid) return ORC_ERR_SOLID;
+                if (A[7 * n + c] > 1e-50) return ORC_ERR_SOLID;
+                double mx = A[c], mn = A[c];
+                for (int s = 1; s < 6; ++s) {
+                    if (A[s * n + c] > mx) mx = A[s * n + c];
+                    if (A[s * n + c] < mn) mn = A[s * n + c];
+                }
+                if (mx > 1e-50 || mn < 0.0) return ORC_ERR_SOLID;
+                A[6 * n + c] = 1.0;
+            }
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* check_poisson_setup: newsolver.f90:275-344                          */
+/* ------------------------------------------------------------------ */
+int orc_check_poisson_setup(const orc_grid *g, const double *A,
+                            const double *umask, const double *vmask,
+                            const double *wmask, double *rowsum) {
+    lvl_t L = grid_lvl(g);
+    long n = ncell(&L);
+    for (long c = 0; c < n; ++c) {
+        for (int s = 0; s < 8; ++s)
+            if (A[s * n + c] != A[s * n + c]) return ORC_ERR_NAN_OR_TINY_DIAG;
+        if (A[6 * n + c] < 1e-55) return ORC_ERR_NAN_OR_TINY_DIAG;
+    }
+    /* Fortran sum() over the whole plane, then divided by the cell count;
+     * the test is one-sided as in the reference */
+    double src = 0.0;
+    for (long c = 0; c < n; ++c) src += A[7 * n + c];
+    src = src / (double)n;
+    if (src > 1e-10) return ORC_ERR_VOLUME_SOURCE;
+    double mx = -1e50, mn = 1e50;
+    for (int k = 0; k < g->nz; ++k)
+        for (int j = 0; j < g->ny; ++j)
+            for (int i = 0; i < g->nx; ++i) {
+                long c = ic(&L, i, j, k);
+                double a7 = A[6 * n + c];
+                if (umask[ip(&L, i - 1, j, k)] < 0.5 && umask[ip(&L, i, j, k)] < 0.5 &&
+                    vmask[ip(&L, i, j - 1, k)] < 0.5 && vmask[ip(&L, i, j, k)] < 0.5 &&
+                    wmask[ip(&L, i, j, k - 1)] < 0.5 && wmask[ip(&L, i, j, k)] < 0.5)
+                    a7 = 0.0;
+                double t = (((((A[c] + A[n + c]) + A[2 * n + c]) + A[3 * n + c]) + A[4 * n + c]) +
+                            A[5 * n + c]) - a7;
+                if (rowsum) rowsum[c] = t;
+                if (t > mx) mx = t;
+                if (t < mn) mn = t;
+            }
+    double big = mx > -mn ? mx : -mn;
+    if (big > 1e-10) return ORC_ERR_NOT_IN_KERNEL;
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* residuals                                                            */
+/* ------------------------------------------------------------------ */
+/* get_residual / compute_residual expression: MGsolver.f90:97-100, 119-123 */
+static inline double resid_mg(const double *A, long n, long c, const double *p,
+                              long q, long sx, long sy, long sz) {
+    return A[7 * n + c] - A[6 * n + c] * p[q] + A[0 * n + c] * p[q - sx] +
+           A[1 * n + c] * p[q + sx] + A[2 * n + c] * p[q - sy] + A[3 * n + c] * p[q + sy] +
+           A[4 * n + c] * p[q - sz] + A[5 * n + c] * p[q + sz];
+}
+
+void orc_residual_field(const orc_grid *g, const double *A, const double *p,
+                        double *r) {
+    lvl_t L = grid_lvl(g);
+    long n = ncell(&L), sx = 1, sy = L.X, sz = (long)L.X * L.Y;
+    for (int k = 0; k < g->nz; ++k)
+        for (int j = 0; j < g->ny; ++j)
+            for (int i = 0; i < g->nx; ++i) {
+                long c = ic(&L, i, j, k);
+                r[c] = resid_mg(A, n, c, p, ip(&L, i, j, k), sx, sy, sz);
+            }
+}
+
+static double powabs(double r, int norm) {
+    double a = fabs(r);
+    if (norm == 1) return a;
+    if (norm == 2) return a * a;
+    return pow(a, (double)norm);
+}
+
+static double compute_residual_lvl(const lvl_t *L, const double *A,
+                                   const double *p, int norm) {
+    long n = ncell(L), sx = 1, sy = L->X, sz = (long)L->X * L->Y;
+    double acc = 0.0;
+    for (int k = 0; k < L->n[2]; ++k)
+        for (int j = 0; j < L->n[1]; ++j)
+            for (int i = 0; i < L->n[0]; ++i) {
+                long c = ic(L, i, j, k);
+                acc = acc + powabs(resid_mg(A, n, c, p, ip(L, i, j, k), sx, sy, sz), norm);
+            }
+    return acc;
+}
+
+double orc_compute_residual(const orc_grid *g, const double *A,
+                            const double *p, int norm) {
+    lvl_t L = grid_lvl(g);
+    return compute_residual_lvl(&L, A, p, norm);
+}
+
+/* calcResidual (modules.f90:3339-3384) without ExcludeBoundCellCalcRes */
+void orc_calc_residuals(const orc_grid *g, const double *A, const double *p,
+                        double out[3]) {
+    lvl_t L = grid_lvl(g);
+    long n = ncell(&L), sx = 1, sy = L.X, sz = (long)L.X * L.Y;
+    double s1 = 0.0, s2 = 0.0, smax = 0.0;
+    for (int k = 0; k < g->nz; ++k)
+        for (int j = 0; j < g->ny; ++j)
+            for (int i = 0; i < g->nx; ++i) {
+                long c = ic(&L, i, j, k), q = ip(&L, i, j, k);
+                double loc = fabs(-p[q] * A[6 * n + c] + A[0 * n + c] * p[q - sx] +
+                                  A[1 * n + c] * p[q + sx] + A[2 * n + c] * p[q - sy] +
+                                  A[3 * n + c] * p[q + sy] + A[4 * n + c] * p[q - sz] +
+                                  A[5 * n + c] * p[q + sz] + A[7 * n + c]);
+                s1 = s1 + loc;
+                s2 = s2 + loc * loc;
+                if (loc > smax) smax = loc;
+            }
+    out[0] = s1 / (double)g->nx / (double)g->ny / (double)g->nz;
+    out[1] = sqrt(s2 / (double)g->nx / (double)g->ny / (double)g->nz);
+    out[2] = smax;
+}
+
+/* ------------------------------------------------------------------ */
+/* ghost exchange, single rank: ghost_MG_x/y/z (modules.f90:1846-1971)  */
+/* ------------------------------------------------------------------ */
+static void apply_bc_lvl(const lvl_t *L, double *p) {
+    const int G = L->ng;
+    if (L->per[0])
+        for (int k = -G; k < L->n[2] + G; ++k)
+            for (int j = -G; j < L->n[1] + G; ++j) {
+                p[ip(L, -1, j, k)] = p[ip(L, L->n[0] - 1, j, k)];
+                p[ip(L, L->n[0], j, k)] = p[ip(L, 0, j, k)];
+            }
+    if (L->per[1])
+        for (int k = -G; k < L->n[2] + G; ++k)
+            for (int i = -G; i < L->n[0] + G; ++i) {
+                p[ip(L, i, -1, k)] = p[ip(L, i, L->n[1] - 1, k)];
+                p[ip(L, i, L->n[1], k)] = p[ip(L, i, 0, k)];
+            }
+    if (L->per[2])
+        for (int j = -G; j < L->n[1] + G; ++j)
+            for (int i = -G; i < L->n[0] + G; ++i) {
+                p[ip(L, i, j, -1)] = p[ip(L, i, j, L->n[2] - 1)];
+                p[ip(L, i, j, L->n[2])] = p[ip(L, i, j, 0)];
+            }
+}
+
+void orc_apply_bc(const orc_grid *g, double *p) {
+    lvl_t L = grid_lvl(g);
+    apply_bc_lvl(&L, p);
+}
+
+/* ------------------------------------------------------------------ */
+/* RedBlackRelax: newsolver.f90:209-247.  Pass 1 updates cells whose    */
+/* local index parity (i+j+k, 0-based) is even, then the halo is         */
+/* exchanged, then pass 2 updates the odd cells.                         */
+/* ------------------------------------------------------------------ */
+static void rb_relax_lvl(const lvl_t *L, const double *A, double *p, double beta) {
+    long n = ncell(L), sx = 1, sy = L->X, sz = (long)L->X * L->Y;
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int k = 0; k < L->n[2]; ++k)
+            for (int j = 0; j < L->n[1]; ++j)
+                for (int i = (j + k + pass) & 1; i < L->n[0]; i += 2) {
+                    long c = ic(L, i, j, k), q = ip(L, i, j, k);
+                    p[q] = (1.0 - beta) * p[q] +
+                           beta / A[6 * n + c] *
+                               (A[0 * n + c] * p[q - sx] + A[1 * n + c] * p[q + sx] +
+                                A[2 * n + c] * p[q - sy] + A[3 * n + c] * p[q + sy] +
+                                A[4 * n + c] * p[q - sz] + A[5 * n + c] * p[q + sz] +
+                                A[7 * n + c]);
+                }
+        if (pass == 0) apply_bc_lvl(L, p);
+    }
+}
+
+void orc_redblack_relax(const orc_grid *g, const double *A, double *p,
+                        double beta) {
+    lvl_t L = grid_lvl(g);
+    rb_relax_lvl(&L, A, p, beta);
+}
+
+/* get_residual, in place on slot 8: MGsolver.f90:84-104 */
+static void get_residual_lvl(const lvl_t *L, double *A, const double *p) {
+    long n = ncell(L), sx = 1, sy = L->X, sz = (long)L->X * L->Y;
+    for (int k = 0; k < L->n[2]; ++k)
+        for (int j = 0; j < L->n[1]; ++j)
+            for (int i = 0; i < L->n[0]; ++i) {
+                long c = ic(L, i, j, k);
+                A[7 * n + c] = resid_mg(A, n, c, p, ip(L, i, j, k), sx, sy, sz);
+            }
+}
+
+/* ------------------------------------------------------------------ */
+/* transfers: MGsolver.f90:128-251                                      */
+/* ------------------------------------------------------------------ */
+/* fill_coefficients (203-220): every slot of a coarse cell is the mean of
+ * its 8 children, summed in the reference's child order */
+void orc_fill_coefficients(const int nf[3], const double *Af, double *Ac) {
+    int nc[3] = {nf[0] / 2, nf[1] / 2, nf[2] / 2};
+    long n_f = (long)nf[0] * nf[1] * nf[2], n_c = (long)nc[0] * nc[1] * nc[2];
+#define FI(i, j, k) ((long)(i) + (long)nf[0] * ((long)(j) + (long)nf[1] * (k)))
+    for (int s = 0; s < 8; ++s)
+        for (int k = 0; k < nc[2]; ++k)
+            for (int j = 0; j < nc[1]; ++j)
+                for (int i = 0; i < nc[0]; ++i) {
+                    const double *a = Af + s * n_f;
+                    int I = 2 * i, J = 2 * j, K = 2 * k;
+                    double t = a[FI(I, J, K)] + a[FI(I, J + 1, K)] + a[FI(I, J, K + 1)] +
+                               a[FI(I, J + 1, K + 1)] + a[FI(I + 1, J, K)] + a[FI(I + 1, J + 1, K)] +
+                               a[FI(I + 1, J, K + 1)] + a[FI(I + 1, J + 1, K + 1)];
+                    Ac[s * n_c + i + (long)nc[0] * (j + (long)nc[1] * k)] = 0.125 * t;
+                }
+}
+
+/* coarse_from_fine (185-201): coarse slot 8 = sum over children, i1 outer,
+ * k1 inner, of 0.125 * child slot 8 */
+void orc_coarse_from_fine(const int nf[3], const double *Af, double *Ac) {
+    int nc[3] = {nf[0] / 2, nf[1] / 2, nf[2] / 2};
+    long n_f = (long)nf[0] * nf[1] * nf[2], n_c = (long)nc[0] * nc[1] * nc[2];
+    const double *a = Af + 7 * n_f;
+    double *o = Ac + 7 * n_c;
+    for (int i = 0; i < nc[0]; ++i)
+        for (int j = 0; j < nc[1]; ++j)
+            for (int k = 0; k < nc[2]; ++k) {
+                double t = 0.0;
+                for (int i1 = 0; i1 < 2; ++i1)
+                    for (int j1 = 0; j1 < 2; ++j1)
+                        for (int k1 = 0; k1 < 2; ++k1)
+                            t = t + a[FI(2 * i + i1, 2 * j + j1, 2 * k + k1)] * 0.125;
+                o[i + (long)nc[0] * (j + (long)nc[1] * k)] = t;
+            }
+}
+#undef FI
+
+/* coarse_fine_interp (128-183) */
+void orc_coarse_fine_interp(const int nf[3], int ng, int sym_mg, double *ef,
+                            const double *ec) {
+    int nc[3] = {nf[0] / 2, nf[1] / 2, nf[2] / 2};
+    int per0[3] = {0, 0, 0};
+    lvl_t F = lvl_make(nf, ng, per0), C = lvl_make(nc, ng, per0);
+    for (int i = 0; i < nc[0]; ++i)
+        for (int j = 0; j < nc[1]; ++j)
+            for (int k = 0; k < nc[2]; ++k) {
+                double pc = ec[ip(&C, i, j, k)];
+                if (sym_mg) {
+                    /* the children are weighted by |ef|; the reference sums in
+                     * the order (000)(100)(010)(110)(001)(101)(011)(111) */
+                    static const int o[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0},
+                                                {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+                    double tot = 0.0;
+                    for (int m = 0; m < 8; ++m)
+                        tot = tot + fabs(ef[ip(&F, 2 * i + o[m][0], 2 * j + o[m][1], 2 * k + o[m][2])]);
+                    tot = tot + 1e-15;
+                    for (int m = 0; m < 8; ++m) {
+                        long q = ip(&F, 2 * i + o[m][0], 2 * j + o[m][1], 2 * k + o[m][2]);
+                        ef[q] = fabs(ef[q]) / tot * pc;
+                    }
+                } else {
+                    for (int di = 0; di < 2; ++di)
+                        for (int dj = 0; dj < 2; ++dj)
+                            for (int dk = 0; dk < 2; ++dk)
+                                ef[ip(&F, 2 * i + di, 2 * j + dj, 2 * k + dk)] = pc;
+                }
+            }
+}
+
+/* ------------------------------------------------------------------ */
+/* NewSolverMG + MG_iteration + init_MG: MGsolver.f90:34-65, 253-405      */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    int lmg;
+    lvl_t *lv;       /* lv[1..lmg] */
+    double **data;   /* 8-slot level operators */
+    double **pmg, **emg;
+} mg_t;
+
+static void mg_free(mg_t *m) {
+    if (!m->lv) return;
+    for (int l = 1; l <= m->lmg; ++l) { free(m->data[l]); free(m->pmg[l]); free(m->emg[l]); }
+    free(m->lv); free(m->data); free(m->pmg); free(m->emg);
+    m->lv = NULL;
+}
+
+static int mg_init(mg_t *m, const orc_grid *g) {
+    int nd[3] = {g->nx, g->ny, g->nz};
+    for (int d = 0; d < 3; ++d)
+        if (nd[d] <= 0 || (nd[d] & (nd[d] - 1))) return ORC_ERR_BAD_SIZE;
+    int lmg = 1 << 30;
+    for (int d = 0; d < 3; ++d) {
+        int l = 0;
+        while ((1 << (l + 1)) <= nd[d]) ++l;
+        if (l < lmg) lmg = l;
+    }
+    if (lmg < 2) return ORC_ERR_BAD_SIZE;  /* the V-cycle touches level 2 */
+    m->lmg = lmg;
+    m->lv = calloc(lmg + 1, sizeof(lvl_t));
+    m->data = calloc(lmg + 1, sizeof(double *));
+    m->pmg = calloc(lmg + 1, sizeof(double *));
+    m->emg = calloc(lmg + 1, sizeof(double *));
+    if (!m->lv || !m->data || !m->pmg || !m->emg) return ORC_ERR_NOMEM;
+    int per[3] = {g->bc[0] == 1, g->bc[1] == 1, g->bc[2] == 1};
+    for (int l = lmg; l >= 1; --l) {
+        int nl[3] = {nd[0] >> (lmg - l), nd[1] >> (lmg - l), nd[2] >> (lmg - l)};
+        m->lv[l] = lvl_make(nl, g->ng, per);
+        m->data[l] = calloc((size_t)ncell(&m->lv[l]) * 8, sizeof(double));
+        m->pmg[l] = calloc((size_t)npad(&m->lv[l]), sizeof(double));
+        m->emg[l] = calloc((size_t)npad(&m->lv[l]), sizeof(double));
+        if (!m->data[l] || !m->pmg[l] || !m->emg[l]) return ORC_ERR_NOMEM;
+    }
+    return ORC_OK;
+}
+
+static void vzero(double *a, long n) { memset(a, 0, sizeof(double) * n); }
+static void vadd(double *a, const double *b, long n) {
+    for (long q = 0; q < n; ++q) a[q] = a[q] + b[q];
+}
+
+/* MG_iteration (253-320) */
+static void mg_iteration(mg_t *m, double beta, int nrelax, int sym_mg) {
+    int lmg = m->lmg;
+    for (int l = lmg; l >= 2; --l) {
+        lvl_t *L = &m->lv[l];
+        vzero(m->pmg[l], npad(L));
+        for (int r = 0; r < nrelax; ++r) {
+            rb_relax_lvl(L, m->data[l], m->pmg[l], beta);
+            apply_bc_lvl(L, m->pmg[l]);
+        }
+        get_residual_lvl(L, m->data[l], m->pmg[l]);
+        orc_coarse_from_fine(L->n, m->data[l], m->data[l - 1]);
+    }
+    rb_relax_lvl(&m->lv[2], m->data[2], m->pmg[2], beta);
+    apply_bc_lvl(&m->lv[2], m->pmg[2]);
+    for (int l = 3; l <= lmg; ++l) {
+        lvl_t *L = &m->lv[l];
+        long np = npad(L);
+        vzero(m->emg[l], np);
+        orc_coarse_fine_interp(L->n, L->ng, sym_mg, m->emg[l], m->pmg[l - 1]);
+        apply_bc_lvl(L, m->emg[l]);
+        get_residual_lvl(L, m->data[l], m->emg[l]);
+        vadd(m->pmg[l], m->emg[l], np);
+        vzero(m->emg[l], np);
+        for (int r = 0; r < nrelax; ++r) {
+            rb_relax_lvl(L, m->data[l], m->emg[l], beta);
+            apply_bc_lvl(L, m->emg[l]);
+        }
+        vadd(m->pmg[l], m->emg[l], np);
+        get_residual_lvl(L, m->data[l], m->emg[l]);
+    }
+}
+
+int orc_newsolver_mg(const orc_grid *g, const double *A, double *p,
+                     double maxError, double beta, int maxit, int norm,
+                     int nrelax, int sym_mg, int *it, double *tres2,
+                     double *hist) {
+    mg_t m;
+    memset(&m, 0, sizeof m);
+    int rc = mg_init(&m, g);
+    if (rc) { mg_free(&m); return rc; }
+    lvl_t *F = &m.lv[m.lmg];
+    long n = ncell(F), np = npad(F);
+    double N = (double)g->nx * (double)g->ny * (double)g->nz;
+    /* get_MGmatrix_coef (322-344); level 1 is filled but never used */
+    memcpy(m.data[m.lmg], A, sizeof(double) * 8 * n);
+    for (int l = m.lmg - 1; l >= 1; --l) orc_fill_coefficients(m.lv[l + 1].n, m.data[l + 1], m.data[l]);
+    /* initial residual into slot 8 (370-377) */
+    memcpy(m.pmg[m.lmg], p, sizeof(double) * np);
+    double res0 = compute_residual_lvl(F, m.data[m.lmg], m.pmg[m.lmg], norm);
+    get_residual_lvl(F, m.data[m.lmg], m.pmg[m.lmg]);
+    double t2 = res0 / N;
+    vzero(m.pmg[m.lmg], np);
+    int k;
+    for (k = 1; k <= maxit; ++k) {
+        mg_iteration(&m, beta, nrelax, sym_mg);
+        vadd(p, m.pmg[m.lmg], np);
+        double res = compute_residual_lvl(F, A, p, norm);
+        t2 = res / N;
+        if (norm == 2) t2 = sqrt(t2);
+        if (hist) hist[k - 1] = t2;
+        if (t2 < maxError) break;
+    }
+    *it = k;
+    *tres2 = t2;
+    mg_free(&m);
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* NewSolver_std: newsolver.f90:78-174                                  */
+/* ------------------------------------------------------------------ */
+int orc_newsolver_std(const orc_grid *g, const double *A, double *p,
+                      double maxError, double beta, int maxit, int norm,
+                      int *it, double *tres2, double *hist) {
+    lvl_t L = grid_lvl(g);
+    long n = ncell(&L), sx = 1, sy = L.X, sz = (long)L.X * L.Y;
+    double N = (double)g->nx * (double)g->ny * (double)g->nz;
+    double t2 = 0.0;
+    int k;
+    for (k = 1; k <= maxit; ++k) {
+        rb_relax_lvl(&L, A, p, beta);
+        /* the halo exchange overlaps the interior sum; the boundary shell is
+         * summed after it, so the total runs interior first, shell second */
+        apply_bc_lvl(&L, p);
+        double res2 = 0.0;
+        for (int pass = 0; pass < 2; ++pass)
+            for (int kk = 0; kk < g->nz; ++kk)
+                for (int j = 0; j < g->ny; ++j)
+                    for (int i = 0; i < g->nx; ++i) {
+                        int inner = i > 0 && i < g->nx - 1 && j > 0 && j < g->ny - 1 &&
+                                    kk > 0 && kk < g->nz - 1;
+                        if (inner != (pass == 0)) continue;
+                        long c = ic(&L, i, j, kk), q = ip(&L, i, j, kk);
+                        double r = -p[q] * A[6 * n + c] + A[0 * n + c] * p[q - sx] +
+                                   A[1 * n + c] * p[q + sx] + A[2 * n + c] * p[q - sy] +
+                                   A[3 * n + c] * p[q + sy] + A[4 * n + c] * p[q - sz] +
+                                   A[5 * n + c] * p[q + sz] + A[7 * n + c];
+                        res2 = res2 + powabs(r, norm);
+                    }
+        res2 = res2 / N;
+        /* catch_divergence (138-173) */
+        for (int kk = 0; kk < g->nz; ++kk)
+            for (int j = 0; j < g->ny; ++j)
+                for (int i = 0; i < g->nx; ++i) {
+                    long c = ic(&L, i, j, kk);
+                    if (p[ip(&L, i, j, kk)] != p[ip(&L, i, j, kk)]) return ORC_ERR_DIVERGED;
+                    for (int s = 0; s < 8; ++s)
+                        if (A[s * n + c] != A[s * n + c]) return ORC_ERR_DIVERGED;
+                }
+        if (res2 > 1e16 || res2 != res2) { *it = k; *tres2 = res2; return ORC_ERR_DIVERGED; }
+        t2 = res2;
+        if (norm == 2) t2 = sqrt(t2);
+        if (hist) hist[k - 1] = t2;
+        if (t2 < maxError) break;
+    }
+    *it = k;
+    *tres2 = t2;
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* Setup_testMG (modules.f90:3160-3186), get_MGtest_err (407-426)        */
+/* `real(8) :: pi=3.14159265359` is a default-real literal: the value   */
+/* stored is the single-precision rounding of pi.                        */
+/* ------------------------------------------------------------------ */
+static const float ORC_PI_SP = 3.14159265359f;
+
+void orc_setup_testmg(const orc_grid *g, const double *rho, double *A) {
+    lvl_t L = grid_lvl(g);
+    long n = ncell(&L);
+    double pi = (double)ORC_PI_SP;
+    axis_t ax, ay, az;
+    axis_alloc(&ax, g->nx, g->ng, g->xlength);
+    axis_alloc(&ay, g->ny, g->ng, g->ylength);
+    axis_alloc(&az, g->nz, g->ng, g->zlength);
+    const int G = g->ng;
+    for (int k = 0; k < g->nz; ++k)
+        for (int j = 0; j < g->ny; ++j)
+            for (int i = 0; i < g->nx; ++i) {
+                long c = ic(&L, i, j, k), q = ip(&L, i, j, k);
+                int I = i + G, J = j + G, K = k + G;
+                double r0 = rho[q];
+                double a1 = 2.0 / (ax.dx[I] * ax.dxh[I - 1] * (rho[ip(&L, i - 1, j, k)] + r0));
+                double a2 = 2.0 / (ax.dx[I] * ax.dxh[I] * (rho[ip(&L, i + 1, j, k)] + r0));
+                double a3 = 2.0 / (ay.dx[J] * ay.dxh[J - 1] * (rho[ip(&L, i, j - 1, k)] + r0));
+                double a4 = 2.0 / (ay.dx[J] * ay.dxh[J] * (rho[ip(&L, i, j + 1, k)] + r0));
+                double a5 = 2.0 / (az.dx[K] * az.dxh[K - 1] * (rho[ip(&L, i, j, k - 1)] + r0));
+                double a6 = 2.0 / (az.dx[K] * az.dxh[K] * (rho[ip(&L, i, j, k + 1)] + r0));
+                A[c] = a1; A[n + c] = a2; A[2 * n + c] = a3;
+                A[3 * n + c] = a4; A[4 * n + c] = a5; A[5 * n + c] = a6;
+                A[6 * n + c] = ((((a1 + a2) + a3) + a4) + a5) + a6;
+                A[7 * n + c] = 3.0 * (2.0 * pi) * (2.0 * pi) * sin(2.0 * pi * ax.x[I]) *
+                               sin(2.0 * pi * ay.x[J]) * sin(2.0 * pi * az.x[K]);
+            }
+    axis_free(&ax); axis_free(&ay); axis_free(&az);
+}
+
+double orc_get_mgtest_err(const orc_grid *g, const double *p) {
+    lvl_t L = grid_lvl(g);
+    double pi = (double)ORC_PI_SP;
+    axis_t ax, ay, az;
+    axis_alloc(&ax, g->nx, g->ng, g->xlength);
+    axis_alloc(&ay, g->ny, g->ng, g->ylength);
+    axis_alloc(&az, g->nz, g->ng, g->zlength);
+    const int G = g->ng;
+    double res = 0.0, ref = 0.0;
+    for (int k = 0; k < g->nz; ++k)
+        for (int j = 0; j < g->ny; ++j)
+            for (int i = 0; i < g->nx; ++i) {
+                double s = sin(2.0 * pi * ax.x[i + G]) * sin(2.0 * pi * ay.x[j + G]) *
+                           sin(2.0 * pi * az.x[k + G]);
+                res = res + fabs(p[ip(&L, i, j, k)] - s);
+                ref = ref + fabs(s);
+            }
+    axis_free(&ax); axis_free(&ay); axis_free(&az);
+    return res / ref;
+}
+
+/* ------------------------------------------------------------------ */
+/* project_velocity, staggered branch: paris.f90:898-908                 */
+/* ------------------------------------------------------------------ */
+void orc_project_velocity(const orc_grid *g, double *u, double *v, double *w,
+                          const double *umask, const double *vmask,
+                          const double *wmask, const double *rho,
+                          const double *p, double dt) {
+    lvl_t L = grid_lvl(g);
+    axis_t ax, ay, az;
+    axis_alloc(&ax, g->nx, g->ng, g->xlength);
+    axis_alloc(&ay, g->ny, g->ng, g->ylength);
+    axis_alloc(&az, g->nz, g->ng, g->zlength);
+    const int G = g->ng;
+    /* ieu/jev/kew: the last face is skipped unless the direction is periodic */
+    int ieu = g->nx - ((g->bc[0] != 1) ? 1 : 0);
+    int jev = g->ny - ((g->bc[1] != 1) ? 1 : 0);
+    int kew = g->nz - ((g->bc[2] != 1) ? 1 : 0);
+    for (int k = 0; k < g->nz; ++k)
+        for (int j = 0; j < g->ny; ++j)
+            for (int i = 0; i < ieu; ++i) {
+                long q = ip(&L, i, j, k), e = ip(&L, i + 1, j, k);
+                u[q] = u[q] - dt * (2.0 * umask[q] / ax.dxh[i + G]) * (p[e] - p[q]) / (rho[e] + rho[q]);
+            }
+    for (int k = 0; k < g->nz; ++k)
+        for (int j = 0; j < jev; ++j)
+            for (int i = 0; i < g->nx; ++i) {
+                long q = ip(&L, i, j, k), e = ip(&L, i, j + 1, k);
+                v[q] = v[q] - dt * (2.0 * vmask[q] / ay.dxh[j + G]) * (p[e] - p[q]) / (rho[e] + rho[q]);
+            }
+    for (int k = 0; k < kew; ++k)
+        for (int j = 0; j < g->ny; ++j)
+            for (int i = 0; i < g->nx; ++i) {
+                long q = ip(&L, i, j, k), e = ip(&L, i, j, k + 1);
+                w[q] = w[q] - dt * (2.0 * wmask[q] / az.dxh[k + G]) * (p[e] - p[q]) / (rho[e] + rho[q]);
+            }
+    axis_free(&ax); axis_free(&ay); axis_free(&az);
+}
+
+/* ------------------------------------------------------------------ */
+/* Independent exact solver: Jacobi-PCG on the 8-slot operator            */
+/* M p = A7 p - sum A_nb p_nb, with wall ghosts held at their input value */
+/* (their contribution moves to the right-hand side) and periodic ghosts  */
+/* wrapped.  Not a reference algorithm: it pins converged solutions.      */
+/* ------------------------------------------------------------------ */
+static void op_apply(const lvl_t *L, const double *A, double *x, double *y) {
+    /* y = M x on the interior; x is a padded vector whose wall ghosts are 0 */
+    apply_bc_lvl(L, x);
+    long n = ncell(L), sx = 1, sy = L->X, sz = (long)L->X * L->Y;
+    for (int k = 0; k < L->n[2]; ++k)
+        for (int j = 0; j < L->n[1]; ++j)
+            for (int i = 0; i < L->n[0]; ++i) {
+                long c = ic(L, i, j, k), q = ip(L, i, j, k);
+                y[c] = A[6 * n + c] * x[q] - (A[0 * n + c] * x[q - sx] + A[1 * n + c] * x[q + sx] +
+                                              A[2 * n + c] * x[q - sy] + A[3 * n + c] * x[q + sy] +
+                                              A[4 * n + c] * x[q - sz] + A[5 * n + c] * x[q + sz]);
+            }
+}
+
+int orc_pcg_solve(const orc_grid *g, const double *A, double *p, double rtol,
+                  int maxit, double *relres) {
+    lvl_t L = grid_lvl(g);
+    long n = ncell(&L), np = npad(&L);
+    double *r = malloc(sizeof(double) * n), *z = malloc(sizeof(double) * n);
+    double *q = malloc(sizeof(double) * n), *d = calloc(np, sizeof(double));
+    double *pc = malloc(sizeof(double) * np);
+    if (!r || !z || !q || !d || !pc) { free(r); free(z); free(q); free(d); free(pc); return -1; }
+    /* r = b - M p using the residual of the reference (ghosts included) */
+    memcpy(pc, p, sizeof(double) * np);
+    apply_bc_lvl(&L, pc);
+    orc_residual_field(g, A, pc, r);
+    double bn = 0.0;
+    for (long c = 0; c < n; ++c) bn += A[7 * n + c] * A[7 * n + c];
+    bn = sqrt(bn);
+    if (bn == 0.0) bn = 1.0;
+    double rz = 0.0, rn = 0.0;
+    for (long c = 0; c < n; ++c) { z[c] = r[c] / A[6 * n + c]; rz += r[c] * z[c]; rn += r[c] * r[c]; }
+    int it = 0;
+    for (int k = 0; k < L.n[2]; ++k)
+        for (int j = 0; j < L.n[1]; ++j)
+            for (int i = 0; i < L.n[0]; ++i) d[ip(&L, i, j, k)] = z[ic(&L, i, j, k)];
+    while (sqrt(rn) / bn > rtol && it < maxit) {
+        op_apply(&L, A, d, q);
+        double dq = 0.0;
+        for (int k = 0; k < L.n[2]; ++k)
+            for (int j = 0; j < L.n[1]; ++j)
+                for (int i = 0; i < L.n[0]; ++i) dq += d[ip(&L, i, j, k)] * q[ic(&L, i, j, k)];
+        double alpha = rz / dq;
+        rn = 0.0;
+        for (int k = 0; k < L.n[2]; ++k)
+            for (int j = 0; j < L.n[1]; ++j)
+                for (int i = 0; i < L.n[0]; ++i) {
+                    long c = ic(&L, i, j, k), qq = ip(&L, i, j, k);
+                    p[qq] += alpha * d[qq];
+                    r[c] -= alpha * q[c];
+                    rn += r[c] * r[c];
+                }
+        double rz2 = 0.0;
+        for (long c = 0; c < n; ++c) { z[c] = r[c] / A[6 * n + c]; rz2 += r[c] * z[c]; }
+        double beta = rz2 / rz;
+        rz = rz2;
+        for (int k = 0; k < L.n[2]; ++k)
+            for (int j = 0; j < L.n[1]; ++j)
+                for (int i = 0; i < L.n[0]; ++i) {
+                    long qq = ip(&L, i, j, k);
+                    d[qq] = z[ic(&L, i, j, k)] + beta * d[qq];
+                }
+        ++it;
+        /* recompute the true residual now and then to avoid drift */
+        if (it % 50 == 0) {
+            memcpy(pc, p, sizeof(double) * np);
+            apply_bc_lvl(&L, pc);
+            orc_residual_field(g, A, pc, r);
+            rn = 0.0;
+            for (long c = 0; c < n; ++c) rn += r[c] * r[c];
+        }
+    }
+    apply_bc_lvl(&L, p);
+    memcpy(pc, p, sizeof(double) * np);
+    orc_residual_field(g, A, pc, r);
+    rn = 0.0;
+    for (long c = 0; c < n; ++c) rn += r[c] * r[c];
+    if (relres) *relres = sqrt(rn) / bn;
+    free(r); free(z); free(q); free(d); free(pc);
+    return it;
+}

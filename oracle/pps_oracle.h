@@ -1,0 +1,127 @@
+/*
+ * pps_oracle.h -- TEST INFRASTRUCTURE, NOT PRODUCT CODE.
+ *
+ * CPU restatement of the PARIS pressure-Poisson path in plain C (FP64).
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it, and only as the checker or the timed CPU baseline.  The product
+ * library (libpps) never links or calls it.
+ *
+ * Conventions (identical to the reference, newsolver.f90:30-46):
+ *   A  : 8 contiguous planes of nx*ny*nz doubles, x fastest (Fortran order);
+ *        slot s (1..8) of cell (i,j,k) (0-based interior indices) is
+ *        A[(s-1)*n + i + nx*(j + ny*k)].  Slots 1..6 = x-,x+,y-,y+,z-,z+
+ *        couplings, 7 = diagonal, 8 = right-hand side.
+ *   p  : (nx+2ng)*(ny+2ng)*(nz+2ng) doubles with ng ghost layers, x fastest.
+ *   equation: A7*p = A1*p(i-1)+A2*p(i+1)+A3*p(j-1)+A4*p(j+1)+A5*p(k-1)+A6*p(k+1)+A8
+ *   bc[6]: bdry_cond(1..6) = x-, y-, z-, x+, y+, z+ ; 0 wall, 1 periodic,
+ *          2 shear, 3 velocity inflow, 4 velocity outflow (modules.f90:870-877).
+ */
+#ifndef PPS_ORACLE_H
+#define PPS_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_grid {
+    int nx, ny, nz;       /* interior cells (one rank owns the whole box) */
+    int ng;               /* ghost layers on p (reference Ng, default 2) */
+    int bc[6];            /* bdry_cond */
+    double xlength, ylength, zlength;
+} orc_grid;
+
+enum {
+    ORC_OK = 0,
+    ORC_ERR_NAN_OR_TINY_DIAG = 1,   /* check_poisson_setup: NaN or A7 < 1e-55 */
+    ORC_ERR_VOLUME_SOURCE = 2,      /* check_poisson_setup: mean(A8) > 1e-10 */
+    ORC_ERR_NOT_IN_KERNEL = 3,      /* check_poisson_setup: row sum != 0 */
+    ORC_ERR_BAD_SIZE = 4,           /* init_MG: not a power of two */
+    ORC_ERR_DIVERGED = 5,           /* catch_divergence: residual > 1e16 or NaN */
+    ORC_ERR_SOLID = 6,              /* check_and_debug_Poisson inconsistencies */
+    ORC_ERR_UNSUPPORTED = 7,
+    ORC_ERR_NOMEM = 8
+};
+
+/* Uniform-grid coordinates exactly as paris.f90:2959-3102 builds them
+ * (xform = 0).  Arrays have N+2ng entries; entry q holds Fortran index q+1. */
+void orc_coords(int N, int ng, double length, double *x, double *xh,
+                double *dx, double *dxh);
+
+/* SetPressureBC (modules.f90:916-946) applied to all-ones masks. */
+void orc_pressure_masks(const orc_grid *g, double *umask, double *vmask,
+                        double *wmask);
+
+/* SetupPoisson + Poisson_BCs(codes 0..4) + check_and_debug_Poisson
+ * (modules.f90:2722-2953).  Fields carry ng ghosts.  Returns ORC_* code. */
+int orc_setup_poisson(const orc_grid *g, const double *u, const double *v,
+                      const double *w, const double *umask,
+                      const double *vmask, const double *wmask,
+                      const double *rho, double dt, double volume_source,
+                      double *A);
+
+/* check_poisson_setup (newsolver.f90:275-344); rowsum (optional, n cells)
+ * receives sum(A1..6) - A7 with A7 := 0 in solid cells. */
+int orc_check_poisson_setup(const orc_grid *g, const double *A,
+                            const double *umask, const double *vmask,
+                            const double *wmask, double *rowsum);
+
+/* Residual r = A8 - A7 p + sum A_nb p_nb (MGsolver.f90:97-100) into r (n). */
+void orc_residual_field(const orc_grid *g, const double *A, const double *p,
+                        double *r);
+/* compute_residual (MGsolver.f90:106-126): sum |r|^norm, reference order. */
+double orc_compute_residual(const orc_grid *g, const double *A,
+                            const double *p, int norm);
+/* calcResiduals (modules.f90:3313-3384): out = {L1, L2, Linf}. */
+void orc_calc_residuals(const orc_grid *g, const double *A, const double *p,
+                        double out[3]);
+
+/* Periodic one-layer ghost exchange of one level (ghost_MG_x/y/z with one
+ * rank; wall directions untouched). */
+void orc_apply_bc(const orc_grid *g, double *p);
+/* RedBlackRelax (newsolver.f90:209-247) on the fine level, one sweep. */
+void orc_redblack_relax(const orc_grid *g, const double *A, double *p,
+                        double beta);
+
+/* MG transfer operators on bare level arrays (MGsolver.f90:128-251).
+ * nf = fine interior dims; Af/Ac: 8-slot level arrays without ghosts;
+ * ef/ec: level vectors with ng ghosts. */
+void orc_fill_coefficients(const int nf[3], const double *Af, double *Ac);
+void orc_coarse_from_fine(const int nf[3], const double *Af, double *Ac);
+void orc_coarse_fine_interp(const int nf[3], int ng, int sym_mg, double *ef,
+                            const double *ec);
+
+/* NewSolverMG (MGsolver.f90:346-405) including the module state of
+ * init_MG (34-65).  hist (optional, maxit entries) receives tres2 after
+ * each iteration.  *it follows the reference: maxit+1 when not converged.
+ * Returns ORC_OK or ORC_ERR_BAD_SIZE. */
+int orc_newsolver_mg(const orc_grid *g, const double *A, double *p,
+                     double maxError, double beta, int maxit, int norm,
+                     int nrelax, int sym_mg, int *it, double *tres2,
+                     double *hist);
+/* NewSolver_std (newsolver.f90:78-174), plain RB-SOR to tolerance. */
+int orc_newsolver_std(const orc_grid *g, const double *A, double *p,
+                      double maxError, double beta, int maxit, int norm,
+                      int *it, double *tres2, double *hist);
+
+/* Setup_testMG (modules.f90:3160-3186) and get_MGtest_err
+ * (MGsolver.f90:407-426); both use the reference's single-precision pi. */
+void orc_setup_testmg(const orc_grid *g, const double *rho, double *A);
+double orc_get_mgtest_err(const orc_grid *g, const double *p);
+
+/* project_velocity, plain staggered branch (paris.f90:898-908). */
+void orc_project_velocity(const orc_grid *g, double *u, double *v, double *w,
+                          const double *umask, const double *vmask,
+                          const double *wmask, const double *rho,
+                          const double *p, double dt);
+
+/* Independent exact solver (not a reference algorithm): Jacobi-preconditioned
+ * CG on the same 8-slot system, ghosts handled as the reference does
+ * (periodic wrap, wall ghosts fixed).  Used to pin converged solutions.
+ * Runs until ||r||2/||b||2 <= rtol or maxit.  Returns iterations done. */
+int orc_pcg_solve(const orc_grid *g, const double *A, double *p, double rtol,
+                  int maxit, double *relres);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,494 @@
+"""paris-git_amd -- MI355X pressure-Poisson solver for PARIS.
+
+Python mirror of the reference's linear-solver interface over the C ABI of
+``libpps.so`` (include/pps.h):
+
+* :meth:`Solver.new_solver` / :meth:`Solver.new_solver_mg` follow
+  ``NewSolver(A,p,maxError,beta,maxit,it,ierr,norm)`` (newsolver.f90:34-57)
+  and ``NewSolverMG(...,tres2)`` (MGsolver.f90:346-405) on host arrays;
+* :meth:`Solver.solve` is the device-resident form on torch tensors;
+* the producer / consumer of the system (SetupPoisson, SetPressureBC,
+  check_poisson_setup, calcResiduals, project_velocity) are free functions;
+* :class:`PressureSolve` is a ``torch.autograd.Function`` whose backward is
+  the adjoint solve plus the fused face kernel.
+
+Array layout is the reference's Fortran order: ``A`` holds 8 planes of
+``nx*ny*nz`` doubles (torch shape ``(8, nz, ny, nx)``), ``p`` carries ``ng``
+ghost layers (torch shape ``(nz+2ng, ny+2ng, nx+2ng)``).
+
+Nothing here computes on the CPU: every operation calls libpps, and a
+missing library raises instead of falling back.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpps.so")
+
+# enum pps_status
+OK = 0
+ERR_NAN_OR_TINY_DIAG = 1
+ERR_VOLUME_SOURCE = 2
+ERR_NOT_IN_KERNEL = 3
+ERR_BAD_SIZE = 4
+ERR_DIVERGED = 5
+ERR_NOT_CONVERGED = 6
+ERR_NONSYMMETRIC = 7
+ERR_BAD_ARG = 8
+ERR_HIP = 9
+ERR_NO_DEVICE = 10
+ERR_UNSUPPORTED = 11
+
+# enum pps_method / pps_outer
+MG_REFERENCE = 0
+MG_FP64 = 1
+MG_MIXED = 2
+SOR = 3
+OUTER_CG = 0
+OUTER_REFINE = 1
+
+METHODS = {"reference": MG_REFERENCE, "fp64": MG_FP64, "mixed": MG_MIXED, "sor": SOR}
+
+# boundary codes (modules.f90:870-877)
+WALL, PERIODIC, SHEAR, VEL_IN, VEL_OUT = 0, 1, 2, 3, 4
+
+
+class PPSError(RuntimeError):
+    """A libpps call returned a non-zero status."""
+
+    def __init__(self, code, what=""):
+        self.code = code
+        msg = _lib().pps_status_string(code).decode() if _LIBH is not None else str(code)
+        super().__init__(f"{what}: {msg} (status {code})" if what else f"{msg} (status {code})")
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("ng", ctypes.c_int), ("bdry_cond", ctypes.c_int * 6),
+                ("xlength", ctypes.c_double), ("ylength", ctypes.c_double),
+                ("zlength", ctypes.c_double)]
+
+    @classmethod
+    def make(cls, nx, ny=None, nz=None, ng=2, bdry_cond=(0, 0, 0, 0, 0, 0),
+             lengths=(1.0, 1.0, 1.0)):
+        ny = nx if ny is None else ny
+        nz = nx if nz is None else nz
+        g = cls()
+        g.nx, g.ny, g.nz, g.ng = nx, ny, nz, ng
+        for q in range(6):
+            g.bdry_cond[q] = int(bdry_cond[q])
+        g.xlength, g.ylength, g.zlength = (float(v) for v in lengths)
+        return g
+
+    @property
+    def shape_A(self):
+        return (8, self.nz, self.ny, self.nx)
+
+    @property
+    def shape_p(self):
+        return (self.nz + 2 * self.ng, self.ny + 2 * self.ng, self.nx + 2 * self.ng)
+
+    @property
+    def cells(self):
+        return self.nx * self.ny * self.nz
+
+    def interior(self, p):
+        """View of the interior cells of a padded array (numpy or torch)."""
+        g = self.ng
+        return p[g:g + self.nz, g:g + self.ny, g:g + self.nx]
+
+    def periodic(self):
+        return tuple(int(self.bdry_cond[d] == 1) for d in range(3))
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("method", ctypes.c_int), ("outer", ctypes.c_int), ("nrelax", ctypes.c_int),
+                ("sym_mg", ctypes.c_int), ("nu_pre", ctypes.c_int), ("nu_post", ctypes.c_int),
+                ("check_setup", ctypes.c_int), ("multigrid", ctypes.c_int)]
+
+
+_LIBH = None
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_IP = ctypes.POINTER(ctypes.c_int)
+_DP = ctypes.POINTER(ctypes.c_double)
+_GP = ctypes.POINTER(Grid)
+
+
+def _lib():
+    """Load libpps.so; raise if it has not been built."""
+    global _LIBH
+    if _LIBH is not None:
+        return _LIBH
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libpps.so is missing at {LIB_PATH}: build it first "
+                           "(python -c 'import __graft_entry__ as g; g.build()')")
+    L = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "pps_default_options": (_I, [ctypes.POINTER(Options)]),
+        "pps_init": (_I, [_GP, ctypes.POINTER(Options), ctypes.POINTER(_P)]),
+        "pps_finalize": (_I, [_P]),
+        "pps_set_options": (_I, [_P, ctypes.POINTER(Options)]),
+        "pps_new_solver": (_I, [_P, _P, _P, _D, _D, _I, _IP, _IP, _I]),
+        "pps_new_solver_mg": (_I, [_P, _P, _P, _D, _D, _I, _IP, _IP, _I, _DP]),
+        "pps_solve_device": (_I, [_P, _P, _P, _D, _D, _I, _I, _IP, _DP, _P]),
+        "pps_last_history": (_I, [_P, _DP, _I]),
+        "pps_last_timing": (_I, [_P, _DP, _DP]),
+        "pps_adjoint_device": (_I, [_P, _P, _P, _P, _D, _I, _IP, _P]),
+        "pps_face_gradient_device": (_I, [_GP, _P, _P, _P, _P, _P, _P, _P, _P]),
+        "pps_pressure_masks_device": (_I, [_GP, _P, _P, _P, _P]),
+        "pps_setup_poisson_device": (_I, [_GP, _P, _P, _P, _P, _P, _P, _P, _D, _D, _P, _P]),
+        "pps_check_setup_device": (_I, [_GP, _P, _P]),
+        "pps_remove_rhs_mean_device": (_I, [_GP, _P, _P]),
+        "pps_calc_residuals_device": (_I, [_GP, _P, _P, _DP, _P]),
+        "pps_project_velocity_device": (_I, [_GP, _P, _P, _P, _P, _P, _P, _P, _P, _D, _P]),
+        "pps_synth_fields_device": (_I, [_GP, _P, _I, _D, _D, _D, ctypes.c_ulonglong,
+                                         _P, _P, _P, _P, _P]),
+        "pps_synth_drops": (_I, [_GP, _I, _D, _D, ctypes.c_ulonglong, _P]),
+        "pps_coords": (_I, [_I, _I, _D, _P, _P, _P, _P]),
+        "pps_status_string": (ctypes.c_char_p, [_I]),
+        "pps_faces_to_A_device": (_I, [_GP, _P, _P, _P, _P, _P, _P, _P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIBH = L
+    return L
+
+
+def exported_symbols():
+    """Names of the C-ABI functions declared in include/pps.h."""
+    return [
+        "pps_default_options", "pps_init", "pps_finalize", "pps_set_options",
+        "pps_new_solver", "pps_new_solver_mg", "pps_solve_device", "pps_last_history",
+        "pps_last_timing", "pps_adjoint_device", "pps_face_gradient_device",
+        "pps_pressure_masks_device", "pps_setup_poisson_device", "pps_check_setup_device",
+        "pps_remove_rhs_mean_device", "pps_calc_residuals_device",
+        "pps_project_velocity_device", "pps_synth_fields_device", "pps_synth_drops",
+        "pps_coords", "pps_status_string", "pps_faces_to_A_device",
+    ]
+
+
+def _check(rc, what, allow=(OK,)):
+    if rc not in allow:
+        raise PPSError(rc, what)
+    return rc
+
+
+def _ptr(t):
+    """Device (or host) address of a contiguous torch tensor / numpy array."""
+    if isinstance(t, np.ndarray):
+        if not (t.flags["C_CONTIGUOUS"] or t.flags["F_CONTIGUOUS"]):
+            raise ValueError("array must be contiguous")
+        if t.dtype != np.float64:
+            raise TypeError("array must be float64")
+        return ctypes.c_void_p(t.ctypes.data)
+    if not t.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _need_gpu(*ts):
+    for t in ts:
+        if isinstance(t, np.ndarray) or not t.is_cuda or t.dtype.is_floating_point is False:
+            raise TypeError("expected a CUDA (HIP) float64 tensor")
+
+
+def default_options():
+    o = Options()
+    _check(_lib().pps_default_options(ctypes.byref(o)), "pps_default_options")
+    return o
+
+
+class Solver:
+    """A solver handle (init_MG / finalize_MG, MGsolver.f90:34-82)."""
+
+    def __init__(self, grid, method="fp64", outer=OUTER_CG, nrelax=3, sym_mg=0, nu_pre=2,
+                 nu_post=2, check_setup=0, multigrid=1):
+        self.grid = grid
+        o = default_options()
+        o.method = METHODS[method] if isinstance(method, str) else int(method)
+        o.outer, o.nrelax, o.sym_mg = int(outer), int(nrelax), int(sym_mg)
+        o.nu_pre, o.nu_post = int(nu_pre), int(nu_post)
+        o.check_setup, o.multigrid = int(check_setup), int(multigrid)
+        self.options = o
+        h = ctypes.c_void_p()
+        _check(_lib().pps_init(ctypes.byref(grid), ctypes.byref(o), ctypes.byref(h)), "pps_init")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib().pps_finalize(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_method(self, method):
+        self.options.method = METHODS[method] if isinstance(method, str) else int(method)
+        _check(_lib().pps_set_options(self._h, ctypes.byref(self.options)), "pps_set_options")
+
+    # -- reference signatures on host arrays ---------------------------
+    def new_solver(self, A, p, maxError, beta=1.2, maxit=50, norm=2):
+        """NewSolver: p (host, float64, padded) is updated in place.
+        Returns (it, ierr); it == maxit+1 when not converged."""
+        it, ierr = ctypes.c_int(), ctypes.c_int()
+        rc = _lib().pps_new_solver(self._h, _ptr(A), _ptr(p), maxError, beta, maxit,
+                                   ctypes.byref(it), ctypes.byref(ierr), norm)
+        _check(rc, "NewSolver", allow=(OK, ERR_NOT_CONVERGED))
+        return it.value, ierr.value
+
+    def new_solver_mg(self, A, p, maxError, beta=1.2, maxit=50, norm=2):
+        """NewSolverMG: returns (it, ierr, tres2)."""
+        it, ierr, t2 = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        rc = _lib().pps_new_solver_mg(self._h, _ptr(A), _ptr(p), maxError, beta, maxit,
+                                      ctypes.byref(it), ctypes.byref(ierr), norm,
+                                      ctypes.byref(t2))
+        _check(rc, "NewSolverMG", allow=(OK, ERR_NOT_CONVERGED))
+        return it.value, ierr.value, t2.value
+
+    # -- device-resident form ---------------------------------------------
+    def solve(self, A, p, maxError, beta=1.2, maxit=50, norm=2, stream=None, allow_maxit=True):
+        """Solve in place on device tensors; returns (it, tres2)."""
+        _need_gpu(A, p)
+        it, t2 = ctypes.c_int(), ctypes.c_double()
+        rc = _lib().pps_solve_device(self._h, _ptr(A), _ptr(p), maxError, beta, maxit, norm,
+                                     ctypes.byref(it), ctypes.byref(t2), _stream(stream))
+        _check(rc, "pps_solve_device",
+               allow=(OK, ERR_NOT_CONVERGED) if allow_maxit else (OK,))
+        return it.value, t2.value
+
+    def adjoint(self, A, g, lam, rtol=1e-12, maxit=100, stream=None):
+        """Solve M lam = g (g, lam padded device tensors); returns iterations."""
+        _need_gpu(A, g, lam)
+        it = ctypes.c_int()
+        rc = _lib().pps_adjoint_device(self._h, _ptr(A), _ptr(g), _ptr(lam), rtol, maxit,
+                                       ctypes.byref(it), _stream(stream))
+        _check(rc, "pps_adjoint_device")
+        return it.value
+
+    def history(self, cap=100000):
+        buf = (ctypes.c_double * cap)()
+        n = _lib().pps_last_history(self._h, buf, cap)
+        return list(buf[:max(n, 0)])
+
+    def timing(self):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        _lib().pps_last_timing(self._h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+
+# ---------------------------------------------------------------------------
+# producer / consumer of the system, synthetic inputs
+# ---------------------------------------------------------------------------
+def coords(N, ng, length):
+    """x, xh, dx, dxh as paris.f90:2959-3102 builds them (numpy)."""
+    arrs = [np.empty(N + 2 * ng) for _ in range(4)]
+    _check(_lib().pps_coords(N, ng, float(length), *[_ptr(a) for a in arrs]), "pps_coords")
+    return tuple(arrs)
+
+
+def synth_drops(grid, ndrop, rmin, rmax, seed):
+    out = np.zeros((max(ndrop, 1), 4))
+    _check(_lib().pps_synth_drops(ctypes.byref(grid), ndrop, rmin, rmax, seed, _ptr(out)),
+           "pps_synth_drops")
+    return out[:ndrop]
+
+
+def synth_fields(grid, drops, rho1, rho2, width=1.5, seed=1, device="cuda"):
+    """rho, u, v, w (padded device tensors) of a droplet field."""
+    import torch
+    drops = np.ascontiguousarray(np.asarray(drops, dtype=np.float64).reshape(-1, 4))
+    rho, u, v, w = (torch.empty(grid.shape_p, dtype=torch.float64, device=device) for _ in range(4))
+    dp = _ptr(drops) if len(drops) else ctypes.c_void_p(0)
+    _check(_lib().pps_synth_fields_device(ctypes.byref(grid), dp, len(drops), rho1, rho2, width,
+                                          seed, _ptr(rho), _ptr(u), _ptr(v), _ptr(w), _stream()),
+           "pps_synth_fields_device")
+    return rho, u, v, w
+
+
+def pressure_masks(grid, device="cuda"):
+    import torch
+    um, vm, wm = (torch.empty(grid.shape_p, dtype=torch.float64, device=device) for _ in range(3))
+    _check(_lib().pps_pressure_masks_device(ctypes.byref(grid), _ptr(um), _ptr(vm), _ptr(wm),
+                                            _stream()), "pps_pressure_masks_device")
+    return um, vm, wm
+
+
+def setup_poisson(grid, u, v, w, um, vm, wm, rho, dt=1.0, volume_source=0.0):
+    """SetupPoisson + Poisson_BCs: the 8-slot system (device tensor)."""
+    import torch
+    A = torch.empty(grid.shape_A, dtype=torch.float64, device=rho.device)
+    _check(_lib().pps_setup_poisson_device(ctypes.byref(grid), _ptr(u), _ptr(v), _ptr(w),
+                                           _ptr(um), _ptr(vm), _ptr(wm), _ptr(rho), dt,
+                                           volume_source, _ptr(A), _stream()),
+           "pps_setup_poisson_device")
+    return A
+
+
+def remove_rhs_mean(grid, A):
+    _check(_lib().pps_remove_rhs_mean_device(ctypes.byref(grid), _ptr(A), _stream()),
+           "pps_remove_rhs_mean_device")
+    return A
+
+
+def check_setup(grid, A):
+    """check_poisson_setup: returns the status code (0 = well posed)."""
+    return _lib().pps_check_setup_device(ctypes.byref(grid), _ptr(A), _stream())
+
+
+def calc_residuals(grid, A, p):
+    """calcResiduals: (L1, L2, Linf)."""
+    out = (ctypes.c_double * 3)()
+    _check(_lib().pps_calc_residuals_device(ctypes.byref(grid), _ptr(A), _ptr(p), out, _stream()),
+           "pps_calc_residuals_device")
+    return tuple(out)
+
+
+def project_velocity(grid, u, v, w, um, vm, wm, rho, p, dt=1.0):
+    _check(_lib().pps_project_velocity_device(ctypes.byref(grid), _ptr(u), _ptr(v), _ptr(w),
+                                              _ptr(um), _ptr(vm), _ptr(wm), _ptr(rho), _ptr(p),
+                                              dt, _stream()), "pps_project_velocity_device")
+
+
+def face_weights(grid, dt=1.0, device="cuda"):
+    """Geometric weights dt/(dx*dxh) of the +x/+y/+z faces (per plane)."""
+    import torch
+    out = []
+    for N, L in ((grid.nx, grid.xlength), (grid.ny, grid.ylength), (grid.nz, grid.zlength)):
+        x, xh, dx, dxh = coords(N, grid.ng, L)
+        sl = slice(grid.ng, grid.ng + N)
+        out.append(torch.tensor(dt / (dx[sl] * dxh[sl]), dtype=torch.float64, device=device))
+    return tuple(out)
+
+
+def face_gradient(grid, A, p, lam, weights):
+    """dJ/dc_f of the +x, +y, +z face of every cell: (3, nz, ny, nx)."""
+    import torch
+    g = torch.empty((3, grid.nz, grid.ny, grid.nx), dtype=torch.float64, device=A.device)
+    wx, wy, wz = weights
+    _check(_lib().pps_face_gradient_device(ctypes.byref(grid), _ptr(A), _ptr(p), _ptr(lam),
+                                           _ptr(wx), _ptr(wy), _ptr(wz), _ptr(g), _stream()),
+           "pps_face_gradient_device")
+    return g
+
+
+def faces_to_A(grid, cface, weights, b):
+    """8-slot system from face values c (3, nz, ny, nx) and an interior RHS."""
+    import torch
+    A = torch.empty(grid.shape_A, dtype=torch.float64, device=cface.device)
+    wx, wy, wz = weights
+    _check(_lib().pps_faces_to_A_device(ctypes.byref(grid), _ptr(cface), _ptr(wx), _ptr(wy),
+                                        _ptr(wz), _ptr(b), _ptr(A), _stream()),
+           "pps_faces_to_A_device")
+    return A
+
+
+# ---------------------------------------------------------------------------
+# benchmark / test configurations (synthetic droplet fields)
+# ---------------------------------------------------------------------------
+CONFIGS = {
+    # 64^3 walls, one sphere R=0.25, density ratio 10
+    "sphere64": dict(n=64, bc=(0, 0, 0, 0, 0, 0), drops=[(0.5, 0.5, 0.5, 0.25)], rho2=10.0),
+    # 128^3 walls, one sphere, density ratio 1000
+    "sphere128": dict(n=128, bc=(0, 0, 0, 0, 0, 0), drops=[(0.5, 0.5, 0.5, 0.25)], rho2=1000.0),
+    # 256^3 periodic-x / walls-yz, 64 droplets R in [0.03, 0.08], ratio 1000
+    "drops256": dict(n=256, bc=(1, 0, 0, 1, 0, 0), ndrop=64, rmin=0.03, rmax=0.08, rho2=1000.0),
+    # 512^3 walls, 512 droplets R in [0.01, 0.04], ratio 1000
+    "drops512": dict(n=512, bc=(0, 0, 0, 0, 0, 0), ndrop=512, rmin=0.01, rmax=0.04, rho2=1000.0),
+}
+
+
+def make_problem(name=None, n=None, bc=(0, 0, 0, 0, 0, 0), drops=None, ndrop=0, rmin=0.0,
+                 rmax=0.0, rho1=1.0, rho2=1000.0, seed=1, width=1.5, ng=2, dt=1.0):
+    """Build a droplet-field pressure system on the GPU.
+
+    Returns (grid, A, fields) with A the 8-slot system (right-hand side
+    mean removed) and fields = dict(rho, u, v, w, um, vm, wm).
+    """
+    if name is not None:
+        cfg = dict(CONFIGS[name])
+        n = cfg.pop("n")
+        bc = cfg.pop("bc")
+        drops = cfg.pop("drops", None)
+        ndrop = cfg.pop("ndrop", 0)
+        rmin = cfg.pop("rmin", 0.0)
+        rmax = cfg.pop("rmax", 0.0)
+        rho2 = cfg.pop("rho2", rho2)
+    grid = Grid.make(n, ng=ng, bdry_cond=bc)
+    if drops is None:
+        drops = synth_drops(grid, ndrop, rmin, rmax, seed)
+    rho, u, v, w = synth_fields(grid, drops, rho1, rho2, width, seed)
+    um, vm, wm = pressure_masks(grid)
+    A = setup_poisson(grid, u, v, w, um, vm, wm, rho, dt)
+    remove_rhs_mean(grid, A)
+    return grid, A, dict(rho=rho, u=u, v=v, w=w, um=um, vm=vm, wm=wm)
+
+
+# ---------------------------------------------------------------------------
+# autograd
+# ---------------------------------------------------------------------------
+def _autograd():
+    import torch
+
+    class PressureSolve(torch.autograd.Function):
+        """p = M(c)^-1 b with M assembled from face values c.
+
+        forward(cface (3,nz,ny,nx), b (nz,ny,nx), solver, weights, tol, maxit)
+        returns the interior pressure (nz,ny,nx).  backward solves the
+        adjoint M lam = grad_p on the same hierarchy (M is symmetric) and
+        forms grad_c = -w (lam_L - lam_R)(p_L - p_R) with the fused face
+        kernel; grad_b = lam.  The forward pressure is kept, not recomputed.
+        """
+
+        @staticmethod
+        def forward(ctx, cface, b, solver, weights, tol=1e-12, maxit=200):
+            grid = solver.grid
+            A = faces_to_A(grid, cface.detach().contiguous(), weights, b.detach().contiguous())
+            p = torch.zeros(grid.shape_p, dtype=torch.float64, device=b.device)
+            bn = float(torch.sqrt((b.detach() ** 2).mean()))
+            solver.solve(A, p, tol * bn if bn > 0 else tol, 1.0, maxit, 2, allow_maxit=False)
+            ctx.solver, ctx.weights, ctx.tol, ctx.maxit = solver, weights, tol, maxit
+            ctx.save_for_backward(A, p)
+            return grid.interior(p).clone()
+
+        @staticmethod
+        def backward(ctx, grad_p):
+            A, p = ctx.saved_tensors
+            solver = ctx.solver
+            grid = solver.grid
+            g = torch.zeros(grid.shape_p, dtype=torch.float64, device=A.device)
+            grid.interior(g).copy_(grad_p)
+            lam = torch.empty_like(g)
+            solver.adjoint(A, g, lam, rtol=ctx.tol, maxit=ctx.maxit)
+            gc = face_gradient(grid, A, p, lam, ctx.weights)
+            return gc, grid.interior(lam).clone(), None, None, None, None
+
+    return PressureSolve
+
+
+def __getattr__(name):
+    if name == "PressureSolve":
+        cls = _autograd()
+        globals()["PressureSolve"] = cls
+        return cls
+    raise AttributeError(name)

@@ -1,0 +1,116 @@
+// pps_adjoint.hip -- fused face kernel of the adjoint gradient.
+//
+// The operator M p = b has, for the face f between cells L and R with face
+// value c_f ("1/rho at the face", c_f = 2/(rho_L+rho_R), modules.f90:2737-
+// 2742) and geometric weight w_f = dt/(dx dxh), the entries
+// M_LL += w_f c_f, M_RR += w_f c_f, M_LR = M_RL -= w_f c_f.  For any
+// objective J(p) with adjoint M lam = dJ/dp, dJ/dc_f = -lam^T (dM/dc_f) p
+//   = -w_f (lam_L - lam_R) (p_L - p_R).
+// One thread per cell writes the gradient of its +x, +y, +z faces; faces
+// with a closed mask (zero coupling) and wall faces get 0.
+#include "pps_internal.h"
+
+namespace pps {
+
+struct FaceK {
+    int nx, ny, nz, ng, X, Y;
+    int px, py, pz;
+};
+
+__global__ void __launch_bounds__(256)
+k_face_grad(FaceK d, const double *__restrict__ A, const double *__restrict__ p,
+            const double *__restrict__ lam, const double *__restrict__ wx,
+            const double *__restrict__ wy, const double *__restrict__ wz,
+            double *__restrict__ g) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= d.nx || j >= d.ny) return;
+    const long long n = (long long)d.nx * d.ny * d.nz;
+    const long long c = i + (long long)d.nx * (j + (long long)d.ny * k);
+    const long long Py = d.X, Pz = (long long)d.X * d.Y;
+    const long long q = (i + d.ng) + Py * (j + d.ng) + Pz * (k + d.ng);
+    const double pc = p[q], lc = lam[q];
+    // +x face
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    if (A[n + c] != 0.0 && (i < d.nx - 1 || d.px)) {
+        const long long e = i < d.nx - 1 ? q + 1 : q - (d.nx - 1);
+        gx = -wx[i] * (lc - lam[e]) * (pc - p[e]);
+    }
+    if (A[3 * n + c] != 0.0 && (j < d.ny - 1 || d.py)) {
+        const long long e = j < d.ny - 1 ? q + Py : q - (long long)(d.ny - 1) * Py;
+        gy = -wy[j] * (lc - lam[e]) * (pc - p[e]);
+    }
+    if (A[5 * n + c] != 0.0 && (k < d.nz - 1 || d.pz)) {
+        const long long e = k < d.nz - 1 ? q + Pz : q - (long long)(d.nz - 1) * Pz;
+        gz = -wz[k] * (lc - lam[e]) * (pc - p[e]);
+    }
+    g[c] = gx;
+    g[n + c] = gy;
+    g[2 * n + c] = gz;
+}
+
+// face values -> 8-slot system (the map k_face_grad differentiates)
+__global__ void __launch_bounds__(256)
+k_faces_to_A(FaceK d, const double *__restrict__ cf, const double *__restrict__ wx,
+             const double *__restrict__ wy, const double *__restrict__ wz,
+             const double *__restrict__ b, double *__restrict__ A) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= d.nx || j >= d.ny) return;
+    const long long n = (long long)d.nx * d.ny * d.nz;
+    const long long c = i + (long long)d.nx * (j + (long long)d.ny * k);
+    const long long sy = d.nx, sz = (long long)d.nx * d.ny;
+    const double *cx = cf, *cy = cf + n, *cz = cf + 2 * n;
+    double a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0, a5 = 0.0, a6 = 0.0;
+    if (i > 0) a1 = wx[i - 1] * cx[c - 1];
+    else if (d.px) a1 = wx[d.nx - 1] * cx[c + d.nx - 1];
+    if (i < d.nx - 1 || d.px) a2 = wx[i] * cx[c];
+    if (j > 0) a3 = wy[j - 1] * cy[c - sy];
+    else if (d.py) a3 = wy[d.ny - 1] * cy[c + (long long)(d.ny - 1) * sy];
+    if (j < d.ny - 1 || d.py) a4 = wy[j] * cy[c];
+    if (k > 0) a5 = wz[k - 1] * cz[c - sz];
+    else if (d.pz) a5 = wz[d.nz - 1] * cz[c + (long long)(d.nz - 1) * sz];
+    if (k < d.nz - 1 || d.pz) a6 = wz[k] * cz[c];
+    A[c] = a1; A[n + c] = a2; A[2 * n + c] = a3; A[3 * n + c] = a4;
+    A[4 * n + c] = a5; A[5 * n + c] = a6;
+    A[6 * n + c] = ((((a1 + a2) + a3) + a4) + a5) + a6;
+    A[7 * n + c] = b[c];
+}
+
+}  // namespace pps
+
+using namespace pps;
+
+extern "C" int pps_faces_to_A_device(const pps_grid *grid, const double *cface, const double *wx,
+                                     const double *wy, const double *wz, const double *b,
+                                     double *A, void *stream) {
+    if (!grid || !cface || !wx || !wy || !wz || !b || !A) return PPS_ERR_BAD_ARG;
+    const Dims D = dims_of(*grid);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    FaceK d;
+    d.nx = D.nx; d.ny = D.ny; d.nz = D.nz; d.ng = D.ng; d.X = D.X(); d.Y = D.Y();
+    d.px = D.per[0]; d.py = D.per[1]; d.pz = D.per[2];
+    const dim3 grid3((unsigned)((d.nx + 63) / 64), (unsigned)((d.ny + 3) / 4), (unsigned)d.nz);
+    k_faces_to_A<<<grid3, dim3(64, 4, 1), 0, s>>>(d, cface, wx, wy, wz, b, A);
+    PPS_HIP(hipGetLastError());
+    PPS_HIP(hipStreamSynchronize(s));
+    return PPS_OK;
+}
+
+extern "C" int pps_face_gradient_device(const pps_grid *grid, const double *dA, const double *dp,
+                                        const double *dlam, const double *wx, const double *wy,
+                                        const double *wz, double *dgrad, void *stream) {
+    if (!grid || !dA || !dp || !dlam || !wx || !wy || !wz || !dgrad) return PPS_ERR_BAD_ARG;
+    const Dims D = dims_of(*grid);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    FaceK d;
+    d.nx = D.nx; d.ny = D.ny; d.nz = D.nz; d.ng = D.ng; d.X = D.X(); d.Y = D.Y();
+    d.px = D.per[0]; d.py = D.per[1]; d.pz = D.per[2];
+    const dim3 grid3((unsigned)((d.nx + 63) / 64), (unsigned)((d.ny + 3) / 4), (unsigned)d.nz);
+    k_face_grad<<<grid3, dim3(64, 4, 1), 0, s>>>(d, dA, dp, dlam, wx, wy, wz, dgrad);
+    PPS_HIP(hipGetLastError());
+    PPS_HIP(hipStreamSynchronize(s));
+    return PPS_OK;
+}

@@ -1,0 +1,220 @@
+// pps_api.cpp -- the C ABI of libpps: handle lifecycle (init_MG /
+// finalize_MG, MGsolver.f90:34-82), NewSolver / NewSolverMG
+// (newsolver.f90:34-57, MGsolver.f90:346-405) on host or device arrays,
+// and the adjoint entry point.
+#include <cmath>
+#include <cstring>
+
+#include "pps_internal.h"
+
+using namespace pps;
+
+extern "C" const char *pps_status_string(int st) {
+    switch (st) {
+    case PPS_OK: return "ok";
+    case PPS_ERR_NAN_OR_TINY_DIAG: return "A is NaN or A7 is too close to 0";
+    case PPS_ERR_VOLUME_SOURCE: return "large volume source";
+    case PPS_ERR_NOT_IN_KERNEL: return "constant p not in kernel";
+    case PPS_ERR_BAD_SIZE: return "multigrid requires 2^n cells per direction";
+    case PPS_ERR_DIVERGED: return "pressure solver diverged";
+    case PPS_ERR_NOT_CONVERGED: return "solver reached maxit";
+    case PPS_ERR_NONSYMMETRIC: return "operator is not symmetric";
+    case PPS_ERR_BAD_ARG: return "bad argument";
+    case PPS_ERR_HIP: return "HIP runtime error";
+    case PPS_ERR_NO_DEVICE: return "no HIP device";
+    case PPS_ERR_UNSUPPORTED: return "unsupported boundary condition";
+    default: return "unknown status";
+    }
+}
+
+extern "C" int pps_default_options(pps_options *o) {
+    if (!o) return PPS_ERR_BAD_ARG;
+    o->method = PPS_MG_FP64;
+    o->outer = PPS_OUTER_CG;
+    o->nrelax = 3;
+    o->sym_mg = 0;
+    o->nu_pre = 2;
+    o->nu_post = 2;
+    o->check_setup = 0;
+    o->multigrid = 1;
+    return PPS_OK;
+}
+
+static int valid_grid(const pps_grid *g) {
+    if (!g || g->nx <= 0 || g->ny <= 0 || g->nz <= 0 || g->ng < 1) return PPS_ERR_BAD_ARG;
+    for (int d = 0; d < 3; ++d) {
+        const bool p0 = g->bdry_cond[d] == 1, p1 = g->bdry_cond[d + 3] == 1;
+        if (p0 != p1) return PPS_ERR_BAD_ARG;   // paris.f90:2980-2986
+    }
+    return PPS_OK;
+}
+
+static int valid_opt(const pps_options *o) {
+    if (!o) return PPS_ERR_BAD_ARG;
+    if (o->method < PPS_MG_REFERENCE || o->method > PPS_SOR) return PPS_ERR_BAD_ARG;
+    if (o->outer != PPS_OUTER_CG && o->outer != PPS_OUTER_REFINE) return PPS_ERR_BAD_ARG;
+    if (o->nrelax < 0 || o->nu_pre < 0 || o->nu_post < 0) return PPS_ERR_BAD_ARG;
+    return PPS_OK;
+}
+
+extern "C" int pps_init(const pps_grid *grid, const pps_options *opt, pps_handle **out) {
+    if (!out) return PPS_ERR_BAD_ARG;
+    *out = nullptr;
+    PPS_TRY(valid_grid(grid));
+    pps_options o;
+    if (opt) o = *opt; else pps_default_options(&o);
+    PPS_TRY(valid_opt(&o));
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return PPS_ERR_NO_DEVICE;
+    pps_handle *h = new pps_handle();
+    h->grid = *grid;
+    h->opt = o;
+    if (hipGetDevice(&h->device) != hipSuccess) { delete h; return PPS_ERR_NO_DEVICE; }
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return PPS_ERR_HIP;
+    }
+    for (auto &e : h->ev) {
+        if (hipEventCreate(&e) != hipSuccess) { pps_finalize(h); return PPS_ERR_HIP; }
+    }
+    *out = h;
+    return PPS_OK;
+}
+
+extern "C" int pps_finalize(pps_handle *h) {
+    if (!h) return PPS_ERR_BAD_ARG;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    ref_destroy(h->ref);
+    fast_destroy(h->fast);
+    h->hA.release();
+    h->hp.release();
+    h->red.release();
+    if (h->pinned) (void)hipHostFree(h->pinned);
+    for (auto &e : h->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return PPS_OK;
+}
+
+extern "C" int pps_set_options(pps_handle *h, const pps_options *opt) {
+    if (!h) return PPS_ERR_BAD_ARG;
+    PPS_TRY(valid_opt(opt));
+    h->opt = *opt;
+    return PPS_OK;
+}
+
+static int solve_with(pps_handle *h, int method, const double *dA, double *dp, double maxError,
+                      double beta, int maxit, int norm, int *it, double *tres2, hipStream_t s) {
+    if (!dA || !dp || !it || !tres2 || norm < 1) return PPS_ERR_BAD_ARG;
+    (void)hipSetDevice(h->device);
+    const Dims D = dims_of(h->grid);
+    if (h->opt.check_setup) PPS_TRY(check_setup(h, D, dA, s));
+    h->setup_ms = 0.0;
+    if (maxit < 1) {
+        // the reference loop does not run: it = maxit+1, tres2 = the
+        // initial sum |r|^norm / N (MGsolver.f90:372-375)
+        double res = 0.0;
+        PPS_TRY(compute_residual_norm(h, D, dA, dp, norm, &res, s));
+        *tres2 = res / (double)D.cells();
+        *it = maxit + 1;
+        h->hist.clear();
+        return PPS_ERR_NOT_CONVERGED;
+    }
+    int rc;
+    if (method == PPS_MG_FP64 || method == PPS_MG_MIXED) {
+        const int saved = h->opt.method;
+        h->opt.method = method;
+        rc = fast_solve(h, dA, dp, maxError, maxit, norm, it, tres2, s);
+        h->opt.method = saved;
+    } else {
+        PPS_HIP(hipEventRecord(h->ev[0], s));
+        if (method == PPS_MG_REFERENCE) rc = ref_solve(h, dA, dp, maxError, beta, maxit, norm, it, tres2, s);
+        else rc = sor_solve(h, dA, dp, maxError, beta, maxit, norm, it, tres2, s);
+        PPS_HIP(hipEventRecord(h->ev[2], s));
+        PPS_HIP(hipEventSynchronize(h->ev[2]));
+        float ms = 0.f;
+        PPS_HIP(hipEventElapsedTime(&ms, h->ev[0], h->ev[2]));
+        h->solve_ms = ms;
+    }
+    if (rc != PPS_OK) return rc;
+    return *it > maxit ? PPS_ERR_NOT_CONVERGED : PPS_OK;
+}
+
+extern "C" int pps_solve_device(pps_handle *h, const double *dA, double *dp, double maxError,
+                                double beta, int maxit, int norm, int *it, double *tres2,
+                                void *stream) {
+    if (!h) return PPS_ERR_BAD_ARG;
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : h->stream;
+    return solve_with(h, h->opt.method, dA, dp, maxError, beta, maxit, norm, it, tres2, s);
+}
+
+// host-array staging shared by NewSolver / NewSolverMG
+static int host_solve(pps_handle *h, int method, const double *A, double *p, double maxError,
+                      double beta, int maxit, int *it, int *ierr, int norm, double *tres2) {
+    if (!h || !A || !p || !it) return PPS_ERR_BAD_ARG;
+    (void)hipSetDevice(h->device);
+    const Dims D = dims_of(h->grid);
+    const size_t bA = sizeof(double) * 8 * (size_t)D.cells();
+    const size_t bp = sizeof(double) * (size_t)D.padded();
+    PPS_TRY(h->hA.ensure(bA));
+    PPS_TRY(h->hp.ensure(bp));
+    hipStream_t s = h->stream;
+    PPS_HIP(hipMemcpyAsync(h->hA.ptr, A, bA, hipMemcpyHostToDevice, s));
+    PPS_HIP(hipMemcpyAsync(h->hp.ptr, p, bp, hipMemcpyHostToDevice, s));
+    double t2 = 0.0;
+    int rc = solve_with(h, method, h->hA.as<double>(), h->hp.as<double>(), maxError, beta, maxit,
+                        norm, it, &t2, s);
+    if (rc == PPS_OK || rc == PPS_ERR_NOT_CONVERGED) {
+        PPS_HIP(hipMemcpyAsync(p, h->hp.ptr, bp, hipMemcpyDeviceToHost, s));
+        PPS_HIP(hipStreamSynchronize(s));
+    }
+    if (tres2) *tres2 = t2;
+    // the reference reports the last MPI code in ierr; a failure aborts it
+    if (ierr) *ierr = (rc == PPS_OK || rc == PPS_ERR_NOT_CONVERGED) ? 0 : rc;
+    return rc;
+}
+
+extern "C" int pps_new_solver(pps_handle *h, const double *A, double *p, double maxError,
+                              double beta, int maxit, int *it, int *ierr, int norm) {
+    if (!h) return PPS_ERR_BAD_ARG;
+    const int method = h->opt.multigrid ? h->opt.method : PPS_SOR;
+    return host_solve(h, method, A, p, maxError, beta, maxit, it, ierr, norm, nullptr);
+}
+
+extern "C" int pps_new_solver_mg(pps_handle *h, const double *A, double *p, double maxError,
+                                 double beta, int maxit, int *it, int *ierr, int norm,
+                                 double *tres2) {
+    if (!h) return PPS_ERR_BAD_ARG;
+    const int method = h->opt.method == PPS_SOR ? PPS_MG_REFERENCE : h->opt.method;
+    return host_solve(h, method, A, p, maxError, beta, maxit, it, ierr, norm, tres2);
+}
+
+extern "C" int pps_last_history(pps_handle *h, double *hist, int cap) {
+    if (!h || (!hist && cap > 0)) return -PPS_ERR_BAD_ARG;
+    int n = (int)h->hist.size();
+    if (n > cap) n = cap;
+    for (int q = 0; q < n; ++q) hist[q] = h->hist[q];
+    return n;
+}
+
+extern "C" int pps_last_timing(pps_handle *h, double *solve_ms, double *setup_ms) {
+    if (!h) return PPS_ERR_BAD_ARG;
+    if (solve_ms) *solve_ms = h->solve_ms;
+    if (setup_ms) *setup_ms = h->setup_ms;
+    return PPS_OK;
+}
+
+extern "C" int pps_adjoint_device(pps_handle *h, const double *dA, const double *dg, double *dlam,
+                                  double rtol, int maxit, int *it, void *stream) {
+    if (!h || !dA || !dg || !dlam || !it || !(rtol > 0.0) || maxit < 1) return PPS_ERR_BAD_ARG;
+    (void)hipSetDevice(h->device);
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : h->stream;
+    // the adjoint runs on the fast hierarchy; the reference-faithful and
+    // plain methods use its FP64 form
+    const int saved = h->opt.method;
+    if (saved != PPS_MG_MIXED) h->opt.method = PPS_MG_FP64;
+    const int rc = fast_adjoint(h, dA, dg, dlam, rtol, maxit, it, s);
+    h->opt.method = saved;
+    return rc;
+}

@@ -1,0 +1,125 @@
+// pps_internal.h -- shared declarations of libpps (HIP, gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <vector>
+
+#include "../../include/pps.h"
+
+#define PPS_HIP(call)                                                        \
+    do {                                                                     \
+        hipError_t e_ = (call);                                              \
+        if (e_ != hipSuccess) {                                              \
+            std::fprintf(stderr, "libpps: %s failed: %s (%s:%d)\n", #call,   \
+                         hipGetErrorString(e_), __FILE__, __LINE__);         \
+            return PPS_ERR_HIP;                                              \
+        }                                                                    \
+    } while (0)
+
+#define PPS_TRY(expr)                 \
+    do {                              \
+        int rc_ = (expr);             \
+        if (rc_ != PPS_OK) return rc_; \
+    } while (0)
+
+namespace pps {
+
+// Geometry of one level: interior dims, ghost layers, padded extents,
+// periodic flags.  Vectors "with ghosts" use the padded extents, x fastest.
+struct Dims {
+    int nx, ny, nz, ng;
+    int per[3];
+    long long cells() const { return (long long)nx * ny * nz; }
+    int X() const { return nx + 2 * ng; }
+    int Y() const { return ny + 2 * ng; }
+    int Z() const { return nz + 2 * ng; }
+    long long padded() const { return (long long)X() * Y() * Z(); }
+};
+
+inline Dims dims_of(const pps_grid &g) {
+    Dims d;
+    d.nx = g.nx; d.ny = g.ny; d.nz = g.nz; d.ng = g.ng;
+    d.per[0] = g.bdry_cond[0] == 1;
+    d.per[1] = g.bdry_cond[1] == 1;
+    d.per[2] = g.bdry_cond[2] == 1;
+    return d;
+}
+
+// Device buffer owned by the handle (freed with the handle).
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t n) {
+        if (n <= bytes) return PPS_OK;
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr; bytes = 0;
+        if (hipMalloc(&ptr, n) != hipSuccess) return PPS_ERR_HIP;
+        bytes = n;
+        return PPS_OK;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr; bytes = 0;
+    }
+    template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+// Reductions: every reducing kernel writes one partial per block; a tiny
+// single-block kernel sums the partials in a fixed order so results do
+// not depend on scheduling.
+constexpr int kRedBlocks = 1024;   // partial slots per reduction
+constexpr int kRedThreads = 256;
+
+struct RefState;                   // reference-faithful MG (pps_ref.hip)
+struct FastState;                  // fast MG (pps_fast.hip)
+
+}  // namespace pps
+
+struct pps_handle {
+    pps_grid grid;
+    pps_options opt;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    pps::RefState *ref = nullptr;
+    pps::FastState *fast = nullptr;
+    std::vector<double> hist;
+    double solve_ms = 0.0, setup_ms = 0.0;
+    // scratch device buffers shared by drivers (A/p staging for host calls)
+    pps::DevBuf hA, hp;
+    pps::DevBuf red;               // reduction partials + results
+    double *pinned = nullptr;      // 16 pinned host doubles for scalar readback
+};
+
+namespace pps {
+
+// drivers (each returns a pps_status)
+int ref_solve(pps_handle *h, const double *dA, double *dp, double maxError,
+              double beta, int maxit, int norm, int *it, double *tres2,
+              hipStream_t s);
+int sor_solve(pps_handle *h, const double *dA, double *dp, double maxError,
+              double beta, int maxit, int norm, int *it, double *tres2,
+              hipStream_t s);
+void ref_destroy(RefState *r);
+
+int fast_solve(pps_handle *h, const double *dA, double *dp, double maxError,
+               int maxit, int norm, int *it, double *tres2, hipStream_t s);
+int fast_adjoint(pps_handle *h, const double *dA, const double *dg,
+                 double *dlam, double rtol, int maxit, int *it, hipStream_t s);
+void fast_destroy(FastState *f);
+
+// shared helpers (pps_assemble.hip); results are read back through the
+// handle's pinned scalars, so each call synchronises the stream.
+int check_setup(pps_handle *h, const Dims &d, const double *A, hipStream_t s);
+// sum |r|^norm of r = A8 - A7 p + sum A_nb p_nb over the interior.
+int compute_residual_norm(pps_handle *h, const Dims &d, const double *A,
+                          const double *p, int norm, double *out_sum,
+                          hipStream_t s);
+// {sum |r|, sum r^2, max |r|} (calcResiduals before normalisation).
+int residual_stats(pps_handle *h, const Dims &d, const double *A,
+                   const double *p, double out[3], hipStream_t s);
+
+}  // namespace pps

@@ -1,0 +1,385 @@
+// pps_ref.hip -- the reference multigrid (NewSolverMG, MGsolver.f90:34-405)
+// and the plain RB-SOR solver (NewSolver_std, newsolver.f90:78-174)
+// restated as gfx950 kernels on the reference's own 8-slot layout.
+//
+// Every arithmetic expression keeps the reference's operand order and no
+// product is fused into an add (fp contract off), so a colour pass, a
+// residual or a transfer gives the same bits as the Fortran loops; only
+// the residual-norm reductions sum in a different order.
+//
+// Single-GPU halo semantics: periodic directions wrap one ghost layer, wall
+// directions leave the ghosts alone (MPI_PROC_NULL in ghost_MG_x/y/z,
+// modules.f90:1846-1971).  Only face ghosts are maintained: edge and corner
+// ghosts are never read by the 7-point stencil.
+#pragma clang fp contract(off)
+
+#include <cmath>
+#include <cstring>
+
+#include "pps_internal.h"
+
+namespace pps {
+
+struct LvlK {
+    int nx, ny, nz, ng, X, Y;
+    int per0, per1, per2;
+};
+
+static LvlK lvlk(int nx, int ny, int nz, int ng, const int per[3]) {
+    LvlK L;
+    L.nx = nx; L.ny = ny; L.nz = nz; L.ng = ng;
+    L.X = nx + 2 * ng; L.Y = ny + 2 * ng;
+    L.per0 = per[0]; L.per1 = per[1]; L.per2 = per[2];
+    return L;
+}
+__host__ __device__ static long long lk_cells(const LvlK &L) { return (long long)L.nx * L.ny * L.nz; }
+static long long lk_padded(const LvlK &L) {
+    return (long long)L.X * L.Y * (L.nz + 2 * L.ng);
+}
+
+__device__ __forceinline__ long long ipad(const LvlK &L, int i, int j, int k) {
+    return (long long)(i + L.ng) + (long long)L.X * ((long long)(j + L.ng) + (long long)L.Y * (k + L.ng));
+}
+__device__ __forceinline__ long long icel(const LvlK &L, int i, int j, int k) {
+    return (long long)i + (long long)L.nx * ((long long)j + (long long)L.ny * k);
+}
+
+// ---------------------------------------------------------------------
+// RedBlackRelax colour pass (newsolver.f90:222-238): cells with
+// (i+j+k) % 2 == colour (0-based local indices), in place.
+// ---------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_rb_pass(LvlK L, const double *__restrict__ A, double *__restrict__ p,
+          double beta, int colour) {
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    const int i = 2 * (int)(blockIdx.x * blockDim.x + threadIdx.x) + ((j + k + colour) & 1);
+    if (j >= L.ny || i >= L.nx) return;
+    const long long n = lk_cells(L);
+    const long long c = icel(L, i, j, k), q = ipad(L, i, j, k);
+    const long long sy = L.X, sz = (long long)L.X * L.Y;
+    const double s = A[c] * p[q - 1] + A[n + c] * p[q + 1] + A[2 * n + c] * p[q - sy] +
+                     A[3 * n + c] * p[q + sy] + A[4 * n + c] * p[q - sz] +
+                     A[5 * n + c] * p[q + sz] + A[7 * n + c];
+    p[q] = (1.0 - beta) * p[q] + beta / A[6 * n + c] * s;
+}
+
+// One-layer periodic wrap of the face ghosts of a level.
+__global__ void __launch_bounds__(256)
+k_halo(LvlK L, double *__restrict__ p) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    if (L.per0 && a < L.ny && b < L.nz) {
+        p[ipad(L, -1, a, b)] = p[ipad(L, L.nx - 1, a, b)];
+        p[ipad(L, L.nx, a, b)] = p[ipad(L, 0, a, b)];
+    }
+    if (L.per1 && a < L.nx && b < L.nz) {
+        p[ipad(L, a, -1, b)] = p[ipad(L, a, L.ny - 1, b)];
+        p[ipad(L, a, L.ny, b)] = p[ipad(L, a, 0, b)];
+    }
+    if (L.per2 && a < L.nx && b < L.ny) {
+        p[ipad(L, a, b, -1)] = p[ipad(L, a, b, L.nz - 1)];
+        p[ipad(L, a, b, L.nz)] = p[ipad(L, a, b, 0)];
+    }
+}
+
+// get_residual in place (MGsolver.f90:96-102): slot 8 <- r.
+__global__ void __launch_bounds__(256)
+k_residual_inplace(LvlK L, double *__restrict__ A, const double *__restrict__ p) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= L.nx || j >= L.ny) return;
+    const long long n = lk_cells(L);
+    const long long c = icel(L, i, j, k), q = ipad(L, i, j, k);
+    const long long sy = L.X, sz = (long long)L.X * L.Y;
+    A[7 * n + c] = A[7 * n + c] - A[6 * n + c] * p[q] + A[c] * p[q - 1] + A[n + c] * p[q + 1] +
+                   A[2 * n + c] * p[q - sy] + A[3 * n + c] * p[q + sy] +
+                   A[4 * n + c] * p[q - sz] + A[5 * n + c] * p[q + sz];
+}
+
+// fill_coefficients (MGsolver.f90:210-220): all 8 slots = child mean.
+__global__ void __launch_bounds__(256)
+k_fill_coefficients(int fx, int fy, int fz, const double *__restrict__ Af,
+                    double *__restrict__ Ac) {
+    const int cx = fx / 2, cy = fy / 2, cz = fz / 2;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= cx || j >= cy) return;
+    const long long nf = (long long)fx * fy * fz, nc = (long long)cx * cy * cz;
+    const long long sy = fx, sz = (long long)fx * fy;
+    const long long f0 = 2LL * i + sy * (2LL * j) + sz * (2LL * k);
+    const long long co = i + (long long)cx * (j + (long long)cy * k);
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const double *a = Af + s * nf;
+        const double t = a[f0] + a[f0 + sy] + a[f0 + sz] + a[f0 + sy + sz] + a[f0 + 1] +
+                         a[f0 + 1 + sy] + a[f0 + 1 + sz] + a[f0 + 1 + sy + sz];
+        Ac[s * nc + co] = 0.125 * t;
+    }
+}
+
+// coarse_from_fine (MGsolver.f90:193-199): coarse slot 8 = sum of
+// 0.125 * child residual, children in (i1, j1, k1) order, k1 fastest.
+__global__ void __launch_bounds__(256)
+k_coarse_from_fine(int fx, int fy, int fz, const double *__restrict__ Af,
+                   double *__restrict__ Ac) {
+    const int cx = fx / 2, cy = fy / 2, cz = fz / 2;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= cx || j >= cy) return;
+    const long long nf = (long long)fx * fy * fz, nc = (long long)cx * cy * cz;
+    const long long sy = fx, sz = (long long)fx * fy;
+    const double *a = Af + 7 * nf;
+    const long long f0 = 2LL * i + sy * (2LL * j) + sz * (2LL * k);
+    double t = 0.0;
+    for (int i1 = 0; i1 < 2; ++i1)
+        for (int j1 = 0; j1 < 2; ++j1)
+            for (int k1 = 0; k1 < 2; ++k1) t = t + a[f0 + i1 + sy * j1 + sz * k1] * 0.125;
+    Ac[7 * nc + i + (long long)cx * (j + (long long)cy * k)] = t;
+}
+
+// coarse_fine_interp (MGsolver.f90:128-183) into the children of each
+// coarse cell; sym = sym_MG.
+__global__ void __launch_bounds__(256)
+k_coarse_fine_interp(LvlK F, LvlK C, double *__restrict__ ef,
+                     const double *__restrict__ ec, int sym) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= C.nx || j >= C.ny) return;
+    const double pc = ec[ipad(C, i, j, k)];
+    if (sym) {
+        const int o[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0},
+                             {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+        double tot = 0.0;
+        for (int m = 0; m < 8; ++m)
+            tot = tot + fabs(ef[ipad(F, 2 * i + o[m][0], 2 * j + o[m][1], 2 * k + o[m][2])]);
+        tot = tot + 1e-15;
+        for (int m = 0; m < 8; ++m) {
+            const long long q = ipad(F, 2 * i + o[m][0], 2 * j + o[m][1], 2 * k + o[m][2]);
+            ef[q] = fabs(ef[q]) / tot * pc;
+        }
+    } else {
+        for (int di = 0; di < 2; ++di)
+            for (int dj = 0; dj < 2; ++dj)
+                for (int dk = 0; dk < 2; ++dk) ef[ipad(F, 2 * i + di, 2 * j + dj, 2 * k + dk)] = pc;
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_zero(double *__restrict__ x, long long n) {
+    for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n;
+         q += (long long)gridDim.x * blockDim.x)
+        x[q] = 0.0;
+}
+
+__global__ void __launch_bounds__(256)
+k_add(double *__restrict__ y, const double *__restrict__ x, long long n) {
+    for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n;
+         q += (long long)gridDim.x * blockDim.x)
+        y[q] = y[q] + x[q];
+}
+
+// ---------------------------------------------------------------------
+// host driver
+// ---------------------------------------------------------------------
+struct RefState {
+    int lmg = 0;
+    std::vector<LvlK> lv;          // lv[1..lmg]
+    std::vector<DevBuf> data, pmg, emg;
+};
+
+void ref_destroy(RefState *r) {
+    if (!r) return;
+    for (auto &b : r->data) b.release();
+    for (auto &b : r->pmg) b.release();
+    for (auto &b : r->emg) b.release();
+    delete r;
+}
+
+static dim3 cell_grid(int nx, int ny, int nz) {
+    return dim3((unsigned)((nx + 63) / 64), (unsigned)((ny + 3) / 4), (unsigned)nz);
+}
+static const dim3 kCellBlock(64, 4, 1);
+
+static unsigned lin_blocks(long long n) {
+    long long b = (n + 255) / 256;
+    if (b > 8192) b = 8192;
+    if (b < 1) b = 1;
+    return (unsigned)b;
+}
+
+static int ref_ensure(pps_handle *h) {
+    if (h->ref) return PPS_OK;
+    const pps_grid &g = h->grid;
+    int nd[3] = {g.nx, g.ny, g.nz};
+    for (int d = 0; d < 3; ++d)
+        if (nd[d] <= 0 || (nd[d] & (nd[d] - 1))) return PPS_ERR_BAD_SIZE;
+    int lmg = 1 << 30;
+    for (int d = 0; d < 3; ++d) {
+        int l = 0;
+        while ((1 << (l + 1)) <= nd[d]) ++l;
+        if (l < lmg) lmg = l;
+    }
+    if (lmg < 2) return PPS_ERR_BAD_SIZE;
+    RefState *r = new RefState();
+    r->lmg = lmg;
+    r->lv.resize(lmg + 1);
+    r->data.resize(lmg + 1);
+    r->pmg.resize(lmg + 1);
+    r->emg.resize(lmg + 1);
+    Dims D = dims_of(g);
+    for (int l = lmg; l >= 1; --l) {
+        r->lv[l] = lvlk(nd[0] >> (lmg - l), nd[1] >> (lmg - l), nd[2] >> (lmg - l), g.ng, D.per);
+        int rc = r->data[l].ensure(sizeof(double) * 8 * lk_cells(r->lv[l]));
+        if (rc == PPS_OK) rc = r->pmg[l].ensure(sizeof(double) * lk_padded(r->lv[l]));
+        if (rc == PPS_OK) rc = r->emg[l].ensure(sizeof(double) * lk_padded(r->lv[l]));
+        if (rc != PPS_OK) { ref_destroy(r); return rc; }
+    }
+    h->ref = r;
+    return PPS_OK;
+}
+
+static void launch_zero(double *x, long long n, hipStream_t s) {
+    k_zero<<<lin_blocks(n), 256, 0, s>>>(x, n);
+}
+static void launch_add(double *y, const double *x, long long n, hipStream_t s) {
+    k_add<<<lin_blocks(n), 256, 0, s>>>(y, x, n);
+}
+static void launch_halo(const LvlK &L, double *p, hipStream_t s) {
+    if (!(L.per0 | L.per1 | L.per2)) return;
+    int a = L.nx > L.ny ? L.nx : L.ny;
+    int b = L.ny > L.nz ? L.ny : L.nz;
+    if (L.nz > b) b = L.nz;
+    k_halo<<<dim3((unsigned)((a + 255) / 256), (unsigned)b), 256, 0, s>>>(L, p);
+}
+// RedBlackRelax: pass 0, halo, pass 1 (newsolver.f90:222-246)
+static void launch_relax(const LvlK &L, const double *A, double *p, double beta, hipStream_t s) {
+    dim3 grid((unsigned)(((L.nx + 1) / 2 + 63) / 64), (unsigned)((L.ny + 3) / 4), (unsigned)L.nz);
+    k_rb_pass<<<grid, kCellBlock, 0, s>>>(L, A, p, beta, 0);
+    launch_halo(L, p, s);
+    k_rb_pass<<<grid, kCellBlock, 0, s>>>(L, A, p, beta, 1);
+}
+static void launch_residual(const LvlK &L, double *A, const double *p, hipStream_t s) {
+    k_residual_inplace<<<cell_grid(L.nx, L.ny, L.nz), kCellBlock, 0, s>>>(L, A, p);
+}
+
+// MG_iteration (MGsolver.f90:253-320)
+static void mg_iteration(RefState *r, double beta, int nrelax, int sym, hipStream_t s) {
+    const int lmg = r->lmg;
+    for (int l = lmg; l >= 2; --l) {
+        const LvlK &L = r->lv[l];
+        double *A = r->data[l].as<double>(), *x = r->pmg[l].as<double>();
+        launch_zero(x, lk_padded(L), s);
+        for (int q = 0; q < nrelax; ++q) {
+            launch_relax(L, A, x, beta, s);
+            launch_halo(L, x, s);
+        }
+        launch_residual(L, A, x, s);
+        const LvlK &C = r->lv[l - 1];
+        k_coarse_from_fine<<<cell_grid(C.nx, C.ny, C.nz), kCellBlock, 0, s>>>(
+            L.nx, L.ny, L.nz, A, r->data[l - 1].as<double>());
+    }
+    launch_relax(r->lv[2], r->data[2].as<double>(), r->pmg[2].as<double>(), beta, s);
+    launch_halo(r->lv[2], r->pmg[2].as<double>(), s);
+    for (int l = 3; l <= lmg; ++l) {
+        const LvlK &L = r->lv[l], &C = r->lv[l - 1];
+        double *A = r->data[l].as<double>();
+        double *e = r->emg[l].as<double>(), *x = r->pmg[l].as<double>();
+        const long long np = lk_padded(L);
+        launch_zero(e, np, s);
+        k_coarse_fine_interp<<<cell_grid(C.nx, C.ny, C.nz), kCellBlock, 0, s>>>(
+            L, C, e, r->pmg[l - 1].as<double>(), sym);
+        launch_halo(L, e, s);
+        launch_residual(L, A, e, s);
+        launch_add(x, e, np, s);
+        launch_zero(e, np, s);
+        for (int q = 0; q < nrelax; ++q) {
+            launch_relax(L, A, e, beta, s);
+            launch_halo(L, e, s);
+        }
+        launch_add(x, e, np, s);
+        launch_residual(L, A, e, s);
+    }
+}
+
+int ref_solve(pps_handle *h, const double *dA, double *dp, double maxError,
+              double beta, int maxit, int norm, int *it, double *tres2,
+              hipStream_t s) {
+    PPS_TRY(ref_ensure(h));
+    RefState *r = h->ref;
+    const int lmg = r->lmg;
+    const LvlK &F = r->lv[lmg];
+    const Dims D = dims_of(h->grid);
+    const long long n = lk_cells(F), np = lk_padded(F);
+    const double N = (double)h->grid.nx * (double)h->grid.ny * (double)h->grid.nz;
+    h->hist.clear();
+
+    // get_MGmatrix_coef (MGsolver.f90:322-344)
+    PPS_HIP(hipMemcpyAsync(r->data[lmg].ptr, dA, sizeof(double) * 8 * n, hipMemcpyDeviceToDevice, s));
+    for (int l = lmg - 1; l >= 1; --l) {
+        const LvlK &C = r->lv[l], &Fl = r->lv[l + 1];
+        k_fill_coefficients<<<cell_grid(C.nx, C.ny, C.nz), kCellBlock, 0, s>>>(
+            Fl.nx, Fl.ny, Fl.nz, r->data[l + 1].as<double>(), r->data[l].as<double>());
+    }
+    // initial residual into slot 8 (MGsolver.f90:370-377)
+    double *x = r->pmg[lmg].as<double>();
+    PPS_HIP(hipMemcpyAsync(x, dp, sizeof(double) * np, hipMemcpyDeviceToDevice, s));
+    double res = 0.0;
+    PPS_TRY(compute_residual_norm(h, D, r->data[lmg].as<double>(), x, norm, &res, s));
+    double t2 = res / N;
+    launch_residual(F, r->data[lmg].as<double>(), x, s);
+    launch_zero(x, np, s);
+    PPS_HIP(hipGetLastError());
+    int k;
+    for (k = 1; k <= maxit; ++k) {
+        mg_iteration(r, beta, h->opt.nrelax, h->opt.sym_mg, s);
+        launch_add(dp, x, np, s);
+        PPS_HIP(hipGetLastError());
+        PPS_TRY(compute_residual_norm(h, D, dA, dp, norm, &res, s));
+        t2 = res / N;
+        if (norm == 2) t2 = std::sqrt(t2);
+        h->hist.push_back(t2);
+        if (t2 < maxError) break;
+    }
+    *it = k;
+    *tres2 = t2;
+    return PPS_OK;
+}
+
+// NewSolver_std (newsolver.f90:78-174) with catch_divergence folded into
+// the residual norm: NaN in A or p makes the norm NaN.
+int sor_solve(pps_handle *h, const double *dA, double *dp, double maxError,
+              double beta, int maxit, int norm, int *it, double *tres2,
+              hipStream_t s) {
+    const pps_grid &g = h->grid;
+    const Dims D = dims_of(g);
+    LvlK L = lvlk(g.nx, g.ny, g.nz, g.ng, D.per);
+    const double N = (double)g.nx * (double)g.ny * (double)g.nz;
+    h->hist.clear();
+    double t2 = 0.0, res = 0.0;
+    int k;
+    for (k = 1; k <= maxit; ++k) {
+        launch_relax(L, dA, dp, beta, s);
+        launch_halo(L, dp, s);
+        PPS_HIP(hipGetLastError());
+        PPS_TRY(compute_residual_norm(h, D, dA, dp, norm, &res, s));
+        double res2 = res / N;
+        if (!(res2 <= 1e16)) {  // > 1e16 or NaN
+            *it = k;
+            *tres2 = res2;
+            return PPS_ERR_DIVERGED;
+        }
+        t2 = norm == 2 ? std::sqrt(res2) : res2;
+        h->hist.push_back(t2);
+        if (t2 < maxError) break;
+    }
+    *it = k;
+    *tres2 = t2;
+    return PPS_OK;
+}
+
+}  // namespace pps

@@ -1,0 +1,87 @@
+! Fortran caller of the drop-in, written the way PARIS calls NewSolver
+! (paris.f90:499): explicit-shape A(is:ie,js:je,ks:ke,8) and
+! p(imin:imax,jmin:jmax,kmin:kmax) with Ng=2 ghost layers, passed straight
+! through.  The system is a 32^3 walled box with one dense sphere (density
+! ratio 100) assembled with SetupPoisson's rule (modules.f90:2737-2746).
+! Usage: newsolver_driver <method 0..3> ; prints PASS/FAIL.
+program newsolver_driver
+  use pps_paris
+  implicit none
+  integer, parameter :: N = 32, Ng = 2
+  integer :: is, ie, imin, imax
+  real(8), allocatable :: A(:,:,:,:), p(:,:,:), rho(:,:,:), u(:,:,:), v(:,:,:), w(:,:,:)
+  integer :: i, j, k, it, ierr, method, bc(6)
+  real(8) :: h, x, y, z, d, c, maxError, res, rms_b, mean8, tres2, seed
+  character(len=16) :: arg
+
+  method = PPS_MG_FP64
+  if (command_argument_count() >= 1) then
+     call get_command_argument(1, arg)
+     read(arg, *) method
+  end if
+  is = Ng + 1; ie = Ng + N; imin = 1; imax = N + 2 * Ng
+  allocate(A(is:ie, is:ie, is:ie, 8), p(imin:imax, imin:imax, imin:imax))
+  allocate(rho(imin:imax, imin:imax, imin:imax), u(imin:imax, imin:imax, imin:imax))
+  allocate(v(imin:imax, imin:imax, imin:imax), w(imin:imax, imin:imax, imin:imax))
+  h = 1d0 / N
+  seed = 0.5d0
+  do k = imin, imax; do j = imin, imax; do i = imin, imax
+     x = (i - Ng - 0.5d0) * h; y = (j - Ng - 0.5d0) * h; z = (k - Ng - 0.5d0) * h
+     d = sqrt((x - 0.5d0)**2 + (y - 0.5d0)**2 + (z - 0.5d0)**2) - 0.25d0
+     c = 0.5d0 * (1d0 - tanh(d / (1.5d0 * h)))
+     rho(i, j, k) = c * (100d0 - 1d0) + 1d0
+     u(i, j, k) = frand(); v(i, j, k) = frand(); w(i, j, k) = frand()
+  end do; end do; end do
+  ! walls: no flux through the boundary faces
+  u(is - 1, :, :) = 0d0; u(ie, :, :) = 0d0
+  v(:, is - 1, :) = 0d0; v(:, ie, :) = 0d0
+  w(:, :, is - 1) = 0d0; w(:, :, ie) = 0d0
+  do k = is, ie; do j = is, ie; do i = is, ie
+     A(i,j,k,1) = merge(0d0, 2d0 / (h * h * (rho(i-1,j,k) + rho(i,j,k))), i == is)
+     A(i,j,k,2) = merge(0d0, 2d0 / (h * h * (rho(i+1,j,k) + rho(i,j,k))), i == ie)
+     A(i,j,k,3) = merge(0d0, 2d0 / (h * h * (rho(i,j-1,k) + rho(i,j,k))), j == is)
+     A(i,j,k,4) = merge(0d0, 2d0 / (h * h * (rho(i,j+1,k) + rho(i,j,k))), j == ie)
+     A(i,j,k,5) = merge(0d0, 2d0 / (h * h * (rho(i,j,k-1) + rho(i,j,k))), k == is)
+     A(i,j,k,6) = merge(0d0, 2d0 / (h * h * (rho(i,j,k+1) + rho(i,j,k))), k == ie)
+     A(i,j,k,7) = sum(A(i,j,k,1:6))
+     A(i,j,k,8) = -((u(i,j,k) - u(i-1,j,k)) + (v(i,j,k) - v(i,j-1,k)) + (w(i,j,k) - w(i,j,k-1))) / h
+  end do; end do; end do
+  mean8 = sum(A(:,:,:,8)) / dble(N)**3
+  A(:,:,:,8) = A(:,:,:,8) - mean8
+  rms_b = sqrt(sum(A(:,:,:,8)**2) / dble(N)**3)
+  maxError = 1d-10 * rms_b
+  p = 0d0
+
+  bc = 0
+  call pps_setup(N, N, N, Ng, bc, 1d0, 1d0, 1d0, .true., 3, method, ierr)
+  if (ierr /= 0) then
+     print *, 'FAIL setup', ierr
+     stop 1
+  end if
+  call pps_newsolver(A, p, maxError, 1.2d0, 4000, it, ierr, 2)
+  ! the reference's own residual (MGsolver.f90:118-124)
+  res = 0d0
+  do k = is, ie; do j = is, ie; do i = is, ie
+     res = res + (A(i,j,k,8) - A(i,j,k,7) * p(i,j,k) + A(i,j,k,1) * p(i-1,j,k) + &
+          A(i,j,k,2) * p(i+1,j,k) + A(i,j,k,3) * p(i,j-1,k) + A(i,j,k,4) * p(i,j+1,k) + &
+          A(i,j,k,5) * p(i,j,k-1) + A(i,j,k,6) * p(i,j,k+1))**2
+  end do; end do; end do
+  tres2 = sqrt(res / dble(N)**3)
+  call pps_teardown()
+  if (ierr == 0 .and. it <= 4000 .and. tres2 < maxError) then
+     print '(a,i2,a,i5,a,es10.3)', 'PASS method=', method, ' it=', it, ' rel=', tres2 / rms_b
+  else
+     print '(a,i2,a,i5,a,i3,a,es10.3)', 'FAIL method=', method, ' it=', it, ' ierr=', ierr, &
+          ' rel=', tres2 / rms_b
+     stop 1
+  end if
+
+contains
+
+  real(8) function frand()
+    ! deterministic sequence in (-1, 1)
+    seed = mod(seed * 997d0 + 0.123456789d0, 1d0)
+    frand = 2d0 * seed - 1d0
+  end function frand
+
+end program newsolver_driver
