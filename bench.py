@@ -1,0 +1,255 @@
+#!/usr/bin/env python3
+"""Pressure-solve benchmark on one MI355X.
+
+Metric: ms per pressure solve to 1e-10 relative residual (and HBM GB/s),
+FP32-V-cycle / FP64-refinement form versus the FP64 GPU multigrid.
+
+Workload (default): 256^3 box, periodic in x, walls in y and z, 64 droplets
+(centres uniform, radii uniform in [0.03, 0.08], seed 1), density ratio 1000,
+1.5-cell tanh interface, right-hand side = minus the divergence of iid
+U(-1,1) face velocities, mean removed.  A step = one complete solve from
+p = 0 to ||b - A p||_2 <= 1e-10 ||b||_2 on the device-resident 8-slot system.
+
+Also reported on the same line:
+  * roofline: live HIP-event timing of every finest-level smoother launch
+    during the timed region (algorithmic bytes / kernel time vs 8 TB/s);
+  * cpu_baseline: the reference algorithm (NewSolverMG restated in C, one
+    thread) timed on this host for a bounded number of V-cycles, scaled to
+    a whole solve with the iteration count of the same algorithm (counted by
+    its GPU restatement, whose residual history is the reference's).
+"""
+import argparse
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "ms per pressure solve to 1e-10 rel. residual + HBM GB/s; mixed vs FP64 GPU MG"
+
+
+def level_dims(n3):
+    """Level sizes of the fast hierarchy (mirrors the library's plan)."""
+    nx, ny, nz = n3
+    dims = [(nx, ny, nz)]
+    while nx * ny * nz > 64 and nx % 2 == 0 and ny % 2 == 0 and nz % 2 == 0:
+        nx, ny, nz = nx // 2, ny // 2, nz // 2
+        dims.append((nx, ny, nz))
+    return dims
+
+
+def solve_bytes(n3, iters, S, nu_pre=2, nu_post=2):
+    """Algorithmic bytes of one fast solve (every array element counted once
+    per launch that must touch it)."""
+    dims = level_dims(n3)
+    n0 = n3[0] * n3[1] * n3[2]
+    v = 0.0
+    for (a, b, c) in dims[:-1]:
+        n = a * b * c
+        passes_first = 1
+        passes = (2 * nu_pre - 1) + 2 * nu_post
+        v += n * S * (4.5 * passes_first + 5.5 * passes)    # colour passes
+        v += n * S * (6 + 1 / 8)                            # residual + restriction
+        v += n * S * (2 + 1 / 8)                            # prolongation + correction
+    nb = dims[-1][0] * dims[-1][1] * dims[-1][2]
+    v += nb * nb * S + 2 * nb * S                           # dense bottom solve
+    mixed = S == 4
+    outer = n0 * ((8 + S) + (56 + S) + (48 + (4 if mixed else 0)))
+    setup = n0 * (120 + (48 if mixed else 0))               # pack (+ FP32 copy)
+    for (a, b, c) in dims[1:]:
+        setup += a * b * c * S * (8 * 4 + 4)               # coarsening
+    final = n0 * (56 + (4 if mixed else 0)) + n0 * (72 + 16)  # initial residual, check, unpack
+    return setup + iters * (v + outer) + final
+
+
+def gpu_run(pps, torch, grid, A, method, tol, steps, warmup, maxit=200):
+    s = pps.Solver(grid, method=method)
+    p = torch.zeros(grid.shape_p, dtype=torch.float64, device="cuda")
+    for _ in range(warmup):
+        p.zero_()
+        s.solve(A, p, tol, 1.2, maxit, allow_maxit=False)
+    s.profile(True)
+    s.profile_read()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    its = []
+    for _ in range(steps):
+        p.zero_()
+        it, t2 = s.solve(A, p, tol, 1.2, maxit, allow_maxit=False)
+        its.append(it)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    kms, launches, kbytes = s.profile_read()
+    s.profile(False)
+    solve_ms_ev, setup_ms_ev = s.timing()
+    s.close()
+    return dict(ms=(t1 - t0) * 1e3 / steps, it=its[-1], tres2=t2, kernel_ms=kms,
+                launches=launches, kernel_bytes=kbytes, event_solve_ms=solve_ms_ev,
+                event_setup_ms=setup_ms_ev)
+
+
+def reference_iterations(pps, torch, grid, A, tol, cap):
+    """Iterations the reference algorithm needs (its GPU restatement)."""
+    s = pps.Solver(grid, method="reference")
+    p = torch.zeros(grid.shape_p, dtype=torch.float64, device="cuda")
+    t0 = time.perf_counter()
+    it, t2 = s.solve(A, p, tol, 1.2, cap)
+    gpu_ms = (time.perf_counter() - t0) * 1e3
+    hist = s.history()
+    s.close()
+    if it <= cap:
+        return it, False, hist, gpu_ms
+    # not converged within the cap: extrapolate the geometric rate of the
+    # last quarter of the history
+    q = max(len(hist) // 4, 2)
+    rate = (hist[-1] / hist[-q]) ** (1.0 / (q - 1))
+    extra = math.log(tol / hist[-1]) / math.log(rate) if rate < 1 else float("inf")
+    return cap + extra, True, hist, gpu_ms
+
+
+def cpu_baseline(A_host, grid, cycles, n_solve_cycles, extrapolated):
+    import numpy as np
+    import __graft_entry__ as ge
+    orc = ge.load_oracle()
+    og = orc.OrcGrid.make(grid.nx, grid.ny, grid.nz, bc=tuple(grid.bdry_cond))
+    p = np.zeros(og.shape_p)
+    t0 = time.perf_counter()
+    orc.newsolver_mg(og, A_host, p, 1e-300, 1.2, cycles)
+    dt = time.perf_counter() - t0
+    per_cycle = dt / cycles
+    cpu = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {
+        "value": per_cycle * n_solve_cycles * 1e3,
+        "unit": "ms",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"{cycles} V-cycles of the reference NewSolverMG algorithm (oracle C port, "
+                   f"1 thread, {cpu}) on the same system: {per_cycle * 1e3:.1f} ms/V-cycle x "
+                   f"{n_solve_cycles:.0f} V-cycles to reach 1e-10 "
+                   f"({'extrapolated' if extrapolated else 'counted'} by the GPU restatement "
+                   f"of the same algorithm)"),
+        "ms_per_vcycle": per_cycle * 1e3,
+        "vcycles_to_solve": n_solve_cycles,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="drops256")
+    ap.add_argument("--cpu-cycles", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ref-cap", type=int, default=3000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the solver runs on a single MI355X "
+              "(multi-GPU sharding is designed, not built); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import __graft_entry__ as ge
+    pps = ge.load_package()
+    if not torch.cuda.is_available():
+        print("no GPU", file=sys.stderr)
+        sys.exit(1)
+
+    print(f"[bench] building {args.workload}", file=sys.stderr, flush=True)
+    grid, A, _ = pps.make_problem(args.workload)
+    n3 = (grid.nx, grid.ny, grid.nz)
+    brms = float(torch.sqrt((A[7] ** 2).mean()))
+    tol = 1e-10 * brms
+
+    res = {}
+    for method in ("fp64", "mixed"):
+        print(f"[bench] {method}: {args.warmup} warmup + {args.steps} timed solves",
+              file=sys.stderr, flush=True)
+        res[method] = gpu_run(pps, torch, grid, A, method, tol, args.steps, args.warmup)
+        r = res[method]
+        S = 8 if method == "fp64" else 4
+        r["solve_bytes"] = solve_bytes(n3, r["it"], S)
+        r["solve_GBps"] = r["solve_bytes"] / (r["ms"] * 1e-3) / 1e9
+        r["smoother_GBps"] = r["kernel_bytes"] / (r["kernel_ms"] * 1e-3) / 1e9
+        print(f"[bench] {method}: {r['ms']:.3f} ms/solve, {r['it']} its, "
+              f"rel.res {r['tres2'] / brms:.2e}, smoother {r['smoother_GBps']:.0f} GB/s",
+              file=sys.stderr, flush=True)
+
+    m = res["mixed"]
+    f64 = res["fp64"]
+    n_cells = grid.cells
+    roof = {
+        "kernel": "k_rb<float,float,*,true> (finest-level red-black Gauss-Seidel colour pass, FP32)",
+        "bound": "hbm",
+        "achieved": round(m["smoother_GBps"], 1),
+        "peak": HBM_PEAK_GBPS,
+        "unit": "GB/s",
+        "frac": round(m["smoother_GBps"] / HBM_PEAK_GBPS, 4),
+        "traffic": None,
+        "launches": m["launches"],
+        "avg_launch_us": round(m["kernel_ms"] / max(m["launches"], 1) * 1e3, 2),
+        "bytes_per_launch": round(m["kernel_bytes"] / max(m["launches"], 1)),
+    }
+
+    cpu = None
+    ref_info = None
+    if not args.no_cpu:
+        print(f"[bench] reference algorithm iteration count on the GPU (cap {args.ref_cap})",
+              file=sys.stderr, flush=True)
+        nref, extrap, hist, ref_ms = reference_iterations(pps, torch, grid, A, tol, args.ref_cap)
+        ref_info = {"vcycles": nref, "extrapolated": extrap, "gpu_ms": ref_ms}
+        print(f"[bench] reference algorithm: {nref:.0f} V-cycles; CPU sample of "
+              f"{args.cpu_cycles} cycles", file=sys.stderr, flush=True)
+        A_host = np.ascontiguousarray(A.cpu().numpy())
+        cpu = cpu_baseline(A_host, grid, args.cpu_cycles, nref, extrap)
+        del A_host
+
+    line = {
+        "metric": METRIC,
+        "value": round(m["ms"], 4),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(m["ms"], 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 V-cycle / f64 refinement",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.workload}: {grid.nx}x{grid.ny}x{grid.nz}, bdry_cond "
+                        f"{list(grid.bdry_cond)}, droplet field, density ratio 1000, "
+                        f"rel. residual 1e-10",
+            "cells": n_cells,
+            "parallelism": "single GPU",
+        },
+        "fp64_ms": round(f64["ms"], 4),
+        "speedup_vs_fp64": round(f64["ms"] / m["ms"], 3),
+        "iterations": {"mixed": m["it"], "fp64": f64["it"]},
+        "rel_residual": {"mixed": m["tres2"] / brms, "fp64": f64["tres2"] / brms},
+        "solve_GBps": {"mixed": round(m["solve_GBps"], 1), "fp64": round(f64["solve_GBps"], 1)},
+        "smoother_GBps": {"mixed": round(m["smoother_GBps"], 1), "fp64": round(f64["smoother_GBps"], 1)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "reference_algorithm_gpu": ref_info,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -207,6 +207,16 @@ const char *pps_status_string(int status);
  * whole solve and of the setup (pack + coarsening) part. */
 int pps_last_timing(pps_handle *h, double *solve_ms, double *setup_ms);
 
+/* Measurement hooks (no reference counterpart; the reference's analogue is
+ * its wall-clock timer bucket "Poisson for pressure", modules.f90:252-296).
+ * When enabled, every finest-level smoother launch of the fast methods is
+ * bracketed by a HIP event pair on the solver's stream; read returns the
+ * summed kernel milliseconds, the launch count and the summed algorithmic
+ * bytes since the last read, and resets them. */
+int pps_profile_enable(pps_handle *h, int on);
+int pps_profile_read(pps_handle *h, double *kernel_ms, long long *launches,
+                     double *bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -151,6 +151,8 @@ def _lib():
         "pps_coords": (_I, [_I, _I, _D, _P, _P, _P, _P]),
         "pps_status_string": (ctypes.c_char_p, [_I]),
         "pps_faces_to_A_device": (_I, [_GP, _P, _P, _P, _P, _P, _P, _P]),
+        "pps_profile_enable": (_I, [_P, _I]),
+        "pps_profile_read": (_I, [_P, _DP, ctypes.POINTER(ctypes.c_longlong), _DP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -170,6 +172,7 @@ def exported_symbols():
         "pps_remove_rhs_mean_device", "pps_calc_residuals_device",
         "pps_project_velocity_device", "pps_synth_fields_device", "pps_synth_drops",
         "pps_coords", "pps_status_string", "pps_faces_to_A_device",
+        "pps_profile_enable", "pps_profile_read",
     ]
 
 
@@ -292,9 +295,21 @@ class Solver:
         return list(buf[:max(n, 0)])
 
     def timing(self):
+        """(solve_ms, setup_ms) of the last solve, HIP events."""
         a, b = ctypes.c_double(), ctypes.c_double()
         _lib().pps_last_timing(self._h, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
+
+    def profile(self, on=True):
+        """Bracket every finest-level smoother launch with HIP events."""
+        _check(_lib().pps_profile_enable(self._h, int(on)), "pps_profile_enable")
+
+    def profile_read(self):
+        """(kernel_ms, launches, algorithmic_bytes) since the last read."""
+        ms, n, b = ctypes.c_double(), ctypes.c_longlong(), ctypes.c_double()
+        _check(_lib().pps_profile_read(self._h, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(b)),
+               "pps_profile_read")
+        return ms.value, n.value, b.value
 
 
 # ---------------------------------------------------------------------------

@@ -90,6 +90,8 @@ extern "C" int pps_finalize(pps_handle *h) {
     h->hp.release();
     h->red.release();
     if (h->pinned) (void)hipHostFree(h->pinned);
+    for (auto &e : h->prof.ev)
+        if (e) (void)hipEventDestroy(e);
     for (auto &e : h->ev)
         if (e) (void)hipEventDestroy(e);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -202,6 +204,29 @@ extern "C" int pps_last_timing(pps_handle *h, double *solve_ms, double *setup_ms
     if (!h) return PPS_ERR_BAD_ARG;
     if (solve_ms) *solve_ms = h->solve_ms;
     if (setup_ms) *setup_ms = h->setup_ms;
+    return PPS_OK;
+}
+
+extern "C" int pps_profile_enable(pps_handle *h, int on) {
+    if (!h) return PPS_ERR_BAD_ARG;
+    (void)hipSetDevice(h->device);
+    if (on && h->prof.ev.empty()) {
+        h->prof.ev.resize(1024, nullptr);
+        for (auto &e : h->prof.ev) PPS_HIP(hipEventCreate(&e));
+    }
+    h->prof.on = on ? 1 : 0;
+    return PPS_OK;
+}
+
+extern "C" int pps_profile_read(pps_handle *h, double *kernel_ms, long long *launches,
+                                double *bytes) {
+    if (!h) return PPS_ERR_BAD_ARG;
+    if (kernel_ms) *kernel_ms = h->prof.ms;
+    if (launches) *launches = h->prof.launches;
+    if (bytes) *bytes = h->prof.bytes;
+    h->prof.ms = 0.0;
+    h->prof.launches = 0;
+    h->prof.bytes = 0.0;
     return PPS_OK;
 }
 

@@ -65,7 +65,9 @@ __device__ __forceinline__ T nb_sum(const LevelK<T> &L, const V *__restrict__ x,
 // red-black Gauss-Seidel colour pass.  colour 0 = (i+j+k) even.  FIRST:
 // the iterate is zero before this pass (no read of x).
 // ---------------------------------------------------------------------
-template <class T, class B, bool FIRST>
+// FINE does not change the code; it gives the finest level's launches
+// their own symbol so profilers can separate them from coarse levels.
+template <class T, class B, bool FIRST, bool FINE>
 __global__ void __launch_bounds__(256)
 k_rb(LevelK<T> L, const B *__restrict__ b, T *__restrict__ x, int colour) {
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -695,21 +697,44 @@ static void hier_build(Hier<T> &H, int singular, hipStream_t s) {
                                          H.minv.template as<T>());
 }
 
+// One colour pass of level l.  Algorithmic bytes of a pass on n cells:
+// all three coupling planes (a cell's -x/-y/-z couplings live in its
+// other-colour neighbours' planes), the diagonal and right-hand side of the
+// updated colour, the other colour's iterate (read) and the updated
+// colour's iterate (write; plus read unless FIRST):
+//   FIRST: (3 + 1/2 + 1/2 + 1/2) n S ;  else (3 + 1/2 + 1/2 + 1/2 + 1/2 + 1/2) n S
+template <class T>
+static void rb_pass(const Level<T> &L, int l, const T *b, int colour, bool first, Prof *prof,
+                    hipStream_t s) {
+    const LevelK<T> K = L.k();
+    const dim3 g2((unsigned)(((L.nx + 1) / 2 + 63) / 64), (unsigned)((L.ny + 3) / 4), (unsigned)L.nz);
+    if (l != 0) {
+        if (first) k_rb<T, T, true, false><<<g2, kB3, 0, s>>>(K, b, L.x, colour);
+        else k_rb<T, T, false, false><<<g2, kB3, 0, s>>>(K, b, L.x, colour);
+        return;
+    }
+    const double bytes = (double)L.n * sizeof(T) * (first ? 4.5 : 5.5);
+    hipEvent_t *e = prof ? prof->begin(bytes) : nullptr;
+    if (e) (void)hipEventRecord(e[0], s);
+    if (first) k_rb<T, T, true, true><<<g2, kB3, 0, s>>>(K, b, L.x, colour);
+    else k_rb<T, T, false, true><<<g2, kB3, 0, s>>>(K, b, L.x, colour);
+    if (e) (void)hipEventRecord(e[1], s);
+}
+
 // one V-cycle: x_0 = V(b_0).  fineb: the fine right-hand side (the FP64
 // residual in FP64 mode, its FP32 copy in mixed mode).
 template <class T>
-static void vcycle(Hier<T> &H, const T *fineb, int nu_pre, int nu_post, hipStream_t s) {
+static void vcycle(Hier<T> &H, const T *fineb, int nu_pre, int nu_post, Prof *prof, hipStream_t s) {
     const int nl = (int)H.lv.size();
     for (int l = 0; l < nl - 1; ++l) {
         Level<T> &L = H.lv[l];
         const LevelK<T> K = L.k();
         const T *b = l == 0 ? fineb : L.b;
-        const dim3 g2((unsigned)(((L.nx + 1) / 2 + 63) / 64), (unsigned)((L.ny + 3) / 4), (unsigned)L.nz);
-        k_rb<T, T, true><<<g2, kB3, 0, s>>>(K, b, L.x, 0);
-        k_rb<T, T, false><<<g2, kB3, 0, s>>>(K, b, L.x, 1);
+        rb_pass(L, l, b, 0, true, prof, s);
+        rb_pass(L, l, b, 1, false, prof, s);
         for (int q = 1; q < nu_pre; ++q) {
-            k_rb<T, T, false><<<g2, kB3, 0, s>>>(K, b, L.x, 0);
-            k_rb<T, T, false><<<g2, kB3, 0, s>>>(K, b, L.x, 1);
+            rb_pass(L, l, b, 0, false, prof, s);
+            rb_pass(L, l, b, 1, false, prof, s);
         }
         Level<T> &C = H.lv[l + 1];
         k_resid_restrict<T, T><<<cgrid3(C.nx, C.ny, C.nz), kB3, 0, s>>>(K, b, L.x, C.b);
@@ -722,14 +747,12 @@ static void vcycle(Hier<T> &H, const T *fineb, int nu_pre, int nu_post, hipStrea
     for (int l = nl - 2; l >= 0; --l) {
         Level<T> &L = H.lv[l];
         const Level<T> &C = H.lv[l + 1];
-        const LevelK<T> K = L.k();
         const T *b = l == 0 ? fineb : L.b;
         k_prolong_add<T><<<cgrid3(L.nx, L.ny, L.nz), kB3, 0, s>>>(L.nx, L.ny, L.nz, H.per[0], H.per[1],
                                                                   H.per[2], L.x, C.x);
-        const dim3 g2((unsigned)(((L.nx + 1) / 2 + 63) / 64), (unsigned)((L.ny + 3) / 4), (unsigned)L.nz);
         for (int q = 0; q < nu_post; ++q) {
-            k_rb<T, T, false><<<g2, kB3, 0, s>>>(K, b, L.x, 1);
-            k_rb<T, T, false><<<g2, kB3, 0, s>>>(K, b, L.x, 0);
+            rb_pass(L, l, b, 1, false, prof, s);
+            rb_pass(L, l, b, 0, false, prof, s);
         }
     }
 }
@@ -850,7 +873,8 @@ static int outer_iterate(pps_handle *h, Hier<T> &H, bool mixed, double tol_abs, 
     for (k = 1; k <= maxit; ++k) {
         // z = V(r)
         T *z = H.lv[0].x;
-        vcycle<T>(H, mixed ? (const T *)(const void *)b32 : (const T *)(const void *)V.r, nu1, nu2, s);
+        vcycle<T>(H, mixed ? (const T *)(const void *)b32 : (const T *)(const void *)V.r, nu1, nu2,
+                  &h->prof, s);
         if (cg) {
             k_dot_rz<T><<<nb, 256, 0, s>>>(n, V.r, z, part);
             k_scal_beta<<<1, 256, 0, s>>>(part, nb, n, f->singular, S);
@@ -916,6 +940,7 @@ int fast_solve(pps_handle *h, const double *dA, double *dp, double maxError, int
     PPS_HIP(hipEventElapsedTime(&ms_all, h->ev[0], h->ev[2]));
     h->setup_ms = ms_setup;
     h->solve_ms = ms_all;
+    PPS_TRY(h->prof.collect());
     if (rc != PPS_OK) return rc;
     *tres2 = t2;
     *it = (t2 < maxError) ? (total < 1 ? 1 : total) : maxit + 1;
@@ -970,6 +995,7 @@ int fast_adjoint(pps_handle *h, const double *dA, const double *dg, double *dlam
     PPS_HIP(hipGetLastError());
     PPS_HIP(hipStreamSynchronize(s));
     *it = k;
+    PPS_TRY(h->prof.collect());
     return last < tol ? PPS_OK : PPS_ERR_NOT_CONVERGED;
 }
 

@@ -76,6 +76,39 @@ constexpr int kRedThreads = 256;
 struct RefState;                   // reference-faithful MG (pps_ref.hip)
 struct FastState;                  // fast MG (pps_fast.hip)
 
+// Event timing of the fine-level smoother launches of a solve: an event
+// pair brackets each launch on the solver's stream; elapsed times are
+// collected after the solve has synchronised.
+struct Prof {
+    int on = 0;
+    std::vector<hipEvent_t> ev;
+    std::vector<double> pend_bytes;
+    int used = 0;
+    double ms = 0.0, bytes = 0.0;
+    long long launches = 0;
+    // returns the event to record before a launch of `b` algorithmic bytes,
+    // or nullptr when not profiling / out of events
+    hipEvent_t *begin(double b) {
+        if (!on || used + 2 > (int)ev.size()) return nullptr;
+        pend_bytes.push_back(b);
+        hipEvent_t *e = &ev[used];
+        used += 2;
+        return e;
+    }
+    int collect() {
+        for (int q = 0; q + 1 < used; q += 2) {
+            float t = 0.f;
+            if (hipEventElapsedTime(&t, ev[q], ev[q + 1]) != hipSuccess) return PPS_ERR_HIP;
+            ms += t;
+            bytes += pend_bytes[q / 2];
+            ++launches;
+        }
+        used = 0;
+        pend_bytes.clear();
+        return PPS_OK;
+    }
+};
+
 }  // namespace pps
 
 struct pps_handle {
@@ -92,6 +125,7 @@ struct pps_handle {
     pps::DevBuf hA, hp;
     pps::DevBuf red;               // reduction partials + results
     double *pinned = nullptr;      // 16 pinned host doubles for scalar readback
+    pps::Prof prof;                // fine-level smoother timing
 };
 
 namespace pps {
