@@ -69,6 +69,11 @@ enum pps_method {
     PPS_SOR = 3
 };
 
+enum pps_smoother {
+    PPS_SMOOTH_FUSED = 0,          /* one kernel per red-black sweep (default) */
+    PPS_SMOOTH_COLOUR_PASSES = 1   /* one kernel per colour */
+};
+
 enum pps_outer {
     PPS_OUTER_CG = 0,      /* preconditioned conjugate gradients */
     PPS_OUTER_REFINE = 1   /* plain iterative refinement p += V(b - A p) */
@@ -87,9 +92,12 @@ typedef struct pps_options {
     int nrelax;            /* reference nrelax (default 3) */
     int sym_mg;            /* reference sym_MG (default 0) */
     int nu_pre, nu_post;   /* fast V-cycle smoothing sweeps (default 2, 2) */
-    int check_setup;       /* run the check_poisson_setup tests first (default 1) */
+    int check_setup;       /* run the check_poisson_setup tests first (default 0) */
     int multigrid;         /* reference MultiGrid flag for pps_new_solver:
                               1 -> method, 0 -> PPS_SOR (default 1) */
+    int smoother;          /* enum pps_smoother (fast methods, default fused) */
+    int sweep_rows;        /* fused sweep: thread rows per block, 0 = automatic */
+    int sweep_planes;      /* fused sweep: planes per block, 0 = automatic */
 } pps_options;
 
 typedef struct pps_handle pps_handle;

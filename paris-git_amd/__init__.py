@@ -48,6 +48,8 @@ MG_MIXED = 2
 SOR = 3
 OUTER_CG = 0
 OUTER_REFINE = 1
+SMOOTH_FUSED = 0
+SMOOTH_COLOUR_PASSES = 1
 
 METHODS = {"reference": MG_REFERENCE, "fp64": MG_FP64, "mixed": MG_MIXED, "sor": SOR}
 
@@ -106,7 +108,9 @@ class Grid(ctypes.Structure):
 class Options(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int), ("outer", ctypes.c_int), ("nrelax", ctypes.c_int),
                 ("sym_mg", ctypes.c_int), ("nu_pre", ctypes.c_int), ("nu_post", ctypes.c_int),
-                ("check_setup", ctypes.c_int), ("multigrid", ctypes.c_int)]
+                ("check_setup", ctypes.c_int), ("multigrid", ctypes.c_int),
+                ("smoother", ctypes.c_int), ("sweep_rows", ctypes.c_int),
+                ("sweep_planes", ctypes.c_int)]
 
 
 _LIBH = None
@@ -217,13 +221,16 @@ class Solver:
     """A solver handle (init_MG / finalize_MG, MGsolver.f90:34-82)."""
 
     def __init__(self, grid, method="fp64", outer=OUTER_CG, nrelax=3, sym_mg=0, nu_pre=2,
-                 nu_post=2, check_setup=0, multigrid=1):
+                 nu_post=2, check_setup=0, multigrid=1, smoother=SMOOTH_FUSED, sweep_rows=0,
+                 sweep_planes=0):
         self.grid = grid
         o = default_options()
         o.method = METHODS[method] if isinstance(method, str) else int(method)
         o.outer, o.nrelax, o.sym_mg = int(outer), int(nrelax), int(sym_mg)
         o.nu_pre, o.nu_post = int(nu_pre), int(nu_post)
         o.check_setup, o.multigrid = int(check_setup), int(multigrid)
+        o.smoother = int(smoother)
+        o.sweep_rows, o.sweep_planes = int(sweep_rows), int(sweep_planes)
         self.options = o
         h = ctypes.c_void_p()
         _check(_lib().pps_init(ctypes.byref(grid), ctypes.byref(o), ctypes.byref(h)), "pps_init")

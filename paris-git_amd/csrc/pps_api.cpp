@@ -37,6 +37,9 @@ extern "C" int pps_default_options(pps_options *o) {
     o->nu_post = 2;
     o->check_setup = 0;
     o->multigrid = 1;
+    o->smoother = PPS_SMOOTH_FUSED;
+    o->sweep_rows = 0;
+    o->sweep_planes = 0;
     return PPS_OK;
 }
 
@@ -54,6 +57,8 @@ static int valid_opt(const pps_options *o) {
     if (o->method < PPS_MG_REFERENCE || o->method > PPS_SOR) return PPS_ERR_BAD_ARG;
     if (o->outer != PPS_OUTER_CG && o->outer != PPS_OUTER_REFINE) return PPS_ERR_BAD_ARG;
     if (o->nrelax < 0 || o->nu_pre < 0 || o->nu_post < 0) return PPS_ERR_BAD_ARG;
+    if (o->smoother != PPS_SMOOTH_FUSED && o->smoother != PPS_SMOOTH_COLOUR_PASSES) return PPS_ERR_BAD_ARG;
+    if (o->sweep_rows < 0 || o->sweep_rows > 1024 || o->sweep_planes < 0) return PPS_ERR_BAD_ARG;
     return PPS_OK;
 }
 

@@ -82,6 +82,227 @@ k_rb(LevelK<T> L, const B *__restrict__ b, T *__restrict__ x, int colour) {
     }
 }
 
+// ---------------------------------------------------------------------
+// One whole red-black Gauss-Seidel sweep in one pass: xout = sweep(xin).
+//
+// Cells are handled in x-pairs (2m, 2m+1); in every plane each pair holds
+// exactly one cell of each colour.  A block owns full rows of `rb` rows
+// (thread rows 0 and rb-1 are a ring that only computes first-colour
+// values for the neighbours of the rb-2 output rows) and marches through a
+// chunk of planes.  At plane step s it
+//   R: updates the first colour of plane s (all rows, ring included) from
+//      the second-colour xin of plane s staged in LDS and the thread's own
+//      column at s-1 / s+1, staging the results in LDS,
+//   K: updates the second colour of plane s-1 (output rows) from the
+//      first-colour values of planes s-2, s-1 (LDS) and s, and stores both
+//      cells of the pair at plane s-1,
+//   S: stages the second-colour xin of plane s+1 for the next step.
+// Every first-colour value uses only old second-colour values and every
+// second-colour value only new first-colour values, so the result equals
+// the two colour passes of k_rb.  Global loads run one plane ahead
+// (xin of plane s+2 and the operator of plane s+1 are requested at step s
+// and consumed at step s+1) and the per-plane barrier orders LDS only, so
+// the prefetches stay in flight across it.  Redundant work: the ring rows
+// and one extra first-colour plane at each end of a chunk.
+// c1: first colour ((x+y+z)&1 of the cells updated first).  FIRST: xin is
+// identically zero (not read).  PX pairs per thread.
+// LDS: X[2][rb+2][np] (second-colour xin), R[2][rb][np] (first colour new).
+// ---------------------------------------------------------------------
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <class T> struct PairOp {   // operator of one pair at one plane
+    T cxa, cxb, cxw;                  // +x couplings of 2m, 2m+1; of 2m-1
+    T cya, cyb, csa, csb;             // +y couplings of the pair; of row j-1
+    T cza, czb, dga, dgb, ba, bb;     // +z couplings, diagonals, rhs
+};
+template <class T> struct CellOp {   // operator of one cell
+    T cx, cxm, cy, cym, cz, dg, b;
+};
+template <class T>
+__device__ __forceinline__ CellOp<T> pick(const PairOp<T> &P, int e) {
+    CellOp<T> c;
+    c.cx = e ? P.cxb : P.cxa;
+    c.cxm = e ? P.cxa : P.cxw;
+    c.cy = e ? P.cyb : P.cya;
+    c.cym = e ? P.csb : P.csa;
+    c.cz = e ? P.czb : P.cza;
+    c.dg = e ? P.dgb : P.dga;
+    c.b = e ? P.bb : P.ba;
+    return c;
+}
+
+template <class T, bool FIRST, int PX>
+__global__ void __launch_bounds__(1024)
+k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *__restrict__ xout,
+           int c1, int zc, int oy) {
+    extern __shared__ unsigned char smem_raw[];
+    T *sm = reinterpret_cast<T *>(smem_raw);
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const int np = nx >> 1;                 // pairs per row
+    const int tpr = blockDim.x;             // threads per row (np / PX)
+    const int rb = blockDim.y;              // thread rows (oy + 2)
+    const int jj = threadIdx.y;
+    const long long sy = nx, sz = (long long)nx * ny;
+    T *Xs = sm;                             // [2][rb+2][np]
+    T *Rs = sm + 2 * (rb + 2) * np;         // [2][rb][np]
+    auto X = [&](int buf, int row, int m) -> T & { return Xs[(buf * (rb + 2) + row) * np + m]; };
+    auto R = [&](int buf, int row, int m) -> T & { return Rs[(buf * rb + row) * np + m]; };
+    const int y0 = blockIdx.x * oy;
+    const int k0 = blockIdx.y * zc;
+    const int k1 = min(k0 + zc, nz);
+    auto wrapz = [&](int k) { k %= nz; return k < 0 ? k + nz : k; };
+    auto wrapy = [&](int j) { j %= ny; return j < 0 ? j + ny : j; };
+    const int j = wrapy(y0 - 1 + jj);
+    const int jm = wrapy(j - 1);
+    const long long dym = (long long)(jm - j) * sy;
+    const bool out_row = jj >= 1 && jj <= oy && (y0 - 1 + jj) < ny;
+    const bool ring_lo = jj == 0, ring_hi = jj == rb - 1;
+    const int jx = ring_lo ? wrapy(j - 1) : wrapy(j + 1);   // extra staged row
+    auto col = [&](int jrow, int k, int m) { return 2LL * m + sy * jrow + sz * k; };
+
+    auto load_op = [&](int k, int m, PairOp<T> &P) {
+        const long long c = col(j, wrapz(k), m);
+        const long long cw = m == 0 ? c + (nx - 1) : c - 1;
+        P.cxa = L.cx[c]; P.cxb = L.cx[c + 1]; P.cxw = L.cx[cw];
+        P.cya = L.cy[c]; P.cyb = L.cy[c + 1];
+        P.csa = L.cy[c + dym]; P.csb = L.cy[c + dym + 1];
+        P.cza = L.cz[c]; P.czb = L.cz[c + 1];
+        P.dga = L.dg[c]; P.dgb = L.dg[c + 1];
+        P.ba = b[c]; P.bb = b[c + 1];
+    };
+    // second-colour xin of the extra row (ring threads only)
+    auto load_extra = [&](int k, int m) -> T {
+        if (FIRST || !(ring_lo || ring_hi)) return (T)0;
+        const int e2x = (c1 + jx + k + 1) & 1;
+        return xin[col(jx, wrapz(k), m) + e2x];
+    };
+    auto stage = [&](int k, int m, T a0, T a1, T ex) {
+        const int buf = k & 1;
+        const int e2 = (c1 + j + k + 1) & 1;   // offset of the second colour
+        X(buf, jj + 1, m) = e2 ? a1 : a0;
+        if (ring_lo) X(buf, 0, m) = ex;
+        if (ring_hi) X(buf, rb + 1, m) = ex;
+    };
+
+    // per pair: xin at planes s-1 .. s+2 (cells a = 2m, b = 2m+1), first
+    // colour new at s-2 .. s, operator of planes s (cur) and s+1 (nxt).
+    // The second-colour cell's operator of plane s-1 and the -z couplings
+    // are read again where needed (cache hits: loaded one step earlier).
+    T xa0[PX], xb0[PX], xa1[PX], xb1[PX], xa2[PX], xb2[PX], xa3[PX], xb3[PX], ex2[PX], ex3[PX];
+    T r_m2[PX], r_m1[PX], r_0[PX];
+    PairOp<T> cur[PX], nxt[PX];
+
+    // prologue: xin planes k0-2 .. k0 (+ extra row of k0), operator of
+    // k0-1 and the +z couplings of k0-2; stage plane k0-1
+#pragma unroll
+    for (int p = 0; p < PX; ++p) {
+        const int m = threadIdx.x + p * tpr;
+        if (FIRST) {
+            xa0[p] = xb0[p] = xa1[p] = xb1[p] = xa2[p] = xb2[p] = (T)0;
+        } else {
+            const long long ca = col(j, wrapz(k0 - 2), m), cb = col(j, wrapz(k0 - 1), m),
+                            cc = col(j, wrapz(k0), m);
+            xa0[p] = xin[ca]; xb0[p] = xin[ca + 1];
+            xa1[p] = xin[cb]; xb1[p] = xin[cb + 1];
+            xa2[p] = xin[cc]; xb2[p] = xin[cc + 1];
+        }
+        ex2[p] = load_extra(k0, m);
+        load_op(k0 - 1, m, cur[p]);
+        r_m2[p] = r_m1[p] = r_0[p] = (T)0;
+        stage(k0 - 1, m, xa1[p], xb1[p], load_extra(k0 - 1, m));
+    }
+    lds_barrier();
+
+    for (int s = k0 - 1; s <= k1; ++s) {
+        const int e1 = (c1 + j + s) & 1;   // first-colour offset at plane s
+        const bool more = s < k1;
+        // prefetch: xin of plane s+2, operator of plane s+1
+#pragma unroll
+        for (int p = 0; p < PX; ++p) {
+            const int m = threadIdx.x + p * tpr;
+            if (more) {
+                if (FIRST) {
+                    xa3[p] = xb3[p] = (T)0;
+                } else {
+                    const long long cc = col(j, wrapz(s + 2), m);
+                    xa3[p] = xin[cc];
+                    xb3[p] = xin[cc + 1];
+                }
+                ex3[p] = load_extra(s + 2, m);
+                load_op(s + 1, m, nxt[p]);
+            }
+        }
+        // R: first colour at plane s
+        const int bs = s & 1;
+#pragma unroll
+        for (int p = 0; p < PX; ++p) {
+            const int m = threadIdx.x + p * tpr;
+            const CellOp<T> o = pick(cur[p], e1);
+            T v = o.b;
+            if (!FIRST) {
+                const int mm = m == 0 ? np - 1 : m - 1, mp = m == np - 1 ? 0 : m + 1;
+                const T xw = e1 == 0 ? X(bs, jj + 1, mm) : xa1[p];
+                const T xe = e1 == 0 ? xb1[p] : X(bs, jj + 1, mp);
+                const T zm = L.cz[col(j, wrapz(s - 1), m) + e1];
+                const T xbz = e1 ? xb0[p] : xa0[p];
+                const T xtz = e1 ? xb2[p] : xa2[p];
+                v += o.cxm * xw + o.cx * xe;
+                v += o.cym * X(bs, jj, m) + o.cy * X(bs, jj + 2, m);
+                v += zm * xbz + o.cz * xtz;
+            }
+            v /= o.dg;
+            R(bs, jj, m) = v;
+            r_m2[p] = r_m1[p];
+            r_m1[p] = r_0[p];
+            r_0[p] = v;
+        }
+        // K: second colour at plane s-1 (offset e1 there), output
+        const int t = s - 1;
+        if (out_row && t >= k0 && t < k1) {
+            const int bt = t & 1;
+            const int e2 = e1;
+#pragma unroll
+            for (int p = 0; p < PX; ++p) {
+                const int m = threadIdx.x + p * tpr;
+                const int mm = m == 0 ? np - 1 : m - 1, mp = m == np - 1 ? 0 : m + 1;
+                const long long c = col(j, wrapz(t), m) + e2;
+                const long long cxm = (m == 0 && e2 == 0) ? c + (nx - 1) : c - 1;
+                CellOp<T> o;
+                o.cx = L.cx[c]; o.cxm = L.cx[cxm];
+                o.cy = L.cy[c]; o.cym = L.cy[c + dym];
+                o.cz = L.cz[c]; o.dg = L.dg[c]; o.b = b[c];
+                const T zm = L.cz[col(j, wrapz(t - 1), m) + e2];
+                const T xw = e2 == 0 ? R(bt, jj, mm) : r_m1[p];
+                const T xe = e2 == 0 ? r_m1[p] : R(bt, jj, mp);
+                T v = o.b;
+                v += o.cxm * xw + o.cx * xe;
+                v += o.cym * R(bt, jj - 1, m) + o.cy * R(bt, jj + 1, m);
+                v += zm * r_m2[p] + o.cz * r_0[p];
+                v /= o.dg;
+                const long long c0 = col(j, wrapz(t), m);
+                xout[c0] = e2 ? r_m1[p] : v;
+                xout[c0 + 1] = e2 ? v : r_m1[p];
+            }
+        }
+        // S: stage the second colour of plane s+1; rotate the registers
+#pragma unroll
+        for (int p = 0; p < PX; ++p) {
+            const int m = threadIdx.x + p * tpr;
+            if (more) stage(s + 1, m, xa2[p], xb2[p], ex2[p]);
+            cur[p] = nxt[p];
+            xa0[p] = xa1[p]; xb0[p] = xb1[p];
+            xa1[p] = xa2[p]; xb1[p] = xb2[p];
+            xa2[p] = xa3[p]; xb2[p] = xb3[p];
+            ex2[p] = ex3[p];
+        }
+        lds_barrier();
+    }
+}
+
 // residual of the 8 children of each coarse cell, averaged into bc
 template <class T, class B>
 __global__ void __launch_bounds__(256)
@@ -357,13 +578,41 @@ __device__ __forceinline__ double wsum(double v) {
     return v;
 }
 // block of 256: writes the block sum of v to part[blockIdx.x]
+// Works for any 256-thread block shape whose x extent is a multiple of 64
+// (each wavefront is 64 consecutive linear thread ids); the partial goes to
+// the block's linear id.
 __device__ __forceinline__ void block_partial(double v, double *part) {
     __shared__ double sm[4];
+    const int t = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    const int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     v = wsum(v);
-    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+    if ((t & 63) == 0) sm[t >> 6] = v;
     __syncthreads();
-    if (threadIdx.x == 0) part[blockIdx.x] = (sm[0] + sm[1]) + (sm[2] + sm[3]);
+    if (t == 0) part[b] = (sm[0] + sm[1]) + (sm[2] + sm[3]);
     __syncthreads();
+}
+
+// 2.5-D launch shape for z-marching kernels: (64, 4) threads over (x, y),
+// each thread walks `zc` consecutive planes; at most kRedBlocks blocks so
+// that one partial slot per block suffices.
+struct ZGrid {
+    dim3 grid;
+    int zc;
+    int blocks;
+};
+static ZGrid zgrid(int nx, int ny, int nz, int target = 2048) {
+    const int gx = (nx + 63) / 64, gy = (ny + 3) / 4;
+    const long long cols = (long long)gx * gy;
+    long long chunks = (target < kRedBlocks ? target : kRedBlocks) / cols;
+    if (chunks < 1) chunks = 1;
+    if (chunks > nz) chunks = nz;
+    const int zc = (int)((nz + chunks - 1) / chunks);
+    chunks = (nz + zc - 1) / zc;
+    ZGrid z;
+    z.grid = dim3((unsigned)gx, (unsigned)gy, (unsigned)chunks);
+    z.zc = zc;
+    z.blocks = (int)(cols * chunks);
+    return z;
 }
 // fixed-order sum of nb partials in a 256-thread block, result in thread 0
 __device__ __forceinline__ double sum_partials(const double *part, int nb) {
@@ -398,22 +647,40 @@ __device__ __forceinline__ double op_apply(const OpK &L, const double *__restric
     return L.dg[c] * v[c] - s;
 }
 
-// r = b - M x ; partials: 0 = sum r^2, 1 = sum b^2 ; optional FP32 copy
+// r = b - M x ; partials: 0 = sum r^2, 1 = sum b^2 ; optional FP32 copy.
+// z-marching over zc planes per thread (launch with zgrid()).
 template <class TB>
 __global__ void __launch_bounds__(256)
-k_outer_resid(OpK L, const double *__restrict__ b, const double *__restrict__ x,
+k_outer_resid(OpK L, int zc, const double *__restrict__ b, const double *__restrict__ x,
               double *__restrict__ r, TB *__restrict__ bt, double *__restrict__ part) {
-    const long long n = (long long)L.nx * L.ny * L.nz;
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int k0 = blockIdx.z * zc;
+    const int k1 = min(k0 + zc, L.nz);
     double rr = 0.0, bb = 0.0;
-    for (long long c = blockIdx.x * 256LL + threadIdx.x; c < n; c += (long long)gridDim.x * 256) {
-        const int i = (int)(c % L.nx);
-        const long long t = c / L.nx;
-        const int j = (int)(t % L.ny), k = (int)(t / L.ny);
-        const double v = b[c] - op_apply(L, x, i, j, k, c);
-        r[c] = v;
-        if (bt) bt[c] = (TB)v;
-        rr += v * v;
-        bb += b[c] * b[c];
+    if (i < L.nx && j < L.ny) {
+        const long long sy = L.nx, sz = (long long)L.nx * L.ny;
+        const int dxm = wrap_m(i, L.nx) - i, dxp = wrap_p(i, L.nx) - i;
+        const long long dym = (long long)(wrap_m(j, L.ny) - j) * sy, dyp = (long long)(wrap_p(j, L.ny) - j) * sy;
+        const long long col = i + sy * j;
+        double xm = x[col + sz * wrap_m(k0, L.nz)];
+        double x0 = x[col + sz * k0];
+        for (int k = k0; k < k1; ++k) {
+            const long long c = col + sz * k;
+            const long long czm = col + sz * wrap_m(k, L.nz);
+            const double xp = x[col + sz * wrap_p(k, L.nz)];
+            double s = L.cx[c + dxm] * x[c + dxm] + L.cx[c] * x[c + dxp];
+            s += L.cy[c + dym] * x[c + dym] + L.cy[c] * x[c + dyp];
+            s += L.cz[czm] * xm + L.cz[c] * xp;
+            const double bv = b[c];
+            const double v = bv - (L.dg[c] * x0 - s);
+            r[c] = v;
+            if (bt) bt[c] = (TB)v;
+            rr += v * v;
+            bb += bv * bv;
+            xm = x0;
+            x0 = xp;
+        }
     }
     block_partial(rr, part);
     block_partial(bb, part + kRedBlocks);
@@ -453,38 +720,47 @@ k_scal_beta(const double *__restrict__ part, int nb, long long n, int singular,
     }
 }
 
-// d_new = (z - zmean) + beta d_old ; q = M d_new ; partial sum d.q
+// d_new = (z - zmean) + beta d_old ; q = M d_new ; partial sum d.q.
+// z-marching (launch with zgrid()); the d_new values of the vertical
+// neighbours are carried in registers, the in-plane ones recomputed from
+// cached z and d_old.
 template <class TZ>
 __global__ void __launch_bounds__(256)
-k_dq(OpK L, const TZ *__restrict__ z, const double *__restrict__ dold, double *__restrict__ dnew,
-     double *__restrict__ q, const double *__restrict__ S, double *__restrict__ part) {
-    const long long n = (long long)L.nx * L.ny * L.nz;
+k_dq(OpK L, int zc, const TZ *__restrict__ z, const double *__restrict__ dold,
+     double *__restrict__ dnew, double *__restrict__ q, const double *__restrict__ S,
+     double *__restrict__ part) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int k0 = blockIdx.z * zc;
+    const int k1 = min(k0 + zc, L.nz);
     const double beta = S[S_BETA], zm = S[S_ZMEAN];
     const bool use_old = beta != 0.0;
     double dq = 0.0;
-    for (long long c = blockIdx.x * 256LL + threadIdx.x; c < n; c += (long long)gridDim.x * 256) {
-        const int i = (int)(c % L.nx);
-        const long long t = c / L.nx;
-        const int j = (int)(t % L.ny), k = (int)(t / L.ny);
+    if (i < L.nx && j < L.ny) {
         auto dv = [&](long long e) {
             const double v = (double)z[e] - zm;
             return use_old ? v + beta * dold[e] : v;
         };
         const long long sy = L.nx, sz = (long long)L.nx * L.ny;
-        const int im = wrap_m(i, L.nx), ip = wrap_p(i, L.nx);
-        const int jm = wrap_m(j, L.ny), jp = wrap_p(j, L.ny);
-        const int km = wrap_m(k, L.nz), kp = wrap_p(k, L.nz);
-        const long long cxm = c + (im - i), cxp = c + (ip - i);
-        const long long cym = c + (long long)(jm - j) * sy, cyp = c + (long long)(jp - j) * sy;
-        const long long czm = c + (long long)(km - k) * sz, czp = c + (long long)(kp - k) * sz;
-        const double d0 = dv(c);
-        double s = L.cx[cxm] * dv(cxm) + L.cx[c] * dv(cxp);
-        s += L.cy[cym] * dv(cym) + L.cy[c] * dv(cyp);
-        s += L.cz[czm] * dv(czm) + L.cz[c] * dv(czp);
-        const double qv = L.dg[c] * d0 - s;
-        dnew[c] = d0;
-        q[c] = qv;
-        dq += d0 * qv;
+        const int dxm = wrap_m(i, L.nx) - i, dxp = wrap_p(i, L.nx) - i;
+        const long long dym = (long long)(wrap_m(j, L.ny) - j) * sy, dyp = (long long)(wrap_p(j, L.ny) - j) * sy;
+        const long long col = i + sy * j;
+        double dm = dv(col + sz * wrap_m(k0, L.nz));
+        double d0 = dv(col + sz * k0);
+        for (int k = k0; k < k1; ++k) {
+            const long long c = col + sz * k;
+            const long long czm = col + sz * wrap_m(k, L.nz);
+            const double dp = dv(col + sz * wrap_p(k, L.nz));
+            double s = L.cx[c + dxm] * dv(c + dxm) + L.cx[c] * dv(c + dxp);
+            s += L.cy[c + dym] * dv(c + dym) + L.cy[c] * dv(c + dyp);
+            s += L.cz[czm] * dm + L.cz[c] * dp;
+            const double qv = L.dg[c] * d0 - s;
+            dnew[c] = d0;
+            q[c] = qv;
+            dq += d0 * qv;
+            dm = d0;
+            d0 = dp;
+        }
     }
     block_partial(dq, part);
 }
@@ -569,6 +845,136 @@ k_unpack(PackK d, const double *__restrict__ x, double *__restrict__ p) {
     if (d.pz && k == 0) p[q + (long long)d.nz * Pz] = v;
 }
 
+// Setup in one pass (z-marching, launch with zgrid()): from the 8-slot A
+// and the padded initial p, write the FP64 operator (couplings, diagonal),
+// the folded right-hand side b, the interior iterate x, the initial
+// residual r = A8 - A7 p + sum A_nb p_nb (the reference's own expression
+// on the padded p, MGsolver.f90:97-100), and in mixed mode the FP32 copies
+// of the operator and of r.  Partials: 0 = sum r^2, 1 = sum b^2.  flags as
+// k_pack.
+__global__ void __launch_bounds__(256)
+k_setup(PackK d, int zc, const double *__restrict__ A, const double *__restrict__ p,
+        double *__restrict__ cx, double *__restrict__ cy, double *__restrict__ cz,
+        double *__restrict__ dg, double *__restrict__ bo, double *__restrict__ x,
+        double *__restrict__ r, float *__restrict__ f32, float *__restrict__ r32,
+        double *__restrict__ part, int *__restrict__ flags) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int k0 = blockIdx.z * zc;
+    const int k1 = min(k0 + zc, d.nz);
+    const long long n = (long long)d.nx * d.ny * d.nz;
+    double rr = 0.0, bb = 0.0;
+    int f = 0;
+    if (i < d.nx && j < d.ny) {
+        const long long sy = d.nx, sz = (long long)d.nx * d.ny;
+        const long long Py = d.X, Pz = (long long)d.X * d.Y;
+        const bool lastx = i == d.nx - 1, lasty = j == d.ny - 1;
+        const long long ex_ = lastx ? -(long long)(d.nx - 1) : 1;
+        const long long ey_ = lasty ? -(long long)(d.ny - 1) * sy : sy;
+        for (int k = k0; k < k1; ++k) {
+            const long long c = i + sy * j + sz * k;
+            const long long q = (i + d.ng) + Py * (j + d.ng) + Pz * (k + d.ng);
+            double a[8];
+#pragma unroll
+            for (int s = 0; s < 8; ++s) a[s] = A[s * n + c];
+            bool nan = false;
+#pragma unroll
+            for (int s = 0; s < 8; ++s) nan |= a[s] != a[s];
+            if (nan || a[6] < 1e-55) f |= 1;
+            const double pc = p[q];
+            const double pw = p[q - 1], pe = p[q + 1], ps = p[q - Py], pn = p[q + Py],
+                         pb = p[q - Pz], pt = p[q + Pz];
+            const double rv = a[7] - a[6] * pc + a[0] * pw + a[1] * pe + a[2] * ps + a[3] * pn +
+                              a[4] * pb + a[5] * pt;
+            const bool lastz = k == d.nz - 1;
+            double rhs = a[7], kept = 0.0;
+            double vx = a[1], vy = a[3], vz = a[5];
+            if (!d.px) {
+                if (i == 0) rhs += a[0] * pw; else kept += a[0];
+                if (lastx) { rhs += a[1] * pe; vx = 0.0; } else kept += a[1];
+            } else {
+                kept += a[0] + a[1];
+            }
+            if (!d.py) {
+                if (j == 0) rhs += a[2] * ps; else kept += a[2];
+                if (lasty) { rhs += a[3] * pn; vy = 0.0; } else kept += a[3];
+            } else {
+                kept += a[2] + a[3];
+            }
+            if (!d.pz) {
+                if (k == 0) rhs += a[4] * pb; else kept += a[4];
+                if (lastz) { rhs += a[5] * pt; vz = 0.0; } else kept += a[5];
+            } else {
+                kept += a[4] + a[5];
+            }
+            if (!lastx || d.px) { if (A[c + ex_] != a[1]) f |= 2; }
+            if (!lasty || d.py) { if (A[2 * n + c + ey_] != a[3]) f |= 2; }
+            if (!lastz || d.pz) {
+                const long long e = lastz ? c - (long long)(d.nz - 1) * sz : c + sz;
+                if (A[4 * n + e] != a[5]) f |= 2;
+            }
+            if (fabs(a[6] - kept) > 1e-12 * fabs(a[6])) f |= 4;
+            cx[c] = vx; cy[c] = vy; cz[c] = vz; dg[c] = a[6];
+            bo[c] = rhs;
+            x[c] = pc;
+            r[c] = rv;
+            if (f32) {
+                f32[c] = (float)vx; f32[n + c] = (float)vy; f32[2 * n + c] = (float)vz;
+                f32[3 * n + c] = (float)a[6];
+                r32[c] = (float)rv;
+            }
+            rr += rv * rv;
+            bb += rhs * rhs;
+        }
+    }
+    if (f) atomicOr(flags, f);
+    block_partial(rr, part);
+    block_partial(bb, part + kRedBlocks);
+}
+
+// Final check fused with the write-back (z-marching): r = b - M x with the
+// FP64 operator, partial sum r^2, and p = x on the interior with the
+// first ghost layer wrapped in periodic directions.
+__global__ void __launch_bounds__(256)
+k_check_unpack(OpK L, PackK d, int zc, const double *__restrict__ b, const double *__restrict__ x,
+               double *__restrict__ p, double *__restrict__ part) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int k0 = blockIdx.z * zc;
+    const int k1 = min(k0 + zc, L.nz);
+    double rr = 0.0;
+    if (i < L.nx && j < L.ny) {
+        const long long sy = L.nx, sz = (long long)L.nx * L.ny;
+        const long long Py = d.X, Pz = (long long)d.X * d.Y;
+        const int dxm = wrap_m(i, L.nx) - i, dxp = wrap_p(i, L.nx) - i;
+        const long long dym = (long long)(wrap_m(j, L.ny) - j) * sy, dyp = (long long)(wrap_p(j, L.ny) - j) * sy;
+        const long long col = i + sy * j;
+        double xm = x[col + sz * wrap_m(k0, L.nz)];
+        double x0 = x[col + sz * k0];
+        for (int k = k0; k < k1; ++k) {
+            const long long c = col + sz * k;
+            const long long czm = col + sz * wrap_m(k, L.nz);
+            const double xp = x[col + sz * wrap_p(k, L.nz)];
+            double s = L.cx[c + dxm] * x[c + dxm] + L.cx[c] * x[c + dxp];
+            s += L.cy[c + dym] * x[c + dym] + L.cy[c] * x[c + dyp];
+            s += L.cz[czm] * xm + L.cz[c] * xp;
+            const double v = b[c] - (L.dg[c] * x0 - s);
+            rr += v * v;
+            const long long q = (i + d.ng) + Py * (j + d.ng) + Pz * (k + d.ng);
+            p[q] = x0;
+            if (d.px && i == d.nx - 1) p[q - d.nx] = x0;
+            if (d.px && i == 0) p[q + d.nx] = x0;
+            if (d.py && j == d.ny - 1) p[q - (long long)d.ny * Py] = x0;
+            if (d.py && j == 0) p[q + (long long)d.ny * Py] = x0;
+            if (d.pz && k == d.nz - 1) p[q - (long long)d.nz * Pz] = x0;
+            if (d.pz && k == 0) p[q + (long long)d.nz * Pz] = x0;
+            xm = x0;
+            x0 = xp;
+        }
+    }
+    block_partial(rr, part);
+}
+
 // adjoint right-hand side: interior of a padded vector
 __global__ void __launch_bounds__(256)
 k_interior(PackK d, const double *__restrict__ p, double *__restrict__ x) {
@@ -591,7 +997,8 @@ k_fill(double *__restrict__ a, long long n, double v) {
 template <class T> struct Level {
     int nx, ny, nz;
     long long n;
-    T *cx, *cy, *cz, *dg, *b, *x;
+    T *cx, *cy, *cz, *dg, *b;
+    T *x, *xt;   // current iterate and the other buffer of the out-of-place sweeps
     LevelK<T> k() const { return LevelK<T>{nx, ny, nz, cx, cy, cz, dg}; }
 };
 
@@ -617,6 +1024,8 @@ struct FastState {
     Hier<float> h32;
     DevBuf b32;      // FP32 right-hand side of the mixed V-cycle
     int mixed_built = 0, fp64_built = 0;
+    int fused = 1;   // one kernel per red-black sweep
+    int have_r = 0;  // the setup pass already wrote r (and its norms) for x
 };
 
 void fast_destroy(FastState *f) {
@@ -635,7 +1044,7 @@ static dim3 cgrid3(int nx, int ny, int nz) {
 static const dim3 kB3(64, 4, 1);
 static int nblk(long long n) {
     long long b = (n + 255) / 256;
-    if (b > kRedBlocks) b = kRedBlocks;
+    if (b > 2048) b = 2048;   // partials per reduction: a few per CU
     if (b < 1) b = 1;
     return (int)b;
 }
@@ -665,17 +1074,18 @@ static int hier_alloc(Hier<T> &H, const std::vector<int> &dims, bool own_fine, c
         L.nx = dims[3 * l]; L.ny = dims[3 * l + 1]; L.nz = dims[3 * l + 2];
         L.n = (long long)L.nx * L.ny * L.nz;
         const bool own = l > 0 || own_fine;
-        // coefficient planes (4) + b + x, or only x when the fine operator
-        // and right-hand side are borrowed from the outer iteration
-        const int planes = own ? 6 : 1;
+        // coefficient planes (4) + b + x + xt, or only xt when the fine
+        // operator, right-hand side and iterate are borrowed from the outer
+        // iteration
+        const int planes = own ? 7 : 1;
         PPS_TRY(H.bufs[l].ensure(sizeof(T) * planes * L.n));
         T *base = H.bufs[l].template as<T>();
         if (own) {
             L.cx = base; L.cy = base + L.n; L.cz = base + 2 * L.n; L.dg = base + 3 * L.n;
-            L.b = base + 4 * L.n; L.x = base + 5 * L.n;
+            L.b = base + 4 * L.n; L.x = base + 5 * L.n; L.xt = base + 6 * L.n;
         } else {
-            L.cx = L.cy = L.cz = L.dg = L.b = nullptr;
-            L.x = base;
+            L.cx = L.cy = L.cz = L.dg = L.b = L.x = nullptr;
+            L.xt = base;
         }
     }
     H.nb = (int)H.lv.back().n;
@@ -721,23 +1131,101 @@ static void rb_pass(const Level<T> &L, int l, const T *b, int colour, bool first
     if (e) (void)hipEventRecord(e[1], s);
 }
 
-// one V-cycle: x_0 = V(b_0).  fineb: the fine right-hand side (the FP64
-// residual in FP64 mode, its FP32 copy in mixed mode).
+// Launch shape of the fused sweep on a level: pairs per thread, thread
+// rows per block, planes per block.  Blocks hold at most 1024 threads and
+// 64 KB of LDS; about 1024 blocks fill the chip.
+struct SweepShape {
+    int px, tpr, rb, oy, zc;
+    dim3 grid, block;
+    size_t lds;
+};
 template <class T>
-static void vcycle(Hier<T> &H, const T *fineb, int nu_pre, int nu_post, Prof *prof, hipStream_t s) {
+static SweepShape sweep_shape(int nx, int ny, int nz, int max_rows, int planes) {
+    SweepShape sh;
+    const int np = nx / 2;
+    sh.px = 1;
+    sh.tpr = np;
+    int rb = 1024 / sh.tpr;
+    if (max_rows > 0 && rb > max_rows) rb = max_rows;
+    const int lds_rows = (int)((65536 / ((long long)np * sizeof(T)) - 4) / 4);
+    if (rb > lds_rows) rb = lds_rows;
+    if (rb > ny + 2) rb = ny + 2;
+    sh.rb = rb;                      // < 3: the caller falls back to colour passes
+    sh.oy = rb - 2 >= ny ? ny : (rb > 2 ? rb - 2 : 1);
+    const int ytiles = (ny + sh.oy - 1) / sh.oy;
+    int zc = planes > 0 ? planes : (int)(((long long)nz * ytiles + 1023) / 1024);
+    if (zc < 2) zc = 2;
+    if (zc > nz) zc = nz;
+    sh.zc = zc;
+    sh.grid = dim3((unsigned)ytiles, (unsigned)((nz + zc - 1) / zc), 1);
+    sh.block = dim3((unsigned)sh.tpr, (unsigned)(rb > 0 ? rb : 1), 1);
+    sh.lds = sizeof(T) * (size_t)np * (2 * (rb + 2) + 2 * rb);
+    return sh;
+}
+
+// block-shape overrides of the fused sweep for the current solve (0 = auto)
+static int g_sweep_rows = 0, g_sweep_planes = 0;
+
+// One fused red-black sweep of level l (out of place, the iterate pointers
+// swap).  Algorithmic bytes: the 4 operator planes, b, the old iterate
+// (unless FIRST) and the new iterate, once each: (6 or 7) n S.
+template <class T>
+static void rb_pass(const Level<T> &L, int l, const T *b, int colour, bool first, Prof *prof,
+                    hipStream_t s);
+
+template <class T>
+static void rb_sweep(Level<T> &L, int l, const T *b, int c1, bool first, Prof *prof, hipStream_t s) {
+    const SweepShape sh = sweep_shape<T>(L.nx, L.ny, L.nz, g_sweep_rows, g_sweep_planes);
+    if (sh.rb < 3 || sh.lds > 65536 || (L.nx & 1) || sh.tpr * sh.rb > 1024) {
+        // rows too long for the LDS planes: two colour passes instead
+        rb_pass(L, l, b, c1, first, prof, s);
+        rb_pass(L, l, b, 1 - c1, false, prof, s);
+        return;
+    }
+    const LevelK<T> K = L.k();
+    const double bytes = (double)L.n * sizeof(T) * (first ? 6.0 : 7.0);
+    hipEvent_t *e = (l == 0 && prof) ? prof->begin(bytes) : nullptr;
+    if (e) (void)hipEventRecord(e[0], s);
+#define PPS_SWEEP(PX)                                                                              \
+    do {                                                                                           \
+        if (first)                                                                                 \
+            k_rb_sweep<T, true, PX><<<sh.grid, sh.block, sh.lds, s>>>(K, b, L.x, L.xt, c1, sh.zc,  \
+                                                                       sh.oy);                     \
+        else                                                                                       \
+            k_rb_sweep<T, false, PX><<<sh.grid, sh.block, sh.lds, s>>>(K, b, L.x, L.xt, c1, sh.zc, \
+                                                                        sh.oy);                    \
+    } while (0)
+    PPS_SWEEP(1);
+#undef PPS_SWEEP
+    if (e) (void)hipEventRecord(e[1], s);
+    T *t = L.x;
+    L.x = L.xt;
+    L.xt = t;
+}
+
+// one V-cycle: x_0 = V(b_0).  fineb: the fine right-hand side (the FP64
+// residual in FP64 mode, its FP32 copy in mixed mode).  fused: one kernel
+// per red-black sweep (k_rb_sweep) instead of one per colour (k_rb).
+template <class T>
+static void vcycle(Hier<T> &H, const T *fineb, int nu_pre, int nu_post, Prof *prof, hipStream_t s,
+                   bool fused = true) {
     const int nl = (int)H.lv.size();
     for (int l = 0; l < nl - 1; ++l) {
         Level<T> &L = H.lv[l];
-        const LevelK<T> K = L.k();
         const T *b = l == 0 ? fineb : L.b;
-        rb_pass(L, l, b, 0, true, prof, s);
-        rb_pass(L, l, b, 1, false, prof, s);
-        for (int q = 1; q < nu_pre; ++q) {
-            rb_pass(L, l, b, 0, false, prof, s);
+        if (fused) {
+            rb_sweep(L, l, b, 0, true, prof, s);
+            for (int q = 1; q < nu_pre; ++q) rb_sweep(L, l, b, 0, false, prof, s);
+        } else {
+            rb_pass(L, l, b, 0, true, prof, s);
             rb_pass(L, l, b, 1, false, prof, s);
+            for (int q = 1; q < nu_pre; ++q) {
+                rb_pass(L, l, b, 0, false, prof, s);
+                rb_pass(L, l, b, 1, false, prof, s);
+            }
         }
         Level<T> &C = H.lv[l + 1];
-        k_resid_restrict<T, T><<<cgrid3(C.nx, C.ny, C.nz), kB3, 0, s>>>(K, b, L.x, C.b);
+        k_resid_restrict<T, T><<<cgrid3(C.nx, C.ny, C.nz), kB3, 0, s>>>(L.k(), b, L.x, C.b);
     }
     {
         Level<T> &Bt = H.lv.back();
@@ -751,8 +1239,12 @@ static void vcycle(Hier<T> &H, const T *fineb, int nu_pre, int nu_post, Prof *pr
         k_prolong_add<T><<<cgrid3(L.nx, L.ny, L.nz), kB3, 0, s>>>(L.nx, L.ny, L.nz, H.per[0], H.per[1],
                                                                   H.per[2], L.x, C.x);
         for (int q = 0; q < nu_post; ++q) {
-            rb_pass(L, l, b, 1, false, prof, s);
-            rb_pass(L, l, b, 0, false, prof, s);
+            if (fused) {
+                rb_sweep(L, l, b, 1, false, prof, s);
+            } else {
+                rb_pass(L, l, b, 1, false, prof, s);
+                rb_pass(L, l, b, 0, false, prof, s);
+            }
         }
     }
 }
@@ -805,9 +1297,24 @@ static int fast_setup(pps_handle *h, const double *dA, const double *dp, bool mi
     const OpK O = opk(f);
     Vecs V = vecs(f);
     int *flags = f->flags.as<int>();
+    std::vector<int> dims;
+    PPS_TRY(plan_levels(d, dims));
+    float *f32 = nullptr, *r32 = nullptr;
+    if (mixed) {
+        if (!f->mixed_built) { PPS_TRY(hier_alloc(f->h32, dims, true, d.per)); f->mixed_built = 1; }
+        PPS_TRY(f->b32.ensure(sizeof(float) * n));
+        f32 = f->h32.lv[0].cx;   // cx, cy, cz, dg are contiguous
+        r32 = f->b32.as<float>();
+    }
+    // operator, right-hand side, iterate and initial residual in one pass
+    double *part = f->red.as<double>();
+    double *S = part + 4 * kRedBlocks;
+    const ZGrid zg = zgrid(d.nx, d.ny, d.nz);
     PPS_HIP(hipMemsetAsync(flags, 0, sizeof(int), s));
-    k_pack<<<cgrid3(d.nx, d.ny, d.nz), kB3, 0, s>>>(packk(d), dA, dp, (double *)O.cx, (double *)O.cy,
-                                                   (double *)O.cz, (double *)O.dg, V.b, V.x, flags);
+    k_setup<<<zg.grid, dim3(64, 4, 1), 0, s>>>(packk(d), zg.zc, dA, dp, (double *)O.cx, (double *)O.cy,
+                                                (double *)O.cz, (double *)O.dg, V.b, V.x, V.r, f32, r32,
+                                                part, flags);
+    k_scal_rr<<<1, 256, 0, s>>>(part, zg.blocks, 1, S);
     PPS_HIP(hipGetLastError());
     int fl = 0;
     PPS_HIP(hipMemcpyAsync(&fl, flags, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -815,18 +1322,10 @@ static int fast_setup(pps_handle *h, const double *dA, const double *dp, bool mi
     if (fl & 1) return PPS_ERR_NAN_OR_TINY_DIAG;
     if (fl & 2) return PPS_ERR_NONSYMMETRIC;
     f->singular = (fl & 4) ? 0 : 1;
+    f->have_r = 1;
 
-    std::vector<int> dims;
-    PPS_TRY(plan_levels(d, dims));
     if (mixed) {
-        Hier<float> &H = f->h32;
-        if (!f->mixed_built) { PPS_TRY(hier_alloc(H, dims, true, d.per)); f->mixed_built = 1; }
-        Level<float> &L0 = H.lv[0];
-        const unsigned nbk = (unsigned)nblk(4 * n);
-        // the four FP64 planes are contiguous, as are the FP32 ones
-        k_convert<double, float><<<nbk, 256, 0, s>>>(O.cx, L0.cx, 4 * n);
-        hier_build(H, f->singular, s);
-        PPS_TRY(f->b32.ensure(sizeof(float) * n));
+        hier_build(f->h32, f->singular, s);
     } else {
         Hier<double> &H = f->h64;
         if (!f->fp64_built) { PPS_TRY(hier_alloc(H, dims, false, d.per)); f->fp64_built = 1; }
@@ -854,14 +1353,19 @@ static int outer_iterate(pps_handle *h, Hier<T> &H, bool mixed, double tol_abs, 
     double *part = f->red.as<double>();
     double *S = part + 4 * kRedBlocks;
     const int nb = nblk(n);
+    const ZGrid zg = zgrid(f->d.nx, f->d.ny, f->d.nz);
+    const dim3 zb(64, 4, 1);
     float *b32 = mixed ? f->b32.as<float>() : nullptr;
     const double N = (double)n;
     const bool cg = h->opt.outer == PPS_OUTER_CG;
     const int nu1 = h->opt.nu_pre < 1 ? 1 : h->opt.nu_pre, nu2 = h->opt.nu_post;
 
-    // r = b - M x
-    k_outer_resid<float><<<nb, 256, 0, s>>>(O, V.b, V.x, V.r, b32, part);
-    k_scal_rr<<<1, 256, 0, s>>>(part, nb, 1, S);
+    // r = b - M x (unless the setup pass just computed it)
+    if (!f->have_r) {
+        k_outer_resid<float><<<zg.grid, zb, 0, s>>>(O, zg.zc, V.b, V.x, V.r, b32, part);
+        k_scal_rr<<<1, 256, 0, s>>>(part, zg.blocks, 1, S);
+    }
+    f->have_r = 0;
     k_fill<<<1, 64, 0, s>>>(S + S_FIRST, 1, 1.0);
     PPS_HIP(hipGetLastError());
     PPS_HIP(hipMemcpyAsync(h->pinned, S + S_RR, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -872,22 +1376,22 @@ static int outer_iterate(pps_handle *h, Hier<T> &H, bool mixed, double tol_abs, 
     if (t2 == 0.0) { *it = 1; return PPS_OK; }   // exact solution: nothing to do
     for (k = 1; k <= maxit; ++k) {
         // z = V(r)
-        T *z = H.lv[0].x;
         vcycle<T>(H, mixed ? (const T *)(const void *)b32 : (const T *)(const void *)V.r, nu1, nu2,
-                  &h->prof, s);
+                  &h->prof, s, f->fused);
+        const T *z = H.lv[0].x;   // the sweeps swap buffers: read after the cycle
         if (cg) {
             k_dot_rz<T><<<nb, 256, 0, s>>>(n, V.r, z, part);
             k_scal_beta<<<1, 256, 0, s>>>(part, nb, n, f->singular, S);
-            k_dq<T><<<nb, 256, 0, s>>>(O, z, V.d[cur], V.d[cur ^ 1], V.q, S, part);
-            k_scal_alpha<<<1, 256, 0, s>>>(part, nb, S);
+            k_dq<T><<<zg.grid, zb, 0, s>>>(O, zg.zc, z, V.d[cur], V.d[cur ^ 1], V.q, S, part);
+            k_scal_alpha<<<1, 256, 0, s>>>(part, zg.blocks, S);
             k_update<float><<<nb, 256, 0, s>>>(n, V.x, V.r, V.d[cur ^ 1], V.q, b32, S, part);
             k_scal_rr<<<1, 256, 0, s>>>(part, nb, 0, S);
             cur ^= 1;
         } else {
             k_zsum<T><<<nb, 256, 0, s>>>(n, z, part);
             k_add_z<T><<<nb, 256, 0, s>>>(n, V.x, z, part, nb, f->singular);
-            k_outer_resid<float><<<nb, 256, 0, s>>>(O, V.b, V.x, V.r, b32, part);
-            k_scal_rr<<<1, 256, 0, s>>>(part, nb, 0, S);
+            k_outer_resid<float><<<zg.grid, zb, 0, s>>>(O, zg.zc, V.b, V.x, V.r, b32, part);
+            k_scal_rr<<<1, 256, 0, s>>>(part, zg.blocks, 0, S);
         }
         PPS_HIP(hipGetLastError());
         PPS_HIP(hipMemcpyAsync(h->pinned, S + S_RR, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -907,6 +1411,9 @@ int fast_solve(pps_handle *h, const double *dA, double *dp, double maxError, int
     PPS_TRY(fast_ensure(h));
     FastState *f = h->fast;
     const bool mixed = h->opt.method == PPS_MG_MIXED;
+    f->fused = h->opt.smoother != PPS_SMOOTH_COLOUR_PASSES;
+    g_sweep_rows = h->opt.sweep_rows;
+    g_sweep_planes = h->opt.sweep_planes;
     h->hist.clear();
     PPS_HIP(hipEventRecord(h->ev[0], s));
     PPS_TRY(fast_setup(h, dA, dp, mixed, s));
@@ -916,22 +1423,37 @@ int fast_solve(pps_handle *h, const double *dA, double *dp, double maxError, int
     int total = 0, k = 0;
     double t2 = 0.0, last = 0.0;
     int rc = PPS_OK;
-    // iterate on the recursive residual, then confirm with the reference's
-    // residual on the caller's A; restart from the true residual if the
-    // recursion drifted
+    // iterate on the recursive residual, then confirm with the true residual
+    // b - M x (norm 2: the FP64 operator, fused with the write-back of p;
+    // other norms: the reference's expression on the caller's A); restart
+    // from the true residual if the recursion drifted
     for (int attempt = 0; attempt < 4 && total < maxit; ++attempt) {
-        if (mixed) rc = outer_iterate<float>(h, f->h32, true, maxError, maxit - total, &k, &last, true, s);
-        else rc = outer_iterate<double>(h, f->h64, false, maxError, maxit - total, &k, &last, true, s);
+        const int budget = maxit - total;
+        if (mixed) rc = outer_iterate<float>(h, f->h32, true, maxError, budget, &k, &last, true, s);
+        else rc = outer_iterate<double>(h, f->h64, false, maxError, budget, &k, &last, true, s);
         if (rc != PPS_OK && rc != PPS_ERR_NOT_CONVERGED) break;
-        total += (k > maxit - total) ? maxit - total : k;
-        k_unpack<<<cgrid3(d.nx, d.ny, d.nz), kB3, 0, s>>>(packk(d), vecs(f).x, dp);
-        PPS_HIP(hipGetLastError());
-        double res = 0.0;
-        PPS_TRY(compute_residual_norm(h, d, dA, dp, norm, &res, s));
-        t2 = res / N;
-        if (norm == 2) t2 = std::sqrt(t2);
+        const bool exhausted = k > budget;
+        total += exhausted ? budget : k;
+        if (norm == 2) {
+            const ZGrid zg = zgrid(d.nx, d.ny, d.nz);
+            double *part = f->red.as<double>();
+            double *S = part + 4 * kRedBlocks;
+            k_check_unpack<<<zg.grid, dim3(64, 4, 1), 0, s>>>(opk(f), packk(d), zg.zc, vecs(f).b,
+                                                               vecs(f).x, dp, part);
+            k_scal_rr<<<1, 256, 0, s>>>(part, zg.blocks, 0, S);
+            PPS_HIP(hipGetLastError());
+            PPS_HIP(hipMemcpyAsync(h->pinned, S + S_RR, sizeof(double), hipMemcpyDeviceToHost, s));
+            PPS_HIP(hipStreamSynchronize(s));
+            t2 = std::sqrt(h->pinned[0] / N);
+        } else {
+            k_unpack<<<cgrid3(d.nx, d.ny, d.nz), kB3, 0, s>>>(packk(d), vecs(f).x, dp);
+            PPS_HIP(hipGetLastError());
+            double res = 0.0;
+            PPS_TRY(compute_residual_norm(h, d, dA, dp, norm, &res, s));
+            t2 = res / N;
+        }
         if (t2 < maxError) break;
-        if (k > maxit) break;
+        if (exhausted) break;
     }
     PPS_HIP(hipEventRecord(h->ev[2], s));
     PPS_HIP(hipEventSynchronize(h->ev[2]));
@@ -976,13 +1498,14 @@ int fast_adjoint(pps_handle *h, const double *dA, const double *dg, double *dlam
     // absolute tolerance from the right-hand side norm
     double *part = f->red.as<double>();
     double *S = part + 4 * kRedBlocks;
-    const int nb = nblk(n);
     const OpK O = opk(f);
-    k_outer_resid<float><<<nb, 256, 0, s>>>(O, V.b, V.x, V.r, nullptr, part);
-    k_scal_rr<<<1, 256, 0, s>>>(part, nb, 1, S);
+    const ZGrid zg = zgrid(d.nx, d.ny, d.nz);
+    k_outer_resid<float><<<zg.grid, dim3(64, 4, 1), 0, s>>>(O, zg.zc, V.b, V.x, V.r, nullptr, part);
+    k_scal_rr<<<1, 256, 0, s>>>(part, zg.blocks, 1, S);
     PPS_HIP(hipMemcpyAsync(h->pinned, S + S_BB, sizeof(double), hipMemcpyDeviceToHost, s));
     PPS_HIP(hipStreamSynchronize(s));
     const double tol = rtol * std::sqrt(h->pinned[0] / (double)n);
+    f->have_r = 0;   // the outer iteration recomputes r (and its FP32 copy) for g
     int k = 0;
     double last = 0.0;
     int rc;

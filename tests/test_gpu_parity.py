@@ -184,6 +184,32 @@ def test_fast_mg_relative_1e10_iterations(pps, orc, method):
     assert t2 < 1e-10 * brms and it <= 20
 
 
+@pytest.mark.parametrize("method,tol", [("fp64", 1e-12), ("mixed", 2e-5)])
+@pytest.mark.parametrize("case", ["walls", "perx", "allper", "aniso"])
+def test_fused_sweep_equals_colour_passes(pps, method, tol, case):
+    """One outer iteration with the fused red-black sweep kernel gives the
+    iterate of the two-colour-pass kernels up to rounding (any indexing or
+    ordering mistake in the fused kernel is an O(1) difference)."""
+    _say("fused vs colour passes", method, case)
+    n, ny, nz, bc = {"walls": (64, 64, 64, (0,) * 6), "perx": (64, 64, 64, (1, 0, 0, 1, 0, 0)),
+                     "allper": (32, 32, 32, (1,) * 6), "aniso": (128, 32, 16, (0, 1, 0, 0, 1, 0))}[case]
+    grid = pps.Grid.make(n, ny, nz, bdry_cond=bc)
+    drops = [(0.3, 0.6, 0.4, 0.2), (0.8, 0.3, 0.7, 0.15)]
+    rho, u, v, w = pps.synth_fields(grid, drops, 1.0, 1000.0, 1.5, seed=12)
+    um, vm, wm = pps.pressure_masks(grid)
+    A = pps.setup_poisson(grid, u, v, w, um, vm, wm, rho, 1.0)
+    pps.remove_rhs_mean(grid, A)
+    out = []
+    for sm in (pps.SMOOTH_FUSED, pps.SMOOTH_COLOUR_PASSES):
+        with pps.Solver(grid, method=method, smoother=sm) as s:
+            p = torch.zeros(grid.shape_p, dtype=torch.float64, device="cuda")
+            s.solve(A, p, 1e-300, 1.2, 1)
+            out.append(_host(grid.interior(p)))
+    d = np.abs(out[0] - out[1]).max() / np.abs(out[1]).max()
+    _say(f"   rel diff {d:.2e}")
+    assert d < tol
+
+
 def test_fast_rejects_nonsymmetric_and_nan(pps):
     _say("fast errors")
     g, A, _ = pps.make_problem(n=16, drops=[(0.5, 0.5, 0.5, 0.25)], rho2=10.0)
