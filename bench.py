@@ -67,6 +67,23 @@ def solve_bytes(n3, iters, S, nu_pre=2, nu_post=2):
     return setup + iters * (v + outer) + final
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_rb_sweep_drops256_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the profiled sweep kernel, from the committed
+    counter summary (FETCH_SIZE + WRITE_SIZE, calibrated against a copy of
+    known size); None when no summary is present."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def gpu_run(pps, torch, grid, A, method, tol, steps, warmup, maxit=200):
     s = pps.Solver(grid, method=method)
     p = torch.zeros(grid.shape_p, dtype=torch.float64, device="cuda")
@@ -155,6 +172,10 @@ def main():
     ap.add_argument("--cpu-cycles", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ref-cap", type=int, default=3000)
+    ap.add_argument("--only", choices=("both", "mixed", "fp64"), default="both",
+                    help="time one form only (for counter runs)")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="run one device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the solver runs on a single MI355X "
@@ -164,10 +185,21 @@ def main():
     import numpy as np
     import torch
     import __graft_entry__ as ge
-    pps = ge.load_package()
     if not torch.cuda.is_available():
         print("no GPU", file=sys.stderr)
         sys.exit(1)
+    if args.copy_calib:
+        # a plain copy of known size: FETCH_SIZE / WRITE_SIZE of this kernel
+        # calibrate the counters (bytes = MiB * 2^20 each way)
+        n = args.copy_calib * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": n * 8}))
+        return
+    pps = ge.load_package()
 
     print(f"[bench] building {args.workload}", file=sys.stderr, flush=True)
     grid, A, _ = pps.make_problem(args.workload)
@@ -176,7 +208,8 @@ def main():
     tol = 1e-10 * brms
 
     res = {}
-    for method in ("fp64", "mixed"):
+    methods = ("fp64", "mixed") if args.only == "both" else (args.only,)
+    for method in methods:
         print(f"[bench] {method}: {args.warmup} warmup + {args.steps} timed solves",
               file=sys.stderr, flush=True)
         res[method] = gpu_run(pps, torch, grid, A, method, tol, args.steps, args.warmup)
@@ -189,17 +222,22 @@ def main():
               f"rel.res {r['tres2'] / brms:.2e}, smoother {r['smoother_GBps']:.0f} GB/s",
               file=sys.stderr, flush=True)
 
+    if args.only != "both":
+        r = res[args.only]
+        print(json.dumps({"form": args.only, "ms": r["ms"], "it": r["it"],
+                          "smoother_GBps": r["smoother_GBps"]}), flush=True)
+        return
     m = res["mixed"]
     f64 = res["fp64"]
     n_cells = grid.cells
     roof = {
-        "kernel": "k_rb<float,float,*,true> (finest-level red-black Gauss-Seidel colour pass, FP32)",
+        "kernel": "k_rb_sweep<float,false,1,true> (finest-level fused red-black Gauss-Seidel sweep, FP32)",
         "bound": "hbm",
         "achieved": round(m["smoother_GBps"], 1),
         "peak": HBM_PEAK_GBPS,
         "unit": "GB/s",
         "frac": round(m["smoother_GBps"] / HBM_PEAK_GBPS, 4),
-        "traffic": None,
+        "traffic": measured_traffic(),
         "launches": m["launches"],
         "avg_launch_us": round(m["kernel_ms"] / max(m["launches"], 1) * 1e3, 2),
         "bytes_per_launch": round(m["kernel_bytes"] / max(m["launches"], 1)),

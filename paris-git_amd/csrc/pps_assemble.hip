@@ -73,7 +73,7 @@ k_final(const double *__restrict__ part, int nblocks, int ncomp, int maxmask,
 
 static int red_blocks(long long n) {
     long long b = (n + 255) / 256;
-    if (b > kRedBlocks) b = kRedBlocks;
+    if (b > 2048) b = 2048;
     if (b < 1) b = 1;
     return (int)b;
 }

@@ -70,7 +70,7 @@ struct DevBuf {
 // Reductions: every reducing kernel writes one partial per block; a tiny
 // single-block kernel sums the partials in a fixed order so results do
 // not depend on scheduling.
-constexpr int kRedBlocks = 8192;   // partial slots per reduction
+constexpr int kRedBlocks = 65536;  // partial slots per reduction (one per block)
 constexpr int kRedThreads = 256;
 
 struct RefState;                   // reference-faithful MG (pps_ref.hip)
