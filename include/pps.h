@@ -98,6 +98,9 @@ typedef struct pps_options {
     int smoother;          /* enum pps_smoother (fast methods, default fused) */
     int sweep_rows;        /* fused sweep: thread rows per block, 0 = automatic */
     int sweep_planes;      /* fused sweep: planes per block, 0 = automatic */
+    int tail_cells;        /* fast V-cycle: levels below the finest with at most
+                              this many cells run in one workgroup;
+                              0 = automatic (4096), < 0 = never */
 } pps_options;
 
 typedef struct pps_handle pps_handle;

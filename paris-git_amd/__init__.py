@@ -110,7 +110,8 @@ class Options(ctypes.Structure):
                 ("sym_mg", ctypes.c_int), ("nu_pre", ctypes.c_int), ("nu_post", ctypes.c_int),
                 ("check_setup", ctypes.c_int), ("multigrid", ctypes.c_int),
                 ("smoother", ctypes.c_int), ("sweep_rows", ctypes.c_int),
-                ("sweep_planes", ctypes.c_int)]
+                ("sweep_planes", ctypes.c_int),
+                ("tail_cells", ctypes.c_int)]
 
 
 _LIBH = None
@@ -222,7 +223,7 @@ class Solver:
 
     def __init__(self, grid, method="fp64", outer=OUTER_CG, nrelax=3, sym_mg=0, nu_pre=2,
                  nu_post=2, check_setup=0, multigrid=1, smoother=SMOOTH_FUSED, sweep_rows=0,
-                 sweep_planes=0):
+                 sweep_planes=0, tail_cells=0):
         self.grid = grid
         o = default_options()
         o.method = METHODS[method] if isinstance(method, str) else int(method)
@@ -231,6 +232,7 @@ class Solver:
         o.check_setup, o.multigrid = int(check_setup), int(multigrid)
         o.smoother = int(smoother)
         o.sweep_rows, o.sweep_planes = int(sweep_rows), int(sweep_planes)
+        o.tail_cells = int(tail_cells)
         self.options = o
         h = ctypes.c_void_p()
         _check(_lib().pps_init(ctypes.byref(grid), ctypes.byref(o), ctypes.byref(h)), "pps_init")

@@ -40,6 +40,7 @@ extern "C" int pps_default_options(pps_options *o) {
     o->smoother = PPS_SMOOTH_FUSED;
     o->sweep_rows = 0;
     o->sweep_planes = 0;
+    o->tail_cells = 0;
     return PPS_OK;
 }
 

@@ -25,6 +25,7 @@
 // runs on FP32 copies of the operator with FP32 vectors and accumulation;
 // the outer CG (residual, update, dot products, mean projection) is FP64.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -61,13 +62,32 @@ __device__ __forceinline__ T nb_sum(const LevelK<T> &L, const V *__restrict__ x,
     return s;
 }
 
+// Diagonal of a row whose excess A7 - sum(couplings) is zero (every row
+// of a singular system): the sum of its six couplings, in one fixed order
+// shared by every kernel of the V-cycle.
+template <class T>
+__device__ __forceinline__ T diag_sum(T cxm, T cx, T cym, T cy, T czm, T cz) {
+    return ((cxm + cx) + (cym + cy)) + (czm + cz);
+}
+
+// Diagonal of cell c: stored (DGI false) or the coupling sum (DGI true).
+template <class T, bool DGI>
+__device__ __forceinline__ T diag_of(const LevelK<T> &L, int i, int j, int k, long long c) {
+    if (!DGI) return L.dg[c];
+    const long long sy = L.nx, sz = (long long)L.nx * L.ny;
+    const long long cxm = c + (wrap_m(i, L.nx) - i);
+    const long long cym = c + (long long)(wrap_m(j, L.ny) - j) * sy;
+    const long long czm = c + (long long)(wrap_m(k, L.nz) - k) * sz;
+    return diag_sum(L.cx[cxm], L.cx[c], L.cy[cym], L.cy[c], L.cz[czm], L.cz[c]);
+}
+
 // ---------------------------------------------------------------------
 // red-black Gauss-Seidel colour pass.  colour 0 = (i+j+k) even.  FIRST:
 // the iterate is zero before this pass (no read of x).
 // ---------------------------------------------------------------------
 // FINE does not change the code; it gives the finest level's launches
 // their own symbol so profilers can separate them from coarse levels.
-template <class T, class B, bool FIRST, bool FINE>
+template <class T, class B, bool FIRST, bool FINE, bool DGI>
 __global__ void __launch_bounds__(256)
 k_rb(LevelK<T> L, const B *__restrict__ b, T *__restrict__ x, int colour) {
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -75,10 +95,11 @@ k_rb(LevelK<T> L, const B *__restrict__ b, T *__restrict__ x, int colour) {
     const int i = 2 * (int)(blockIdx.x * blockDim.x + threadIdx.x) + ((j + k + colour) & 1);
     if (j >= L.ny || i >= L.nx) return;
     const long long c = i + (long long)L.nx * (j + (long long)L.ny * k);
+    const T d = diag_of<T, DGI>(L, i, j, k, c);
     if (FIRST) {
-        x[c] = (T)b[c] / L.dg[c];
+        x[c] = (T)b[c] / d;
     } else {
-        x[c] = ((T)b[c] + nb_sum(L, x, i, j, k, c)) / L.dg[c];
+        x[c] = ((T)b[c] + nb_sum(L, x, i, j, k, c)) / d;
     }
 }
 
@@ -137,7 +158,8 @@ __device__ __forceinline__ CellOp<T> pick(const PairOp<T> &P, int e) {
 
 // FINE does not change the code; it gives the finest level's launches
 // their own symbol so profilers can separate them from coarse levels.
-template <class T, bool FIRST, int PX, bool FINE>
+// DGI: the diagonal is the coupling sum (not read).
+template <class T, bool FIRST, int PX, bool FINE, bool DGI, bool KEEP>
 __global__ void __launch_bounds__(1024)
 k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *__restrict__ xout,
            int c1, int zc, int oy, int ytiles) {
@@ -174,7 +196,7 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
         P.cya = L.cy[c]; P.cyb = L.cy[c + 1];
         P.csa = L.cy[c + dym]; P.csb = L.cy[c + dym + 1];
         P.cza = L.cz[c]; P.czb = L.cz[c + 1];
-        P.dga = L.dg[c]; P.dgb = L.dg[c + 1];
+        if (!DGI) { P.dga = L.dg[c]; P.dgb = L.dg[c + 1]; }
         P.ba = b[c]; P.bb = b[c + 1];
     };
     // second-colour xin of the extra row (ring threads only)
@@ -193,11 +215,15 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
 
     // per pair: xin at planes s-1 .. s+2 (cells a = 2m, b = 2m+1), first
     // colour new at s-2 .. s, operator of planes s (cur) and s+1 (nxt).
-    // The second-colour cell's operator of plane s-1 and the -z couplings
-    // are read again where needed (cache hits: loaded one step earlier).
+    // KEEP: the operator of plane s-1 (prv) and the +z couplings of plane
+    // s-2 (pza, pzb) stay in registers for the second-colour update and the
+    // -z couplings; otherwise they are read again from memory, where they
+    // were loaded one step earlier (fewer registers, about 1.4x the fetched
+    // bytes of the FP32 sweep at 256^3).
     T xa0[PX], xb0[PX], xa1[PX], xb1[PX], xa2[PX], xb2[PX], xa3[PX], xb3[PX], ex2[PX], ex3[PX];
     T r_m2[PX], r_m1[PX], r_0[PX];
-    PairOp<T> cur[PX], nxt[PX];
+    PairOp<T> cur[PX], nxt[PX], prv[PX];
+    T pza[PX], pzb[PX];
 
     // prologue: xin planes k0-2 .. k0 (+ extra row of k0), operator of
     // k0-1 and the +z couplings of k0-2; stage plane k0-1
@@ -215,6 +241,12 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
         }
         ex2[p] = load_extra(k0, m);
         load_op(k0 - 1, m, cur[p]);
+        if (KEEP) {
+            const long long cz2 = col(j, wrapz(k0 - 2), m);
+            prv[p].cza = L.cz[cz2];
+            prv[p].czb = L.cz[cz2 + 1];
+            pza[p] = pzb[p] = (T)0;
+        }
         r_m2[p] = r_m1[p] = r_0[p] = (T)0;
         stage(k0 - 1, m, xa1[p], xb1[p], load_extra(k0 - 1, m));
     }
@@ -246,18 +278,20 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
             const int m = threadIdx.x + p * tpr;
             const CellOp<T> o = pick(cur[p], e1);
             T v = o.b;
+            T zm = (T)0;
+            if (KEEP) zm = e1 ? prv[p].czb : prv[p].cza;
+            else if (!FIRST || DGI) zm = L.cz[col(j, wrapz(s - 1), m) + e1];
             if (!FIRST) {
                 const int mm = m == 0 ? np - 1 : m - 1, mp = m == np - 1 ? 0 : m + 1;
                 const T xw = e1 == 0 ? X(bs, jj + 1, mm) : xa1[p];
                 const T xe = e1 == 0 ? xb1[p] : X(bs, jj + 1, mp);
-                const T zm = L.cz[col(j, wrapz(s - 1), m) + e1];
                 const T xbz = e1 ? xb0[p] : xa0[p];
                 const T xtz = e1 ? xb2[p] : xa2[p];
                 v += o.cxm * xw + o.cx * xe;
                 v += o.cym * X(bs, jj, m) + o.cy * X(bs, jj + 2, m);
                 v += zm * xbz + o.cz * xtz;
             }
-            v /= o.dg;
+            v /= DGI ? diag_sum(o.cxm, o.cx, o.cym, o.cy, zm, o.cz) : o.dg;
             R(bs, jj, m) = v;
             r_m2[p] = r_m1[p];
             r_m1[p] = r_0[p];
@@ -272,13 +306,21 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
             for (int p = 0; p < PX; ++p) {
                 const int m = threadIdx.x + p * tpr;
                 const int mm = m == 0 ? np - 1 : m - 1, mp = m == np - 1 ? 0 : m + 1;
-                const long long c = col(j, wrapz(t), m) + e2;
-                const long long cxm = (m == 0 && e2 == 0) ? c + (nx - 1) : c - 1;
                 CellOp<T> o;
-                o.cx = L.cx[c]; o.cxm = L.cx[cxm];
-                o.cy = L.cy[c]; o.cym = L.cy[c + dym];
-                o.cz = L.cz[c]; o.dg = L.dg[c]; o.b = b[c];
-                const T zm = L.cz[col(j, wrapz(t - 1), m) + e2];
+                T zm;
+                if (KEEP) {
+                    o = pick(prv[p], e2);
+                    zm = e2 ? pzb[p] : pza[p];
+                } else {
+                    const long long c = col(j, wrapz(t), m) + e2;
+                    const long long cxm = (m == 0 && e2 == 0) ? c + (nx - 1) : c - 1;
+                    o.cx = L.cx[c]; o.cxm = L.cx[cxm];
+                    o.cy = L.cy[c]; o.cym = L.cy[c + dym];
+                    o.cz = L.cz[c]; o.b = b[c];
+                    if (!DGI) o.dg = L.dg[c];
+                    zm = L.cz[col(j, wrapz(t - 1), m) + e2];
+                }
+                if (DGI) o.dg = diag_sum(o.cxm, o.cx, o.cym, o.cy, zm, o.cz);
                 const T xw = e2 == 0 ? R(bt, jj, mm) : r_m1[p];
                 const T xe = e2 == 0 ? r_m1[p] : R(bt, jj, mp);
                 T v = o.b;
@@ -296,6 +338,11 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
         for (int p = 0; p < PX; ++p) {
             const int m = threadIdx.x + p * tpr;
             if (more) stage(s + 1, m, xa2[p], xb2[p], ex2[p]);
+            if (KEEP) {
+                pza[p] = prv[p].cza;
+                pzb[p] = prv[p].czb;
+                prv[p] = cur[p];
+            }
             cur[p] = nxt[p];
             xa0[p] = xa1[p]; xb0[p] = xb1[p];
             xa1[p] = xa2[p]; xb1[p] = xb2[p];
@@ -307,7 +354,7 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
 }
 
 // residual of the 8 children of each coarse cell, averaged into bc
-template <class T, class B>
+template <class T, class B, bool DGI>
 __global__ void __launch_bounds__(256)
 k_resid_restrict(LevelK<T> L, const B *__restrict__ b, const T *__restrict__ x,
                  T *__restrict__ bc) {
@@ -325,7 +372,7 @@ k_resid_restrict(LevelK<T> L, const B *__restrict__ b, const T *__restrict__ x,
             for (int di = 0; di < 2; ++di) {
                 const int i = 2 * I + di, j = 2 * J + dj, k = 2 * K + dk;
                 const long long c = i + (long long)L.nx * (j + (long long)L.ny * k);
-                acc += ((T)b[c] + nb_sum(L, x, i, j, k, c)) - L.dg[c] * x[c];
+                acc += ((T)b[c] + nb_sum(L, x, i, j, k, c)) - diag_of<T, DGI>(L, i, j, k, c) * x[c];
             }
     bc[I + (long long)cxn * (J + (long long)cyn * K)] = acc * (T)0.125;
 }
@@ -519,7 +566,7 @@ k_bottom_build(LevelK<T> L, int singular, double *__restrict__ M, T *__restrict_
         const int ip = wrap_p(i, L.nx), jp = wrap_p(j, L.ny), kp = wrap_p(k, L.nz);
         const int im = wrap_m(i, L.nx), jm = wrap_m(j, L.ny), km = wrap_m(k, L.nz);
         double *row = M + (long long)c * nb;
-        row[c] += (double)L.dg[c];
+        row[c] += singular ? (double)diag_of<T, true>(L, i, j, k, c) : (double)L.dg[c];
         row[c + (ip - i)] -= (double)L.cx[c];
         row[c + (im - i)] -= (double)L.cx[c + (im - i)];
         row[c + (jp - j) * sy] -= (double)L.cy[c];
@@ -571,6 +618,120 @@ k_bottom_solve(int nb, const T *__restrict__ Minv, const T *__restrict__ b, T *_
 }
 
 // ---------------------------------------------------------------------
+// The coarse end of a V-cycle in one workgroup.  Levels 0 .. nl-1 of the
+// tail (level nl-1 is the coarsest, solved with the dense inverse) are
+// small enough to stay in L2; colour passes, restriction, the bottom
+// solve and prolongation run back to back with workgroup barriers in
+// between, so the whole tail costs one launch instead of several per
+// level.  The arithmetic is that of k_rb / k_resid_restrict /
+// k_bottom_solve / k_prolong_add (in-place colour passes: pre-smoothing
+// colour 0 first, post-smoothing colour 1 first).
+// ---------------------------------------------------------------------
+constexpr int kTailMax = 8;
+template <class T> struct TailK {
+    int nl, nb, nu_pre, nu_post;
+    int per[3];
+    LevelK<T> L[kTailMax];
+    T *b[kTailMax];
+    T *x[kTailMax];
+    const T *minv;
+};
+
+template <class T, bool DGI>
+__global__ void __launch_bounds__(1024)
+k_tail(TailK<T> K) {
+    const int tid = threadIdx.x;
+    constexpr int NT = 1024;
+    auto cpass = [&](int l, int colour, bool first) {
+        const LevelK<T> &L = K.L[l];
+        const T *b = K.b[l];
+        T *x = K.x[l];
+        const int nxh = (L.nx + 1) >> 1;
+        const int total = nxh * L.ny * L.nz;
+        for (int e = tid; e < total; e += NT) {
+            const int ih = e % nxh, j = (e / nxh) % L.ny, k = e / (nxh * L.ny);
+            const int i = 2 * ih + ((j + k + colour) & 1);
+            if (i >= L.nx) continue;
+            const long long c = i + (long long)L.nx * (j + (long long)L.ny * k);
+            const T d = diag_of<T, DGI>(L, i, j, k, c);
+            x[c] = first ? b[c] / d : (b[c] + nb_sum(L, x, i, j, k, c)) / d;
+        }
+        __syncthreads();
+    };
+    // pre-smoothing and restriction down to the coarsest level
+    for (int l = 0; l < K.nl - 1; ++l) {
+        cpass(l, 0, true);
+        cpass(l, 1, false);
+        for (int q = 1; q < K.nu_pre; ++q) {
+            cpass(l, 0, false);
+            cpass(l, 1, false);
+        }
+        const LevelK<T> &L = K.L[l];
+        const T *b = K.b[l];
+        const T *x = K.x[l];
+        const int cxn = L.nx >> 1, cyn = L.ny >> 1, czn = L.nz >> 1;
+        T *bc = K.b[l + 1];
+        for (int e = tid; e < cxn * cyn * czn; e += NT) {
+            const int I = e % cxn, J = (e / cxn) % cyn, Kz = e / (cxn * cyn);
+            T acc = 0;
+            for (int dk = 0; dk < 2; ++dk)
+                for (int dj = 0; dj < 2; ++dj)
+                    for (int di = 0; di < 2; ++di) {
+                        const int i = 2 * I + di, j = 2 * J + dj, k = 2 * Kz + dk;
+                        const long long c = i + (long long)L.nx * (j + (long long)L.ny * k);
+                        acc += (b[c] + nb_sum(L, x, i, j, k, c)) - diag_of<T, DGI>(L, i, j, k, c) * x[c];
+                    }
+            bc[e] = acc * (T)0.125;
+        }
+        __syncthreads();
+    }
+    // dense solve on the coarsest level
+    {
+        const int nb = K.nb;
+        const T *b = K.b[K.nl - 1];
+        T *x = K.x[K.nl - 1];
+        for (int r = tid; r < nb; r += NT) {
+            T acc = 0;
+            const T *row = K.minv + (long long)r * nb;
+            for (int e = 0; e < nb; ++e) acc += row[e] * b[e];
+            x[r] = acc;
+        }
+        __syncthreads();
+    }
+    // prolongation and post-smoothing back up
+    for (int l = K.nl - 2; l >= 0; --l) {
+        const LevelK<T> &L = K.L[l];
+        const int fx = L.nx, fy = L.ny, fz = L.nz;
+        const int cx = fx >> 1, cy = fy >> 1, cz = fz >> 1;
+        T *xf = K.x[l];
+        const T *xc = K.x[l + 1];
+        for (int e = tid; e < fx * fy * fz; e += NT) {
+            const int i = e % fx, j = (e / fx) % fy, k = e / (fx * fy);
+            const int I = i >> 1, J = j >> 1, Kz = k >> 1;
+            auto other = [](int f, int C, int nC, int per) {
+                int o = (f & 1) ? C + 1 : C - 1;
+                if (o < 0) o = per ? nC - 1 : 0;
+                if (o >= nC) o = per ? 0 : nC - 1;
+                return o;
+            };
+            const int I2 = other(i, I, cx, K.per[0]), J2 = other(j, J, cy, K.per[1]),
+                      K2 = other(k, Kz, cz, K.per[2]);
+            const long long sy = cx, sz = (long long)cx * cy;
+            const T w1 = (T)0.75, w2 = (T)0.25;
+            auto X = [&](int a, int b_, int c_) { return xc[a + sy * b_ + sz * c_]; };
+            const T v = w1 * (w1 * (w1 * X(I, J, Kz) + w2 * X(I2, J, Kz)) + w2 * (w1 * X(I, J2, Kz) + w2 * X(I2, J2, Kz))) +
+                        w2 * (w1 * (w1 * X(I, J, K2) + w2 * X(I2, J, K2)) + w2 * (w1 * X(I, J2, K2) + w2 * X(I2, J2, K2)));
+            xf[e] += v;
+        }
+        __syncthreads();
+        for (int q = 0; q < K.nu_post; ++q) {
+            cpass(l, 1, false);
+            cpass(l, 0, false);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------
 // FP64 outer iteration kernels.  Scalars live on the device:
 // ---------------------------------------------------------------------
 enum { S_RZ = 0, S_ZMEAN, S_BETA, S_DQ, S_ALPHA, S_RR, S_BB, S_FIRST, S_N };
@@ -580,18 +741,22 @@ __device__ __forceinline__ double wsum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
     return v;
 }
-// block of 256: writes the block sum of v to part[blockIdx.x]
-// Works for any 256-thread block shape whose x extent is a multiple of 64
-// (each wavefront is 64 consecutive linear thread ids); the partial goes to
-// the block's linear id.
+// Writes the block sum of v to part[block linear id].  Any block of at most
+// 1024 threads whose x extent is a multiple of 64 (each wavefront is 64
+// consecutive linear thread ids); wavefront sums are added in a fixed order.
 __device__ __forceinline__ void block_partial(double v, double *part) {
-    __shared__ double sm[4];
+    __shared__ double sm[16];
     const int t = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
     const int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const int nw = (blockDim.x * blockDim.y * blockDim.z) >> 6;
     v = wsum(v);
     if ((t & 63) == 0) sm[t >> 6] = v;
     __syncthreads();
-    if (t == 0) part[b] = (sm[0] + sm[1]) + (sm[2] + sm[3]);
+    if (t == 0) {
+        double a = 0.0;
+        for (int w = 0; w < nw; ++w) a += sm[w];
+        part[b] = a;
+    }
     __syncthreads();
 }
 
@@ -726,47 +891,128 @@ k_scal_beta(const double *__restrict__ part, int nb, long long n, int singular,
     }
 }
 
-// d_new = (z - zmean) + beta d_old ; q = M d_new ; partial sum d.q.
-// z-marching (launch with zgrid()); the d_new values of the vertical
-// neighbours are carried in registers, the in-plane ones recomputed from
-// cached z and d_old.
+// Tiled launch shape of the FP64 outer stencil kernels: blocks of 64 x 16
+// threads own a 64 x 16 column of cells and march zc planes.
+struct TGrid {
+    int tx, ty, zc, blocks;
+};
+static TGrid tgrid(int nx, int ny, int nz, int target = 1024) {
+    TGrid g;
+    g.tx = (nx + 63) / 64;
+    g.ty = (ny + 15) / 16;
+    const long long cols = (long long)g.tx * g.ty;
+    long long chunks = target / cols;
+    if (chunks * cols > kRedBlocks) chunks = kRedBlocks / cols;
+    if (chunks < 1) chunks = 1;
+    if (chunks > nz) chunks = nz;
+    g.zc = (int)((nz + chunks - 1) / chunks);
+    chunks = (nz + g.zc - 1) / g.zc;
+    g.blocks = (int)(cols * chunks);
+    return g;
+}
+
+// d_new = (z - zmean) + beta d_old ; q = M d_new ; partial sum d.q, on
+// LDS tiles: d_new of every plane (own cells plus a one-cell halo
+// ring, coordinates wrapped) is written to one of three LDS planes; the
+// in-plane neighbours of q = M d_new are read there, the vertical ones
+// from registers.  The raw z / d_old of plane s+2 and the operator of
+// plane s are requested at the start of step s, so the loads overlap the
+// arithmetic of the step; one barrier per plane (three buffers: the plane
+// written at step s+1 was last read at step s-2).
 template <class TZ>
-__global__ void __launch_bounds__(256)
-k_dq(OpK L, int zc, const TZ *__restrict__ z, const double *__restrict__ dold,
-     double *__restrict__ dnew, double *__restrict__ q, const double *__restrict__ S,
-     double *__restrict__ part) {
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * 4 + threadIdx.y;
-    const int k0 = blockIdx.z * zc;
-    const int k1 = min(k0 + zc, L.nz);
+__global__ void __launch_bounds__(1024)
+k_dq_tiled(OpK L, TGrid g, const TZ *__restrict__ z, const double *__restrict__ dold,
+           double *__restrict__ dnew, double *__restrict__ q, const double *__restrict__ S,
+           double *__restrict__ part) {
+    __shared__ double D[3][18][66];
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const int bid = blockIdx.x;
+    const int tx = bid % g.tx, ty = (bid / g.tx) % g.ty, ch = bid / (g.tx * g.ty);
+    const int li = threadIdx.x, lj = threadIdx.y;
+    const int i = tx * 64 + li, j = ty * 16 + lj;
+    const int k0 = ch * g.zc, k1 = min(k0 + g.zc, nz);
+    const bool in = i < nx && j < ny;
     const double beta = S[S_BETA], zm = S[S_ZMEAN];
     const bool use_old = beta != 0.0;
-    double dq = 0.0;
-    if (i < L.nx && j < L.ny) {
-        auto dv = [&](long long e) {
-            const double v = (double)z[e] - zm;
-            return use_old ? v + beta * dold[e] : v;
-        };
-        const long long sy = L.nx, sz = (long long)L.nx * L.ny;
-        const int dxm = wrap_m(i, L.nx) - i, dxp = wrap_p(i, L.nx) - i;
-        const long long dym = (long long)(wrap_m(j, L.ny) - j) * sy, dyp = (long long)(wrap_p(j, L.ny) - j) * sy;
-        const long long col = i + sy * j;
-        double dm = dv(col + sz * wrap_m(k0, L.nz));
-        double d0 = dv(col + sz * k0);
-        for (int k = k0; k < k1; ++k) {
-            const long long c = col + sz * k;
-            const long long czm = col + sz * wrap_m(k, L.nz);
-            const double dp = dv(col + sz * wrap_p(k, L.nz));
-            double s = L.cx[c + dxm] * dv(c + dxm) + L.cx[c] * dv(c + dxp);
-            s += L.cy[c + dym] * dv(c + dym) + L.cy[c] * dv(c + dyp);
-            s += L.cz[czm] * dm + L.cz[c] * dp;
-            const double qv = L.dg[c] * d0 - s;
-            dnew[c] = d0;
-            q[c] = qv;
-            dq += d0 * qv;
-            dm = d0;
-            d0 = dp;
+    const long long sy = nx, sz = (long long)nx * ny;
+    auto wrapn = [](int a, int n) { a %= n; return a < 0 ? a + n : a; };
+    auto wrapk = [&](int k) { return wrapn(k, nz); };
+    // own slot (li+1, lj+1) holds cell (wrap i, wrap j); threads 0..163 also
+    // fill one halo slot: rows -1 / 16 (66 slots each), columns -1 / 64
+    const long long own = wrapn(i, nx) + sy * wrapn(j, ny);
+    const int t = li + 64 * lj;
+    int hx = 0, hy = 0;
+    if (t < 66) { hx = t - 1; hy = -1; }
+    else if (t < 132) { hx = t - 67; hy = 16; }
+    else if (t < 148) { hx = -1; hy = t - 132; }
+    else { hx = 64; hy = t - 148; }
+    const bool halo = t < 164;
+    const long long hcol = wrapn(tx * 64 + hx, nx) + sy * wrapn(ty * 16 + hy, ny);
+    auto dval = [&](TZ zr, double dr) {
+        const double v = (double)zr - zm;
+        return use_old ? v + beta * dr : v;
+    };
+    // in-plane operator offsets of the own cell
+    const long long dxm = in ? (long long)(wrap_m(i, nx) - i) : 0;
+    const long long dym = in ? (long long)(wrap_m(j, ny) - j) * sy : 0;
+
+    // prologue: d_new at k0-1 (register), k0 (register + LDS); raw of k0+1
+    double d_m = 0.0, d_0 = 0.0;
+    TZ zo1 = 0, zh1 = 0;
+    double do1 = 0.0, dh1 = 0.0;
+    {
+        const long long pm = sz * wrapk(k0 - 1), p0 = sz * wrapk(k0), p1 = sz * wrapk(k0 + 1);
+        d_m = dval(z[own + pm], use_old ? dold[own + pm] : 0.0);
+        d_0 = dval(z[own + p0], use_old ? dold[own + p0] : 0.0);
+        D[k0 % 3][lj + 1][li + 1] = d_0;
+        if (halo) D[k0 % 3][hy + 1][hx + 1] = dval(z[hcol + p0], use_old ? dold[hcol + p0] : 0.0);
+        zo1 = z[own + p1];
+        if (use_old) do1 = dold[own + p1];
+        if (halo) {
+            zh1 = z[hcol + p1];
+            if (use_old) dh1 = dold[hcol + p1];
         }
+    }
+    double czm = in ? L.cz[own + sz * wrapk(k0 - 1)] : 0.0;
+    __syncthreads();
+    double dq = 0.0;
+    for (int s = k0; s < k1; ++s) {
+        // requests: raw z / d_old of plane s+2, operator of plane s
+        TZ zo2 = 0, zh2 = 0;
+        double do2 = 0.0, dh2 = 0.0;
+        if (s + 1 < k1) {
+            const long long p2 = sz * wrapk(s + 2);
+            zo2 = z[own + p2];
+            if (use_old) do2 = dold[own + p2];
+            if (halo) {
+                zh2 = z[hcol + p2];
+                if (use_old) dh2 = dold[hcol + p2];
+            }
+        }
+        const long long c = own + sz * s;
+        double cxw = 0.0, cxe = 0.0, cys = 0.0, cyn = 0.0, czp = 0.0, dgc = 0.0;
+        if (in) {
+            cxw = L.cx[c + dxm]; cxe = L.cx[c];
+            cys = L.cy[c + dym]; cyn = L.cy[c];
+            czp = L.cz[c]; dgc = L.dg[c];
+        }
+        // d_new of plane s+1 into LDS
+        const int wb = (s + 1) % 3, rb = s % 3;
+        const double d_p = dval(zo1, do1);
+        D[wb][lj + 1][li + 1] = d_p;
+        if (halo) D[wb][hy + 1][hx + 1] = dval(zh1, dh1);
+        if (in) {
+            const double sx = cxw * D[rb][lj + 1][li] + cxe * D[rb][lj + 1][li + 2];
+            const double syv = cys * D[rb][lj][li + 1] + cyn * D[rb][lj + 2][li + 1];
+            const double szv = czm * d_m + czp * d_p;
+            const double qv = dgc * d_0 - ((sx + syv) + szv);
+            dnew[c] = d_0;
+            q[c] = qv;
+            dq += d_0 * qv;
+        }
+        d_m = d_0; d_0 = d_p; czm = czp;
+        zo1 = zo2; do1 = do2; zh1 = zh2; dh1 = dh2;
+        __syncthreads();
     }
     block_partial(dq, part);
 }
@@ -1030,7 +1276,6 @@ struct FastState {
     Hier<float> h32;
     DevBuf b32;      // FP32 right-hand side of the mixed V-cycle
     int mixed_built = 0, fp64_built = 0;
-    int fused = 1;   // one kernel per red-black sweep
     int have_r = 0;  // the setup pass already wrote r (and its norms) for x
 };
 
@@ -1121,22 +1366,28 @@ static void hier_build(Hier<T> &H, int singular, hipStream_t s) {
 // updated colour, the other colour's iterate (read) and the updated
 // colour's iterate (write; plus read unless FIRST):
 //   FIRST: (3 + 1/2 + 1/2 + 1/2) n S ;  else (3 + 1/2 + 1/2 + 1/2 + 1/2 + 1/2) n S
-template <class T>
-static void rb_pass(const Level<T> &L, int l, const T *b, int colour, bool first, Prof *prof,
-                    hipStream_t s) {
+template <class T, bool DGI>
+static void rb_pass_t(const Level<T> &L, int l, const T *b, int colour, bool first, Prof *prof,
+                      hipStream_t s) {
     const LevelK<T> K = L.k();
     const dim3 g2((unsigned)(((L.nx + 1) / 2 + 63) / 64), (unsigned)((L.ny + 3) / 4), (unsigned)L.nz);
     if (l != 0) {
-        if (first) k_rb<T, T, true, false><<<g2, kB3, 0, s>>>(K, b, L.x, colour);
-        else k_rb<T, T, false, false><<<g2, kB3, 0, s>>>(K, b, L.x, colour);
+        if (first) k_rb<T, T, true, false, DGI><<<g2, kB3, 0, s>>>(K, b, L.x, colour);
+        else k_rb<T, T, false, false, DGI><<<g2, kB3, 0, s>>>(K, b, L.x, colour);
         return;
     }
-    const double bytes = (double)L.n * sizeof(T) * (first ? 4.5 : 5.5);
+    const double bytes = (double)L.n * sizeof(T) * ((first ? 4.5 : 5.5) - (DGI ? 0.5 : 0.0));
     hipEvent_t *e = prof ? prof->begin(bytes) : nullptr;
     if (e) (void)hipEventRecord(e[0], s);
-    if (first) k_rb<T, T, true, true><<<g2, kB3, 0, s>>>(K, b, L.x, colour);
-    else k_rb<T, T, false, true><<<g2, kB3, 0, s>>>(K, b, L.x, colour);
+    if (first) k_rb<T, T, true, true, DGI><<<g2, kB3, 0, s>>>(K, b, L.x, colour);
+    else k_rb<T, T, false, true, DGI><<<g2, kB3, 0, s>>>(K, b, L.x, colour);
     if (e) (void)hipEventRecord(e[1], s);
+}
+template <class T>
+static void rb_pass(const Level<T> &L, int l, const T *b, int colour, bool first, bool dgi,
+                    Prof *prof, hipStream_t s) {
+    if (dgi) rb_pass_t<T, true>(L, l, b, colour, first, prof, s);
+    else rb_pass_t<T, false>(L, l, b, colour, first, prof, s);
 }
 
 // Launch shape of the fused sweep on a level: pairs per thread, thread
@@ -1172,37 +1423,55 @@ static SweepShape sweep_shape(int nx, int ny, int nz, int max_rows, int planes) 
     return sh;
 }
 
-// block-shape overrides of the fused sweep for the current solve (0 = auto)
-static int g_sweep_rows = 0, g_sweep_planes = 0;
+// How a V-cycle runs (from the handle's options and the system):
+//   nu_pre, nu_post  smoothing sweeps;
+//   dgi              every diagonal is the coupling sum (singular systems);
+//   fused            one kernel per red-black sweep (k_rb_sweep) instead of
+//                    one per colour (k_rb);
+//   rows, planes     fused-sweep block shape overrides (0 = automatic);
+//   tail_cells       levels below the finest with at most this many cells
+//                    form the one-workgroup tail (0: no tail).
+struct Cycle {
+    int nu_pre = 2, nu_post = 2;
+    bool dgi = false, fused = true;
+    int rows = 0, planes = 0;
+    long long tail_cells = 4096;
+};
 
 // One fused red-black sweep of level l (out of place, the iterate pointers
-// swap).  Algorithmic bytes: the 4 operator planes, b, the old iterate
-// (unless FIRST) and the new iterate, once each: (6 or 7) n S.
-template <class T>
-static void rb_pass(const Level<T> &L, int l, const T *b, int colour, bool first, Prof *prof,
-                    hipStream_t s);
+// swap).  Algorithmic bytes: the operator planes (3 couplings, plus the
+// diagonal unless dgi), b, the old iterate (unless FIRST) and the new
+// iterate, once each.
+template <class T, bool DGI, bool KEEP>
+static void sweep_launch(const SweepShape &sh, const Level<T> &L, int l, const T *b, int c1, bool first,
+                         hipStream_t s) {
+    const LevelK<T> K = L.k();
+    if (first) {
+        if (l == 0) k_rb_sweep<T, true, 1, true, DGI, KEEP><<<sh.grid, sh.block, sh.lds, s>>>(K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles);
+        else k_rb_sweep<T, true, 1, false, DGI, KEEP><<<sh.grid, sh.block, sh.lds, s>>>(K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles);
+    } else {
+        if (l == 0) k_rb_sweep<T, false, 1, true, DGI, KEEP><<<sh.grid, sh.block, sh.lds, s>>>(K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles);
+        else k_rb_sweep<T, false, 1, false, DGI, KEEP><<<sh.grid, sh.block, sh.lds, s>>>(K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles);
+    }
+}
 
 template <class T>
-static void rb_sweep(Level<T> &L, int l, const T *b, int c1, bool first, Prof *prof, hipStream_t s) {
-    const SweepShape sh = sweep_shape<T>(L.nx, L.ny, L.nz, g_sweep_rows, g_sweep_planes);
+static void rb_sweep(Level<T> &L, int l, const T *b, int c1, bool first, const Cycle &cy, Prof *prof,
+                     hipStream_t s) {
+    const bool dgi = cy.dgi;
+    const SweepShape sh = sweep_shape<T>(L.nx, L.ny, L.nz, cy.rows, cy.planes);
     if (sh.rb < 3 || sh.lds > 65536 || (L.nx & 1) || sh.tpr * sh.rb > 1024) {
         // rows too long for the LDS planes: two colour passes instead
-        rb_pass(L, l, b, c1, first, prof, s);
-        rb_pass(L, l, b, 1 - c1, false, prof, s);
+        rb_pass(L, l, b, c1, first, dgi, prof, s);
+        rb_pass(L, l, b, 1 - c1, false, dgi, prof, s);
         return;
     }
-    const LevelK<T> K = L.k();
     // the profiled kernel: finest level, not the first sweep of a cycle
-    const double bytes = (double)L.n * sizeof(T) * 7.0;
+    const double bytes = (double)L.n * sizeof(T) * (dgi ? 6.0 : 7.0);
     hipEvent_t *e = (l == 0 && !first && prof) ? prof->begin(bytes) : nullptr;
     if (e) (void)hipEventRecord(e[0], s);
-    if (first) {
-        if (l == 0) k_rb_sweep<T, true, 1, true><<<sh.grid, sh.block, sh.lds, s>>>(K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles);
-        else k_rb_sweep<T, true, 1, false><<<sh.grid, sh.block, sh.lds, s>>>(K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles);
-    } else {
-        if (l == 0) k_rb_sweep<T, false, 1, true><<<sh.grid, sh.block, sh.lds, s>>>(K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles);
-        else k_rb_sweep<T, false, 1, false><<<sh.grid, sh.block, sh.lds, s>>>(K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles);
-    }
+    if (dgi) sweep_launch<T, true, true>(sh, L, l, b, c1, first, s);
+    else sweep_launch<T, false, true>(sh, L, l, b, c1, first, s);
     if (e) (void)hipEventRecord(e[1], s);
     T *t = L.x;
     L.x = L.xt;
@@ -1210,35 +1479,60 @@ static void rb_sweep(Level<T> &L, int l, const T *b, int c1, bool first, Prof *p
 }
 
 // one V-cycle: x_0 = V(b_0).  fineb: the fine right-hand side (the FP64
-// residual in FP64 mode, its FP32 copy in mixed mode).  fused: one kernel
-// per red-black sweep (k_rb_sweep) instead of one per colour (k_rb).
+// residual in FP64 mode, its FP32 copy in mixed mode).
 template <class T>
-static void vcycle(Hier<T> &H, const T *fineb, int nu_pre, int nu_post, Prof *prof, hipStream_t s,
-                   bool fused = true) {
+static void vcycle(Hier<T> &H, const T *fineb, const Cycle &cy, Prof *prof, hipStream_t s) {
     const int nl = (int)H.lv.size();
-    for (int l = 0; l < nl - 1; ++l) {
+    const int nu_pre = cy.nu_pre, nu_post = cy.nu_post;
+    const bool dgi = cy.dgi, fused = cy.fused;
+    // levels lt .. nl-1 run in one workgroup (k_tail): the first level
+    // below the finest with at most cy.tail_cells cells
+    int lt = nl;
+    if (cy.tail_cells > 0) {
+        for (int l = 1; l < nl; ++l) {
+            if (H.lv[l].n <= cy.tail_cells) { lt = l; break; }
+        }
+        if (nl - lt > kTailMax) lt = nl;
+    }
+    for (int l = 0; l < (lt < nl ? lt : nl - 1); ++l) {
         Level<T> &L = H.lv[l];
         const T *b = l == 0 ? fineb : L.b;
         if (fused) {
-            rb_sweep(L, l, b, 0, true, prof, s);
-            for (int q = 1; q < nu_pre; ++q) rb_sweep(L, l, b, 0, false, prof, s);
+            rb_sweep(L, l, b, 0, true, cy, prof, s);
+            for (int q = 1; q < nu_pre; ++q) rb_sweep(L, l, b, 0, false, cy, prof, s);
         } else {
-            rb_pass(L, l, b, 0, true, prof, s);
-            rb_pass(L, l, b, 1, false, prof, s);
+            rb_pass(L, l, b, 0, true, dgi, prof, s);
+            rb_pass(L, l, b, 1, false, dgi, prof, s);
             for (int q = 1; q < nu_pre; ++q) {
-                rb_pass(L, l, b, 0, false, prof, s);
-                rb_pass(L, l, b, 1, false, prof, s);
+                rb_pass(L, l, b, 0, false, dgi, prof, s);
+                rb_pass(L, l, b, 1, false, dgi, prof, s);
             }
         }
         Level<T> &C = H.lv[l + 1];
-        k_resid_restrict<T, T><<<cgrid3(C.nx, C.ny, C.nz), kB3, 0, s>>>(L.k(), b, L.x, C.b);
+        if (dgi) k_resid_restrict<T, T, true><<<cgrid3(C.nx, C.ny, C.nz), kB3, 0, s>>>(L.k(), b, L.x, C.b);
+        else k_resid_restrict<T, T, false><<<cgrid3(C.nx, C.ny, C.nz), kB3, 0, s>>>(L.k(), b, L.x, C.b);
     }
-    {
+    if (lt < nl) {
+        TailK<T> K;
+        K.nl = nl - lt;
+        K.nb = H.nb;
+        K.nu_pre = nu_pre;
+        K.nu_post = nu_post;
+        for (int a = 0; a < 3; ++a) K.per[a] = H.per[a];
+        for (int l = lt; l < nl; ++l) {
+            K.L[l - lt] = H.lv[l].k();
+            K.b[l - lt] = H.lv[l].b;
+            K.x[l - lt] = H.lv[l].x;
+        }
+        K.minv = H.minv.template as<T>();
+        if (dgi) k_tail<T, true><<<1, 1024, 0, s>>>(K);
+        else k_tail<T, false><<<1, 1024, 0, s>>>(K);
+    } else {
         Level<T> &Bt = H.lv.back();
         const T *b = nl == 1 ? fineb : Bt.b;
         k_bottom_solve<T><<<1, 512, 0, s>>>(H.nb, H.minv.template as<T>(), b, Bt.x);
     }
-    for (int l = nl - 2; l >= 0; --l) {
+    for (int l = (lt < nl ? lt : nl - 1) - 1; l >= 0; --l) {
         Level<T> &L = H.lv[l];
         const Level<T> &C = H.lv[l + 1];
         const T *b = l == 0 ? fineb : L.b;
@@ -1246,10 +1540,10 @@ static void vcycle(Hier<T> &H, const T *fineb, int nu_pre, int nu_post, Prof *pr
                                                                   H.per[2], L.x, C.x);
         for (int q = 0; q < nu_post; ++q) {
             if (fused) {
-                rb_sweep(L, l, b, 1, false, prof, s);
+                rb_sweep(L, l, b, 1, false, cy, prof, s);
             } else {
-                rb_pass(L, l, b, 1, false, prof, s);
-                rb_pass(L, l, b, 0, false, prof, s);
+                rb_pass(L, l, b, 1, false, dgi, prof, s);
+                rb_pass(L, l, b, 0, false, dgi, prof, s);
             }
         }
     }
@@ -1360,11 +1654,19 @@ static int outer_iterate(pps_handle *h, Hier<T> &H, bool mixed, double tol_abs, 
     double *S = part + 4 * kRedBlocks;
     const int nb = nblk(n);
     const ZGrid zg = zgrid(f->d.nx, f->d.ny, f->d.nz);
+    const TGrid tg = tgrid(f->d.nx, f->d.ny, f->d.nz);
     const dim3 zb(64, 4, 1);
     float *b32 = mixed ? f->b32.as<float>() : nullptr;
     const double N = (double)n;
     const bool cg = h->opt.outer == PPS_OUTER_CG;
-    const int nu1 = h->opt.nu_pre < 1 ? 1 : h->opt.nu_pre, nu2 = h->opt.nu_post;
+    Cycle cy;
+    cy.nu_pre = h->opt.nu_pre < 1 ? 1 : h->opt.nu_pre;
+    cy.nu_post = h->opt.nu_post;
+    cy.dgi = f->singular != 0;
+    cy.fused = h->opt.smoother != PPS_SMOOTH_COLOUR_PASSES;
+    cy.rows = h->opt.sweep_rows;
+    cy.planes = h->opt.sweep_planes;
+    cy.tail_cells = h->opt.tail_cells == 0 ? 4096 : (h->opt.tail_cells < 0 ? 0 : h->opt.tail_cells);
 
     // r = b - M x (unless the setup pass just computed it)
     if (!f->have_r) {
@@ -1382,14 +1684,13 @@ static int outer_iterate(pps_handle *h, Hier<T> &H, bool mixed, double tol_abs, 
     if (t2 == 0.0) { *it = 1; return PPS_OK; }   // exact solution: nothing to do
     for (k = 1; k <= maxit; ++k) {
         // z = V(r)
-        vcycle<T>(H, mixed ? (const T *)(const void *)b32 : (const T *)(const void *)V.r, nu1, nu2,
-                  &h->prof, s, f->fused);
+        vcycle<T>(H, mixed ? (const T *)(const void *)b32 : (const T *)(const void *)V.r, cy, &h->prof, s);
         const T *z = H.lv[0].x;   // the sweeps swap buffers: read after the cycle
         if (cg) {
             k_dot_rz<T><<<nb, 256, 0, s>>>(n, V.r, z, part);
             k_scal_beta<<<1, 1024, 0, s>>>(part, nb, n, f->singular, S);
-            k_dq<T><<<zg.grid, zb, 0, s>>>(O, zg.zc, z, V.d[cur], V.d[cur ^ 1], V.q, S, part);
-            k_scal_alpha<<<1, 1024, 0, s>>>(part, zg.blocks, S);
+            k_dq_tiled<T><<<tg.blocks, dim3(64, 16, 1), 0, s>>>(O, tg, z, V.d[cur], V.d[cur ^ 1], V.q, S, part);
+            k_scal_alpha<<<1, 1024, 0, s>>>(part, tg.blocks, S);
             k_update<float><<<nb, 256, 0, s>>>(n, V.x, V.r, V.d[cur ^ 1], V.q, b32, S, part);
             k_scal_rr<<<1, 1024, 0, s>>>(part, nb, 0, S);
             cur ^= 1;
@@ -1417,9 +1718,6 @@ int fast_solve(pps_handle *h, const double *dA, double *dp, double maxError, int
     PPS_TRY(fast_ensure(h));
     FastState *f = h->fast;
     const bool mixed = h->opt.method == PPS_MG_MIXED;
-    f->fused = h->opt.smoother != PPS_SMOOTH_COLOUR_PASSES;
-    g_sweep_rows = h->opt.sweep_rows;
-    g_sweep_planes = h->opt.sweep_planes;
     h->hist.clear();
     PPS_HIP(hipEventRecord(h->ev[0], s));
     PPS_TRY(fast_setup(h, dA, dp, mixed, s));

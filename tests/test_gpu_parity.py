@@ -210,6 +210,53 @@ def test_fused_sweep_equals_colour_passes(pps, method, tol, case):
     assert d < tol
 
 
+@pytest.mark.parametrize("method", ["fp64", "mixed"])
+def test_fast_mg_nonsingular_matches_oracle(pps, orc, method):
+    """A symmetric system with a positive shift on A7 (no null space): the
+    V-cycle reads the stored diagonals instead of the coupling sums, and the
+    coarse diagonals carry the children's excess."""
+    _say("fast non-singular", method)
+    g, og = _grids(pps, orc, 32, (0, 1, 0, 0, 1, 0))
+    _, At, _ = pps.make_problem(n=32, bc=(0, 1, 0, 0, 1, 0), drops=[(0.4, 0.5, 0.5, 0.2)],
+                                rho2=100.0, seed=3)
+    At[6] += 0.05 * At[6].mean()
+    A = _host(At)
+    pref = np.zeros(og.shape_p)
+    orc.newsolver_mg(og, A, pref, 1e-300, 1.2, 1500)
+    ref = og.interior(pref)
+    brms = np.sqrt((A[7] ** 2).mean())
+    with pps.Solver(g, method=method) as s:
+        p = torch.zeros(g.shape_p, dtype=torch.float64, device="cuda")
+        it, t2 = s.solve(At, p, 1e-14 * brms, 1.2, 200)
+    x = og.interior(_host(p))
+    err = np.abs(x - ref).max() / np.abs(ref).max()
+    _say(f"   it={it} err={err:.2e}")
+    assert it <= 100 and err <= 1e-10
+
+
+@pytest.mark.parametrize("method,tol", [("fp64", 1e-12), ("mixed", 2e-5)])
+@pytest.mark.parametrize("case", ["walls", "perx", "shifted"])
+def test_tail_kernel_equals_level_kernels(pps, method, tol, case):
+    """One outer iteration with the coarse levels in the one-workgroup tail
+    kernel gives the iterate of the per-level kernels up to rounding."""
+    _say("tail vs per-level kernels", method, case)
+    n, bc = {"walls": (64, (0,) * 6), "perx": (64, (1, 0, 0, 1, 0, 0)),
+             "shifted": (32, (0, 1, 0, 0, 1, 0))}[case]
+    grid, A, _ = pps.make_problem(n=n, bc=bc, drops=[(0.3, 0.6, 0.4, 0.2)], rho2=1000.0, seed=5)
+    if case == "shifted":
+        A[6] += 0.05 * A[6].mean()
+    out = []
+    for tail in (0, 4096, 32768, -1):
+        with pps.Solver(grid, method=method, tail_cells=tail) as s:
+            p = torch.zeros(grid.shape_p, dtype=torch.float64, device="cuda")
+            s.solve(A, p, 1e-300, 1.2, 1)
+            out.append(_host(grid.interior(p)))
+    for o in out[:3]:
+        d = np.abs(o - out[3]).max() / np.abs(out[3]).max()
+        _say(f"   rel diff {d:.2e}")
+        assert d < tol
+
+
 def test_fast_rejects_nonsymmetric_and_nan(pps):
     _say("fast errors")
     g, A, _ = pps.make_problem(n=16, drops=[(0.5, 0.5, 0.5, 0.25)], rho2=10.0)
