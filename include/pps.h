@@ -25,7 +25,9 @@
  *
  * All entry points are synchronous on return, thread-compatible, and not
  * re-entrant per handle.  Device-pointer variants take a hipStream_t as
- * void*; NULL means the handle's own stream.
+ * void* and run on it, after the work already queued there (so inputs
+ * written on that stream are ready); NULL means the default stream.
+ * Host-array variants run on the handle's own stream.
  */
 #ifndef PPS_H
 #define PPS_H

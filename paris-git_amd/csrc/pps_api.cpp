@@ -153,7 +153,7 @@ extern "C" int pps_solve_device(pps_handle *h, const double *dA, double *dp, dou
                                 double beta, int maxit, int norm, int *it, double *tres2,
                                 void *stream) {
     if (!h) return PPS_ERR_BAD_ARG;
-    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : h->stream;
+    hipStream_t s = static_cast<hipStream_t>(stream);   // NULL: the default stream
     return solve_with(h, h->opt.method, dA, dp, maxError, beta, maxit, norm, it, tres2, s);
 }
 
@@ -240,7 +240,7 @@ extern "C" int pps_adjoint_device(pps_handle *h, const double *dA, const double 
                                   double rtol, int maxit, int *it, void *stream) {
     if (!h || !dA || !dg || !dlam || !it || !(rtol > 0.0) || maxit < 1) return PPS_ERR_BAD_ARG;
     (void)hipSetDevice(h->device);
-    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : h->stream;
+    hipStream_t s = static_cast<hipStream_t>(stream);   // NULL: the default stream
     // the adjoint runs on the fast hierarchy; the reference-faithful and
     // plain methods use its FP64 form
     const int saved = h->opt.method;

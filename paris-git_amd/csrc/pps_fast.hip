@@ -16,9 +16,8 @@
 // for a face twice as wide, over a spacing twice as long); the diagonal is
 // the sum of the coarse couplings plus the mean of the children's excess
 // A7 - sum(couplings).  Restriction is the 8-child average of the
-// residual (full weighting for cell-centred grids), prolongation is
-// trilinear in cell-centred form (weights 3/4, 1/4 per direction, Neumann
-// reflection at walls, wrap in periodic directions).  The coarsest level
+// residual, prolongation its transpose (each child gets its parent's
+// value), which keeps the V-cycle symmetric for CG.  The coarsest level
 // (<= 512 cells) is solved with a dense inverse built once per solve.
 //
 // Precision.  PPS_MG_FP64: everything FP64.  PPS_MG_MIXED: the V-cycle
@@ -126,7 +125,11 @@ k_rb(LevelK<T> L, const B *__restrict__ b, T *__restrict__ x, int colour) {
 // the prefetches stay in flight across it.  Redundant work: the ring rows
 // and one extra first-colour plane at each end of a chunk.
 // c1: first colour ((x+y+z)&1 of the cells updated first).  FIRST: xin is
-// identically zero (not read).  PX pairs per thread.
+// identically zero (not read).  PROL: the coarse correction ec is added to
+// every xin value as it is loaded (xin + P ec, P the piecewise-constant
+// prolongation: both cells of a pair share the parent m), so the sweep
+// that follows a coarse-grid correction needs no separate prolongation
+// pass.  PX pairs per thread.
 // LDS: X[2][rb+2][np] (second-colour xin), R[2][rb][np] (first colour new).
 // ---------------------------------------------------------------------
 __device__ __forceinline__ void lds_barrier() {
@@ -159,10 +162,10 @@ __device__ __forceinline__ CellOp<T> pick(const PairOp<T> &P, int e) {
 // FINE does not change the code; it gives the finest level's launches
 // their own symbol so profilers can separate them from coarse levels.
 // DGI: the diagonal is the coupling sum (not read).
-template <class T, bool FIRST, int PX, bool FINE, bool DGI, bool KEEP>
+template <class T, bool FIRST, int PX, bool FINE, bool DGI, bool KEEP, bool PROL>
 __global__ void __launch_bounds__(1024)
 k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *__restrict__ xout,
-           int c1, int zc, int oy, int ytiles) {
+           int c1, int zc, int oy, int ytiles, const T *__restrict__ ec) {
     extern __shared__ unsigned char smem_raw[];
     T *sm = reinterpret_cast<T *>(smem_raw);
     const int nx = L.nx, ny = L.ny, nz = L.nz;
@@ -200,10 +203,15 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
         P.ba = b[c]; P.bb = b[c + 1];
     };
     // second-colour xin of the extra row (ring threads only)
+    // parent of pair m of row jrow, plane k (wrapped) in the coarse grid
+    auto par = [&](int jrow, int k, int m) {
+        return (long long)m + (long long)(nx >> 1) * ((jrow >> 1) + (long long)(ny >> 1) * (k >> 1));
+    };
     auto load_extra = [&](int k, int m) -> T {
         if (FIRST || !(ring_lo || ring_hi)) return (T)0;
         const int e2x = (c1 + jx + k + 1) & 1;
-        return xin[col(jx, wrapz(k), m) + e2x];
+        const int kw = wrapz(k);
+        return xin[col(jx, kw, m) + e2x] + (PROL ? ec[par(jx, kw, m)] : (T)0);
     };
     auto stage = [&](int k, int m, T a0, T a1, T ex) {
         const int buf = k & 1;
@@ -233,11 +241,17 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
         if (FIRST) {
             xa0[p] = xb0[p] = xa1[p] = xb1[p] = xa2[p] = xb2[p] = (T)0;
         } else {
-            const long long ca = col(j, wrapz(k0 - 2), m), cb = col(j, wrapz(k0 - 1), m),
-                            cc = col(j, wrapz(k0), m);
+            const int ka = wrapz(k0 - 2), kb = wrapz(k0 - 1), kc = wrapz(k0);
+            const long long ca = col(j, ka, m), cb = col(j, kb, m), cc = col(j, kc, m);
             xa0[p] = xin[ca]; xb0[p] = xin[ca + 1];
             xa1[p] = xin[cb]; xb1[p] = xin[cb + 1];
             xa2[p] = xin[cc]; xb2[p] = xin[cc + 1];
+            if (PROL) {
+                const T ea = ec[par(j, ka, m)], eb = ec[par(j, kb, m)], ecc = ec[par(j, kc, m)];
+                xa0[p] += ea; xb0[p] += ea;
+                xa1[p] += eb; xb1[p] += eb;
+                xa2[p] += ecc; xb2[p] += ecc;
+            }
         }
         ex2[p] = load_extra(k0, m);
         load_op(k0 - 1, m, cur[p]);
@@ -263,9 +277,15 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
                 if (FIRST) {
                     xa3[p] = xb3[p] = (T)0;
                 } else {
-                    const long long cc = col(j, wrapz(s + 2), m);
+                    const int kw = wrapz(s + 2);
+                    const long long cc = col(j, kw, m);
                     xa3[p] = xin[cc];
                     xb3[p] = xin[cc + 1];
+                    if (PROL) {
+                        const T e = ec[par(j, kw, m)];
+                        xa3[p] += e;
+                        xb3[p] += e;
+                    }
                 }
                 ex3[p] = load_extra(s + 2, m);
                 load_op(s + 1, m, nxt[p]);
@@ -353,6 +373,121 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
     }
 }
 
+// Tiled launch shape of the z-marching stencil kernels: blocks of 64 x 16
+// threads own a 64 x 16 column of cells and march zc planes (zc even when
+// `even`, for kernels that pair planes).
+struct TGrid {
+    int tx, ty, zc, blocks;
+};
+static TGrid tgrid(int nx, int ny, int nz, int target = 1024, bool even = false) {
+    TGrid g;
+    g.tx = (nx + 63) / 64;
+    g.ty = (ny + 15) / 16;
+    const long long cols = (long long)g.tx * g.ty;
+    long long chunks = target / cols;
+    if (chunks * cols > kRedBlocks) chunks = kRedBlocks / cols;
+    if (chunks < 1) chunks = 1;
+    if (chunks > nz) chunks = nz;
+    g.zc = (int)((nz + chunks - 1) / chunks);
+    if (even && (g.zc & 1)) ++g.zc;
+    chunks = (nz + g.zc - 1) / g.zc;
+    g.blocks = (int)(cols * chunks);
+    return g;
+}
+
+// The same restriction on LDS tiles (64 x 16 fine cells, zc fine planes
+// per block, zc even; nx, ny, nz even): the iterate of each plane (own
+// cells plus a one-cell halo ring, coordinates wrapped) goes to one of
+// three LDS planes, the residual of a cell is added to its z-partner's in
+// registers, then over its x/y partners through LDS.  The iterate of plane
+// s+2 and the operator of plane s are requested at the start of step s.
+template <class T, bool DGI>
+__global__ void __launch_bounds__(1024)
+k_resid_restrict_tiled(LevelK<T> L, TGrid g, const T *__restrict__ b, const T *__restrict__ x,
+                       T *__restrict__ bc) {
+    __shared__ T X[3][18][66];
+    __shared__ T Rz[16][64];
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const int cnx = nx >> 1, cny = ny >> 1;
+    const int bid = blockIdx.x;
+    const int tx = bid % g.tx, ty = (bid / g.tx) % g.ty, ch = bid / (g.tx * g.ty);
+    const int li = threadIdx.x, lj = threadIdx.y;
+    const int i = tx * 64 + li, j = ty * 16 + lj;
+    const int k0 = ch * g.zc, k1 = min(k0 + g.zc, nz);
+    const bool in = i < nx && j < ny;
+    const bool emit = in && !(li & 1) && !(lj & 1);
+    const long long sy = nx, sz = (long long)nx * ny;
+    auto wrapn = [](int a, int n) { a %= n; return a < 0 ? a + n : a; };
+    auto wrapk = [&](int k) { return wrapn(k, nz); };
+    const long long own = wrapn(i, nx) + sy * wrapn(j, ny);
+    const int t = li + 64 * lj;
+    int hx = 0, hy = 0;
+    if (t < 66) { hx = t - 1; hy = -1; }
+    else if (t < 132) { hx = t - 67; hy = 16; }
+    else if (t < 148) { hx = -1; hy = t - 132; }
+    else { hx = 64; hy = t - 148; }
+    const bool halo = t < 164;
+    const long long hcol = wrapn(tx * 64 + hx, nx) + sy * wrapn(ty * 16 + hy, ny);
+    const long long dxm = in ? (long long)(wrap_m(i, nx) - i) : 0;
+    const long long dym = in ? (long long)(wrap_m(j, ny) - j) * sy : 0;
+    const long long cco = (long long)(i >> 1) + (long long)cnx * (j >> 1);   // coarse column
+    const long long csz = (long long)cnx * cny;
+    auto put = [&](int kc) {   // the 2x2 sum of the pair stored in Rz
+        if (emit) {
+            const T v = (Rz[lj][li] + Rz[lj][li + 1]) + (Rz[lj + 1][li] + Rz[lj + 1][li + 1]);
+            bc[cco + csz * kc] = v * (T)0.125;
+        }
+    };
+
+    T x_m = x[own + sz * wrapk(k0 - 1)];
+    T x_0 = x[own + sz * k0];
+    X[k0 % 3][lj + 1][li + 1] = x_0;
+    if (halo) X[k0 % 3][hy + 1][hx + 1] = x[hcol + sz * k0];
+    T xo1 = x[own + sz * wrapk(k0 + 1)];
+    T xh1 = halo ? x[hcol + sz * wrapk(k0 + 1)] : (T)0;
+    T czm = in ? L.cz[own + sz * wrapk(k0 - 1)] : (T)0;
+    __syncthreads();
+    T rpair = 0;
+    for (int s = k0; s < k1; ++s) {
+        T xo2 = 0, xh2 = 0;
+        if (s + 1 < k1) {
+            const long long p2 = sz * wrapk(s + 2);
+            xo2 = x[own + p2];
+            if (halo) xh2 = x[hcol + p2];
+        }
+        const long long c = own + sz * s;
+        T cxw = 0, cxe = 0, cys = 0, cyn = 0, czp = 0, dgc = 0, bv = 0;
+        if (in) {
+            cxw = L.cx[c + dxm]; cxe = L.cx[c];
+            cys = L.cy[c + dym]; cyn = L.cy[c];
+            czp = L.cz[c]; bv = b[c];
+            dgc = DGI ? diag_sum(cxw, cxe, cys, cyn, czm, czp) : L.dg[c];
+        }
+        const int wb = (s + 1) % 3, rb = s % 3;
+        X[wb][lj + 1][li + 1] = xo1;
+        if (halo) X[wb][hy + 1][hx + 1] = xh1;
+        const bool odd = (s - k0) & 1;
+        if (!odd && s > k0) put((s - 2) >> 1);
+        T r = 0;
+        if (in) {
+            T v = bv;
+            v += cxw * X[rb][lj + 1][li] + cxe * X[rb][lj + 1][li + 2];
+            v += cys * X[rb][lj][li + 1] + cyn * X[rb][lj + 2][li + 1];
+            v += czm * x_m + czp * xo1;
+            r = v - dgc * x_0;
+        }
+        if (!odd) {
+            rpair = r;
+        } else {
+            Rz[lj][li] = rpair + r;
+        }
+        x_m = x_0; x_0 = xo1; czm = czp;
+        xo1 = xo2; xh1 = xh2;
+        __syncthreads();
+    }
+    put((k1 - 2) >> 1);
+}
+
 // residual of the 8 children of each coarse cell, averaged into bc
 template <class T, class B, bool DGI>
 __global__ void __launch_bounds__(256)
@@ -377,31 +512,29 @@ k_resid_restrict(LevelK<T> L, const B *__restrict__ b, const T *__restrict__ x,
     bc[I + (long long)cxn * (J + (long long)cyn * K)] = acc * (T)0.125;
 }
 
-// x_f += trilinear interpolation of x_c (cell-centred)
+// x_f += x_c of the parent cell: piecewise-constant prolongation, the
+// transpose of the 8-child average (times 8), so the V-cycle (symmetric
+// smoothing, R = P^T / 8) is a symmetric operator and CG's recurrences hold
+// for it.  (Trilinear interpolation breaks that symmetry: the forward
+// solves still converge, but smooth right-hand sides such as adjoint
+// sources stall for tens of iterations.)  One thread per coarse cell.
 template <class T>
 __global__ void __launch_bounds__(256)
-k_prolong_add(int fx, int fy, int fz, int px, int py, int pz, T *__restrict__ xf,
-              const T *__restrict__ xc) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
-    if (i >= fx || j >= fy) return;
-    const int cx = fx >> 1, cy = fy >> 1, cz = fz >> 1;
-    const int I = i >> 1, J = j >> 1, K = k >> 1;
-    auto other = [](int f, int C, int nC, int per) {
-        int o = (f & 1) ? C + 1 : C - 1;
-        if (o < 0) o = per ? nC - 1 : 0;
-        if (o >= nC) o = per ? 0 : nC - 1;
-        return o;
-    };
-    const int I2 = other(i, I, cx, px), J2 = other(j, J, cy, py), K2 = other(k, K, cz, pz);
-    const long long sy = cx, sz = (long long)cx * cy;
-    const T w1 = (T)0.75, w2 = (T)0.25;
-    auto X = [&](int a, int b_, int c_) { return xc[a + sy * b_ + sz * c_]; };
-    const T v = w1 * (w1 * (w1 * X(I, J, K) + w2 * X(I2, J, K)) + w2 * (w1 * X(I, J2, K) + w2 * X(I2, J2, K))) +
-                w2 * (w1 * (w1 * X(I, J, K2) + w2 * X(I2, J, K2)) + w2 * (w1 * X(I, J2, K2) + w2 * X(I2, J2, K2)));
-    const long long c = i + (long long)fx * (j + (long long)fy * k);
-    xf[c] += v;
+k_prolong_add(int fx, int fy, int fz, T *__restrict__ xf, const T *__restrict__ xc) {
+    const int cx = fx >> 1, cy = fy >> 1;
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int J = blockIdx.y * blockDim.y + threadIdx.y;
+    const int K = blockIdx.z;
+    if (I >= cx || J >= cy) return;
+    const T v = xc[I + (long long)cx * (J + (long long)cy * K)];
+#pragma unroll
+    for (int dk = 0; dk < 2; ++dk)
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj) {
+            const long long row = (long long)fx * ((2 * J + dj) + (long long)fy * (2 * K + dk));
+            xf[row + 2 * I] += v;
+            xf[row + 2 * I + 1] += v;
+        }
 }
 
 // ---------------------------------------------------------------------
@@ -618,34 +751,70 @@ k_bottom_solve(int nb, const T *__restrict__ Minv, const T *__restrict__ b, T *_
 }
 
 // ---------------------------------------------------------------------
-// The coarse end of a V-cycle in one workgroup.  Levels 0 .. nl-1 of the
-// tail (level nl-1 is the coarsest, solved with the dense inverse) are
-// small enough to stay in L2; colour passes, restriction, the bottom
-// solve and prolongation run back to back with workgroup barriers in
-// between, so the whole tail costs one launch instead of several per
-// level.  The arithmetic is that of k_rb / k_resid_restrict /
+// The coarse end of a V-cycle in one workgroup, in LDS.  Levels 0 .. nl-1
+// of the tail (level nl-1 is the coarsest, solved with the dense inverse)
+// are staged into LDS at the start: their operators, the right-hand side
+// of level 0 and the inverse.  Colour passes, restriction, the bottom
+// solve and prolongation then run back to back on LDS with workgroup
+// barriers in between, and the iterate of level 0 is written back at the
+// end: one launch instead of several per level, and LDS latency instead of
+// L2 latency per pass.  The arithmetic is that of k_rb / k_resid_restrict /
 // k_bottom_solve / k_prolong_add (in-place colour passes: pre-smoothing
 // colour 0 first, post-smoothing colour 1 first).
+// LDS layout: off[l] = {cx, cy, cz, dg (DGI false only), b, x} of level l,
+// off_minv = the inverse; all offsets in elements of T.
 // ---------------------------------------------------------------------
 constexpr int kTailMax = 8;
 template <class T> struct TailK {
     int nl, nb, nu_pre, nu_post;
     int per[3];
-    LevelK<T> L[kTailMax];
-    T *b[kTailMax];
-    T *x[kTailMax];
+    LevelK<T> L[kTailMax];   // operators in global memory
+    const T *b0;             // right-hand side of level 0 (global)
+    T *x0;                   // iterate of level 0 (global, written at the end)
     const T *minv;
+    int off[kTailMax][6];
+    int off_minv;
 };
 
 template <class T, bool DGI>
 __global__ void __launch_bounds__(1024)
 k_tail(TailK<T> K) {
+    extern __shared__ unsigned char smem_raw[];
+    T *sm = reinterpret_cast<T *>(smem_raw);
     const int tid = threadIdx.x;
     constexpr int NT = 1024;
+    // stage operators, the right-hand side of level 0 and the inverse
+    for (int l = 0; l < K.nl; ++l) {
+        const LevelK<T> &G = K.L[l];
+        const int n = G.nx * G.ny * G.nz;
+        T *cx = sm + K.off[l][0], *cy = sm + K.off[l][1], *cz = sm + K.off[l][2];
+        T *dg = sm + K.off[l][3];
+        for (int e = tid; e < n; e += NT) {
+            cx[e] = G.cx[e];
+            cy[e] = G.cy[e];
+            cz[e] = G.cz[e];
+            if (!DGI) dg[e] = G.dg[e];
+        }
+    }
+    {
+        const int n0 = K.L[0].nx * K.L[0].ny * K.L[0].nz;
+        T *b = sm + K.off[0][4];
+        for (int e = tid; e < n0; e += NT) b[e] = K.b0[e];
+        T *mi = sm + K.off_minv;
+        for (int e = tid; e < K.nb * K.nb; e += NT) mi[e] = K.minv[e];
+    }
+    __syncthreads();
+    auto lev = [&](int l) {
+        LevelK<T> L;
+        L.nx = K.L[l].nx; L.ny = K.L[l].ny; L.nz = K.L[l].nz;
+        L.cx = sm + K.off[l][0]; L.cy = sm + K.off[l][1]; L.cz = sm + K.off[l][2];
+        L.dg = sm + K.off[l][3];
+        return L;
+    };
     auto cpass = [&](int l, int colour, bool first) {
-        const LevelK<T> &L = K.L[l];
-        const T *b = K.b[l];
-        T *x = K.x[l];
+        const LevelK<T> L = lev(l);
+        const T *b = sm + K.off[l][4];
+        T *x = sm + K.off[l][5];
         const int nxh = (L.nx + 1) >> 1;
         const int total = nxh * L.ny * L.nz;
         for (int e = tid; e < total; e += NT) {
@@ -666,11 +835,11 @@ k_tail(TailK<T> K) {
             cpass(l, 0, false);
             cpass(l, 1, false);
         }
-        const LevelK<T> &L = K.L[l];
-        const T *b = K.b[l];
-        const T *x = K.x[l];
+        const LevelK<T> L = lev(l);
+        const T *b = sm + K.off[l][4];
+        const T *x = sm + K.off[l][5];
         const int cxn = L.nx >> 1, cyn = L.ny >> 1, czn = L.nz >> 1;
-        T *bc = K.b[l + 1];
+        T *bc = sm + K.off[l + 1][4];
         for (int e = tid; e < cxn * cyn * czn; e += NT) {
             const int I = e % cxn, J = (e / cxn) % cyn, Kz = e / (cxn * cyn);
             T acc = 0;
@@ -688,46 +857,37 @@ k_tail(TailK<T> K) {
     // dense solve on the coarsest level
     {
         const int nb = K.nb;
-        const T *b = K.b[K.nl - 1];
-        T *x = K.x[K.nl - 1];
+        const T *b = sm + K.off[K.nl - 1][4];
+        T *x = sm + K.off[K.nl - 1][5];
+        const T *mi = sm + K.off_minv;
         for (int r = tid; r < nb; r += NT) {
             T acc = 0;
-            const T *row = K.minv + (long long)r * nb;
-            for (int e = 0; e < nb; ++e) acc += row[e] * b[e];
+            for (int e = 0; e < nb; ++e) acc += mi[r * nb + e] * b[e];
             x[r] = acc;
         }
         __syncthreads();
     }
     // prolongation and post-smoothing back up
     for (int l = K.nl - 2; l >= 0; --l) {
-        const LevelK<T> &L = K.L[l];
-        const int fx = L.nx, fy = L.ny, fz = L.nz;
-        const int cx = fx >> 1, cy = fy >> 1, cz = fz >> 1;
-        T *xf = K.x[l];
-        const T *xc = K.x[l + 1];
+        const int fx = K.L[l].nx, fy = K.L[l].ny, fz = K.L[l].nz;
+        const int cx = fx >> 1, cy = fy >> 1;
+        T *xf = sm + K.off[l][5];
+        const T *xc = sm + K.off[l + 1][5];
         for (int e = tid; e < fx * fy * fz; e += NT) {
             const int i = e % fx, j = (e / fx) % fy, k = e / (fx * fy);
-            const int I = i >> 1, J = j >> 1, Kz = k >> 1;
-            auto other = [](int f, int C, int nC, int per) {
-                int o = (f & 1) ? C + 1 : C - 1;
-                if (o < 0) o = per ? nC - 1 : 0;
-                if (o >= nC) o = per ? 0 : nC - 1;
-                return o;
-            };
-            const int I2 = other(i, I, cx, K.per[0]), J2 = other(j, J, cy, K.per[1]),
-                      K2 = other(k, Kz, cz, K.per[2]);
-            const long long sy = cx, sz = (long long)cx * cy;
-            const T w1 = (T)0.75, w2 = (T)0.25;
-            auto X = [&](int a, int b_, int c_) { return xc[a + sy * b_ + sz * c_]; };
-            const T v = w1 * (w1 * (w1 * X(I, J, Kz) + w2 * X(I2, J, Kz)) + w2 * (w1 * X(I, J2, Kz) + w2 * X(I2, J2, Kz))) +
-                        w2 * (w1 * (w1 * X(I, J, K2) + w2 * X(I2, J, K2)) + w2 * (w1 * X(I, J2, K2) + w2 * X(I2, J2, K2)));
-            xf[e] += v;
+            xf[e] += xc[(i >> 1) + cx * ((j >> 1) + cy * (k >> 1))];
         }
         __syncthreads();
         for (int q = 0; q < K.nu_post; ++q) {
             cpass(l, 1, false);
             cpass(l, 0, false);
         }
+    }
+    // the iterate of level 0 back to global memory
+    {
+        const int n0 = K.L[0].nx * K.L[0].ny * K.L[0].nz;
+        const T *x = sm + K.off[0][5];
+        for (int e = tid; e < n0; e += NT) K.x0[e] = x[e];
     }
 }
 
@@ -782,15 +942,17 @@ static ZGrid zgrid(int nx, int ny, int nz, int target = 16384) {
     z.blocks = (int)(cols * chunks);
     return z;
 }
-// fixed-order sum of nb partials in a 256-thread block, result in thread 0
-// (any 1-D block of a multiple of 64 threads, at most 1024)
+// fixed-order sum of nb partials, the same value in every thread (any
+// block of at most 1024 threads whose x extent is a multiple of 64)
 __device__ __forceinline__ double sum_partials(const double *part, int nb) {
     __shared__ double sm[16];
-    const int nw = blockDim.x >> 6;
+    const int nt = blockDim.x * blockDim.y * blockDim.z;
+    const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    const int nw = nt >> 6;
     double v = 0.0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) v += part[b];
+    for (int b = tid; b < nb; b += nt) v += part[b];
     v = wsum(v);
-    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+    if ((tid & 63) == 0) sm[tid >> 6] = v;
     __syncthreads();
     double t = 0.0;
     for (int w = 0; w < nw; ++w) t += sm[w];
@@ -798,29 +960,27 @@ __device__ __forceinline__ double sum_partials(const double *part, int nb) {
     return t;
 }
 
+
 struct OpK {  // FP64 fine operator for the outer iteration
     int nx, ny, nz;
     const double *cx, *cy, *cz, *dg;
 };
 
-__device__ __forceinline__ double op_apply(const OpK &L, const double *__restrict__ v, int i, int j,
-                                           int k, long long c) {
-    const long long sy = L.nx, sz = (long long)L.nx * L.ny;
-    const int im = wrap_m(i, L.nx), ip = wrap_p(i, L.nx);
-    const int jm = wrap_m(j, L.ny), jp = wrap_p(j, L.ny);
-    const int km = wrap_m(k, L.nz), kp = wrap_p(k, L.nz);
-    const long long cxm = c + (im - i), cxp = c + (ip - i);
-    const long long cym = c + (long long)(jm - j) * sy, cyp = c + (long long)(jp - j) * sy;
-    const long long czm = c + (long long)(km - k) * sz, czp = c + (long long)(kp - k) * sz;
-    double s = L.cx[cxm] * v[cxm] + L.cx[c] * v[cxp];
-    s += L.cy[cym] * v[cym] + L.cy[c] * v[cyp];
-    s += L.cz[czm] * v[czm] + L.cz[c] * v[czp];
-    return L.dg[c] * v[c] - s;
+// Diagonal of the FP64 outer operator at cell c (cxm, cym, czm: the
+// indices of the cell's -x/-y/-z couplings): stored, or (DGI, used when the
+// setup pass found every A7 equal to it) the six couplings added in the
+// order of the assembly's sum(A(i,j,k,1:6)) (modules.f90:2667), which
+// reproduces A7 bit for bit.
+template <bool DGI>
+__device__ __forceinline__ double outer_diag(const OpK &L, long long c, long long cxm, long long cym,
+                                             long long czm) {
+    if (!DGI) return L.dg[c];
+    return ((((L.cx[cxm] + L.cx[c]) + L.cy[cym]) + L.cy[c]) + L.cz[czm]) + L.cz[c];
 }
 
 // r = b - M x ; partials: 0 = sum r^2, 1 = sum b^2 ; optional FP32 copy.
 // z-marching over zc planes per thread (launch with zgrid()).
-template <class TB>
+template <class TB, bool DGI>
 __global__ void __launch_bounds__(256)
 k_outer_resid(OpK L, int zc, const double *__restrict__ b, const double *__restrict__ x,
               double *__restrict__ r, TB *__restrict__ bt, double *__restrict__ part) {
@@ -844,7 +1004,7 @@ k_outer_resid(OpK L, int zc, const double *__restrict__ b, const double *__restr
             s += L.cy[c + dym] * x[c + dym] + L.cy[c] * x[c + dyp];
             s += L.cz[czm] * xm + L.cz[c] * xp;
             const double bv = b[c];
-            const double v = bv - (L.dg[c] * x0 - s);
+            const double v = bv - (outer_diag<DGI>(L, c, c + dxm, c + dym, czm) * x0 - s);
             r[c] = v;
             if (bt) bt[c] = (TB)v;
             rr += v * v;
@@ -873,43 +1033,7 @@ k_dot_rz(long long n, const double *__restrict__ r, const TZ *__restrict__ z, do
     block_partial(rs, part + 2 * kRedBlocks);
 }
 
-// beta step: zmean, rho = r.(z - zmean), beta = rho / rho_old
-__global__ void __launch_bounds__(1024)
-k_scal_beta(const double *__restrict__ part, int nb, long long n, int singular,
-            double *__restrict__ S) {
-    const double rz = sum_partials(part, nb);
-    const double zs = sum_partials(part + kRedBlocks, nb);
-    const double rs = sum_partials(part + 2 * kRedBlocks, nb);
-    if (threadIdx.x == 0) {
-        const double zm = singular ? zs / (double)n : 0.0;
-        const double rho = rz - zm * rs;
-        const double first = S[S_FIRST];
-        S[S_BETA] = first != 0.0 ? 0.0 : rho / S[S_RZ];
-        S[S_RZ] = rho;
-        S[S_ZMEAN] = zm;
-        S[S_FIRST] = 0.0;
-    }
-}
 
-// Tiled launch shape of the FP64 outer stencil kernels: blocks of 64 x 16
-// threads own a 64 x 16 column of cells and march zc planes.
-struct TGrid {
-    int tx, ty, zc, blocks;
-};
-static TGrid tgrid(int nx, int ny, int nz, int target = 1024) {
-    TGrid g;
-    g.tx = (nx + 63) / 64;
-    g.ty = (ny + 15) / 16;
-    const long long cols = (long long)g.tx * g.ty;
-    long long chunks = target / cols;
-    if (chunks * cols > kRedBlocks) chunks = kRedBlocks / cols;
-    if (chunks < 1) chunks = 1;
-    if (chunks > nz) chunks = nz;
-    g.zc = (int)((nz + chunks - 1) / chunks);
-    chunks = (nz + g.zc - 1) / g.zc;
-    g.blocks = (int)(cols * chunks);
-    return g;
-}
 
 // d_new = (z - zmean) + beta d_old ; q = M d_new ; partial sum d.q, on
 // LDS tiles: d_new of every plane (own cells plus a one-cell halo
@@ -919,7 +1043,7 @@ static TGrid tgrid(int nx, int ny, int nz, int target = 1024) {
 // plane s are requested at the start of step s, so the loads overlap the
 // arithmetic of the step; one barrier per plane (three buffers: the plane
 // written at step s+1 was last read at step s-2).
-template <class TZ>
+template <class TZ, bool DGI>
 __global__ void __launch_bounds__(1024)
 k_dq_tiled(OpK L, TGrid g, const TZ *__restrict__ z, const double *__restrict__ dold,
            double *__restrict__ dnew, double *__restrict__ q, const double *__restrict__ S,
@@ -994,7 +1118,8 @@ k_dq_tiled(OpK L, TGrid g, const TZ *__restrict__ z, const double *__restrict__ 
         if (in) {
             cxw = L.cx[c + dxm]; cxe = L.cx[c];
             cys = L.cy[c + dym]; cyn = L.cy[c];
-            czp = L.cz[c]; dgc = L.dg[c];
+            czp = L.cz[c];
+            dgc = DGI ? ((((cxw + cxe) + cys) + cyn) + czm) + czp : L.dg[c];
         }
         // d_new of plane s+1 into LDS
         const int wb = (s + 1) % 3, rb = s % 3;
@@ -1017,14 +1142,6 @@ k_dq_tiled(OpK L, TGrid g, const TZ *__restrict__ z, const double *__restrict__ 
     block_partial(dq, part);
 }
 
-__global__ void __launch_bounds__(1024)
-k_scal_alpha(const double *__restrict__ part, int nb, double *__restrict__ S) {
-    const double dq = sum_partials(part, nb);
-    if (threadIdx.x == 0) {
-        S[S_DQ] = dq;
-        S[S_ALPHA] = S[S_RZ] / dq;
-    }
-}
 
 // x += alpha d ; r -= alpha q ; partial sum r^2 ; optional FP32 copy of r
 template <class TB>
@@ -1065,6 +1182,46 @@ k_add_z(long long n, double *__restrict__ x, const TZ *__restrict__ z, const dou
         x[c] += (double)z[c] - zm;
 }
 
+// the beta step from k_dot_rz's partials: zmean, rho = r.(z - zmean),
+// beta = rho / rho_old
+__global__ void __launch_bounds__(1024)
+k_scal_beta(const double *__restrict__ part, int nb, long long n, int singular,
+            double *__restrict__ S) {
+    const double rz = sum_partials(part, nb);
+    const double zs = sum_partials(part + kRedBlocks, nb);
+    const double rs = sum_partials(part + 2 * kRedBlocks, nb);
+    if (threadIdx.x == 0) {
+        const double zm = singular ? zs / (double)n : 0.0;
+        const double rho = rz - zm * rs;
+        const double first = S[S_FIRST];
+        S[S_BETA] = first != 0.0 ? 0.0 : rho / S[S_RZ];
+        S[S_RZ] = rho;
+        S[S_ZMEAN] = zm;
+        S[S_FIRST] = 0.0;
+    }
+}
+
+// the alpha step from k_dq's partials
+__global__ void __launch_bounds__(1024)
+k_scal_alpha(const double *__restrict__ part, int nb, double *__restrict__ S) {
+    const double dq = sum_partials(part, nb);
+    if (threadIdx.x == 0) {
+        S[S_DQ] = dq;
+        S[S_ALPHA] = S[S_RZ] / dq;
+    }
+}
+
+// b -= mean(b), the mean from the partial sums of k_zsum
+__global__ void __launch_bounds__(256)
+k_sub_mean(long long n, double *__restrict__ b, const double *__restrict__ part, int nb) {
+    __shared__ double m_s;
+    const double bs = sum_partials(part, nb);
+    if (threadIdx.x == 0) m_s = bs / (double)n;
+    __syncthreads();
+    const double m = m_s;
+    for (long long c = blockIdx.x * 256LL + threadIdx.x; c < n; c += (long long)gridDim.x * 256) b[c] -= m;
+}
+
 __global__ void __launch_bounds__(1024)
 k_scal_rr(const double *__restrict__ part, int nb, int with_bb, double *__restrict__ S) {
     const double rr = sum_partials(part, nb);
@@ -1103,7 +1260,7 @@ k_unpack(PackK d, const double *__restrict__ x, double *__restrict__ p) {
 // residual r = A8 - A7 p + sum A_nb p_nb (the reference's own expression
 // on the padded p, MGsolver.f90:97-100), and in mixed mode the FP32 copies
 // of the operator and of r.  Partials: 0 = sum r^2, 1 = sum b^2.  flags as
-// k_pack.
+// k_pack, plus bit3: some A7 differs from the coupling sum outer_diag forms.
 __global__ void __launch_bounds__(256)
 k_setup(PackK d, int zc, const double *__restrict__ A, const double *__restrict__ p,
         double *__restrict__ cx, double *__restrict__ cy, double *__restrict__ cz,
@@ -1166,6 +1323,13 @@ k_setup(PackK d, int zc, const double *__restrict__ A, const double *__restrict_
                 if (A[4 * n + e] != a[5]) f |= 2;
             }
             if (fabs(a[6] - kept) > 1e-12 * fabs(a[6])) f |= 4;
+            {
+                // is A7 the coupling sum of the stored operator, bit for bit?
+                const double vxm = (i == 0 && !d.px) ? 0.0 : a[0];
+                const double vym = (j == 0 && !d.py) ? 0.0 : a[2];
+                const double vzm = (k == 0 && !d.pz) ? 0.0 : a[4];
+                if (a[6] != ((((vxm + vx) + vym) + vy) + vzm) + vz) f |= 8;
+            }
             cx[c] = vx; cy[c] = vy; cz[c] = vz; dg[c] = a[6];
             bo[c] = rhs;
             x[c] = pc;
@@ -1187,6 +1351,7 @@ k_setup(PackK d, int zc, const double *__restrict__ A, const double *__restrict_
 // Final check fused with the write-back (z-marching): r = b - M x with the
 // FP64 operator, partial sum r^2, and p = x on the interior with the
 // first ghost layer wrapped in periodic directions.
+template <bool DGI>
 __global__ void __launch_bounds__(256)
 k_check_unpack(OpK L, PackK d, int zc, const double *__restrict__ b, const double *__restrict__ x,
                double *__restrict__ p, double *__restrict__ part) {
@@ -1210,7 +1375,7 @@ k_check_unpack(OpK L, PackK d, int zc, const double *__restrict__ b, const doubl
             double s = L.cx[c + dxm] * x[c + dxm] + L.cx[c] * x[c + dxp];
             s += L.cy[c + dym] * x[c + dym] + L.cy[c] * x[c + dyp];
             s += L.cz[czm] * xm + L.cz[c] * xp;
-            const double v = b[c] - (L.dg[c] * x0 - s);
+            const double v = b[c] - (outer_diag<DGI>(L, c, c + dxm, c + dym, czm) * x0 - s);
             rr += v * v;
             const long long q = (i + d.ng) + Py * (j + d.ng) + Pz * (k + d.ng);
             p[q] = x0;
@@ -1262,6 +1427,7 @@ template <class T> struct Hier {
     int per[3] = {0, 0, 0};
 };
 
+
 struct FastState {
     Dims d;
     int built = 0;
@@ -1271,12 +1437,14 @@ struct FastState {
     DevBuf red;      // partials (4 x kRedBlocks) + scalars
     DevBuf flags;
     int singular = 1;
+    int exact = 0;   // every A7 is the coupling sum of outer_diag, bit for bit
     const double *built_for = nullptr;
     Hier<double> h64;
     Hier<float> h32;
     DevBuf b32;      // FP32 right-hand side of the mixed V-cycle
     int mixed_built = 0, fp64_built = 0;
     int have_r = 0;  // the setup pass already wrote r (and its norms) for x
+    size_t lds_max = 65536;  // LDS bytes per workgroup on this device
 };
 
 void fast_destroy(FastState *f) {
@@ -1423,6 +1591,27 @@ static SweepShape sweep_shape(int nx, int ny, int nz, int max_rows, int planes) 
     return sh;
 }
 
+// LDS bytes of a tail that starts at level lt (and its offsets in K)
+template <class T>
+static size_t tail_layout(const Hier<T> &H, int lt, bool dgi, TailK<T> *K) {
+    const int nl = (int)H.lv.size();
+    size_t off = 0;
+    for (int l = lt; l < nl; ++l) {
+        const size_t n = (size_t)H.lv[l].n;
+        int o[6];
+        o[0] = (int)off; off += n;
+        o[1] = (int)off; off += n;
+        o[2] = (int)off; off += n;
+        o[3] = (int)off; if (!dgi) off += n;
+        o[4] = (int)off; off += n;
+        o[5] = (int)off; off += n;
+        if (K) for (int f = 0; f < 6; ++f) K->off[l - lt][f] = o[f];
+    }
+    if (K) K->off_minv = (int)off;
+    off += (size_t)H.nb * H.nb;
+    return off * sizeof(T);
+}
+
 // How a V-cycle runs (from the handle's options and the system):
 //   nu_pre, nu_post  smoothing sweeps;
 //   dgi              every diagonal is the coupling sum (singular systems);
@@ -1436,42 +1625,58 @@ struct Cycle {
     bool dgi = false, fused = true;
     int rows = 0, planes = 0;
     long long tail_cells = 4096;
+    size_t lds_max = 65536;   // LDS bytes one workgroup may use
 };
 
 // One fused red-black sweep of level l (out of place, the iterate pointers
 // swap).  Algorithmic bytes: the operator planes (3 couplings, plus the
 // diagonal unless dgi), b, the old iterate (unless FIRST) and the new
 // iterate, once each.
+template <class T, bool FIRST, bool DGI, bool KEEP, bool PROL>
+static void sweep_launch1(const SweepShape &sh, const Level<T> &L, int l, const T *b, int c1,
+                          const T *ec, hipStream_t s) {
+    const LevelK<T> K = L.k();
+    if (l == 0)
+        k_rb_sweep<T, FIRST, 1, true, DGI, KEEP, PROL><<<sh.grid, sh.block, sh.lds, s>>>(
+            K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec);
+    else
+        k_rb_sweep<T, FIRST, 1, false, DGI, KEEP, PROL><<<sh.grid, sh.block, sh.lds, s>>>(
+            K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec);
+}
 template <class T, bool DGI, bool KEEP>
 static void sweep_launch(const SweepShape &sh, const Level<T> &L, int l, const T *b, int c1, bool first,
-                         hipStream_t s) {
-    const LevelK<T> K = L.k();
-    if (first) {
-        if (l == 0) k_rb_sweep<T, true, 1, true, DGI, KEEP><<<sh.grid, sh.block, sh.lds, s>>>(K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles);
-        else k_rb_sweep<T, true, 1, false, DGI, KEEP><<<sh.grid, sh.block, sh.lds, s>>>(K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles);
-    } else {
-        if (l == 0) k_rb_sweep<T, false, 1, true, DGI, KEEP><<<sh.grid, sh.block, sh.lds, s>>>(K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles);
-        else k_rb_sweep<T, false, 1, false, DGI, KEEP><<<sh.grid, sh.block, sh.lds, s>>>(K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles);
-    }
+                         const T *ec, hipStream_t s) {
+    if (first) sweep_launch1<T, true, DGI, KEEP, false>(sh, L, l, b, c1, nullptr, s);
+    else if (ec) sweep_launch1<T, false, DGI, KEEP, true>(sh, L, l, b, c1, ec, s);
+    else sweep_launch1<T, false, DGI, KEEP, false>(sh, L, l, b, c1, nullptr, s);
 }
 
+// Whether level l's sweeps run as one fused kernel (else two colour passes).
+template <class T>
+static bool sweep_fused(const Level<T> &L, const Cycle &cy) {
+    const SweepShape sh = sweep_shape<T>(L.nx, L.ny, L.nz, cy.rows, cy.planes);
+    return cy.fused && !(sh.rb < 3 || sh.lds > 65536 || (L.nx & 1) || sh.tpr * sh.rb > 1024);
+}
+
+// ec: the coarse correction to add to the iterate first (fused sweeps only).
 template <class T>
 static void rb_sweep(Level<T> &L, int l, const T *b, int c1, bool first, const Cycle &cy, Prof *prof,
-                     hipStream_t s) {
+                     hipStream_t s, const T *ec = nullptr) {
     const bool dgi = cy.dgi;
     const SweepShape sh = sweep_shape<T>(L.nx, L.ny, L.nz, cy.rows, cy.planes);
-    if (sh.rb < 3 || sh.lds > 65536 || (L.nx & 1) || sh.tpr * sh.rb > 1024) {
+    if (!sweep_fused(L, cy)) {
         // rows too long for the LDS planes: two colour passes instead
         rb_pass(L, l, b, c1, first, dgi, prof, s);
         rb_pass(L, l, b, 1 - c1, false, dgi, prof, s);
         return;
     }
-    // the profiled kernel: finest level, not the first sweep of a cycle
+    // the profiled kernel: finest level, neither the first sweep of a cycle
+    // nor the one that adds the coarse correction
     const double bytes = (double)L.n * sizeof(T) * (dgi ? 6.0 : 7.0);
-    hipEvent_t *e = (l == 0 && !first && prof) ? prof->begin(bytes) : nullptr;
+    hipEvent_t *e = (l == 0 && !first && !ec && prof) ? prof->begin(bytes) : nullptr;
     if (e) (void)hipEventRecord(e[0], s);
-    if (dgi) sweep_launch<T, true, true>(sh, L, l, b, c1, first, s);
-    else sweep_launch<T, false, true>(sh, L, l, b, c1, first, s);
+    if (dgi) sweep_launch<T, true, true>(sh, L, l, b, c1, first, ec, s);
+    else sweep_launch<T, false, true>(sh, L, l, b, c1, first, ec, s);
     if (e) (void)hipEventRecord(e[1], s);
     T *t = L.x;
     L.x = L.xt;
@@ -1492,7 +1697,9 @@ static void vcycle(Hier<T> &H, const T *fineb, const Cycle &cy, Prof *prof, hipS
         for (int l = 1; l < nl; ++l) {
             if (H.lv[l].n <= cy.tail_cells) { lt = l; break; }
         }
-        if (nl - lt > kTailMax) lt = nl;
+        // the tail must fit the workgroup's LDS
+        while (lt < nl && (nl - lt > kTailMax || tail_layout(H, lt, dgi, (TailK<T> *)nullptr) > cy.lds_max))
+            ++lt;
     }
     for (int l = 0; l < (lt < nl ? lt : nl - 1); ++l) {
         Level<T> &L = H.lv[l];
@@ -1509,8 +1716,14 @@ static void vcycle(Hier<T> &H, const T *fineb, const Cycle &cy, Prof *prof, hipS
             }
         }
         Level<T> &C = H.lv[l + 1];
-        if (dgi) k_resid_restrict<T, T, true><<<cgrid3(C.nx, C.ny, C.nz), kB3, 0, s>>>(L.k(), b, L.x, C.b);
-        else k_resid_restrict<T, T, false><<<cgrid3(C.nx, C.ny, C.nz), kB3, 0, s>>>(L.k(), b, L.x, C.b);
+        if (L.nx >= 32 && L.ny >= 16) {
+            const TGrid rg = tgrid(L.nx, L.ny, L.nz, 1024, true);
+            if (dgi) k_resid_restrict_tiled<T, true><<<rg.blocks, dim3(64, 16, 1), 0, s>>>(L.k(), rg, b, L.x, C.b);
+            else k_resid_restrict_tiled<T, false><<<rg.blocks, dim3(64, 16, 1), 0, s>>>(L.k(), rg, b, L.x, C.b);
+        } else {
+            if (dgi) k_resid_restrict<T, T, true><<<cgrid3(C.nx, C.ny, C.nz), kB3, 0, s>>>(L.k(), b, L.x, C.b);
+            else k_resid_restrict<T, T, false><<<cgrid3(C.nx, C.ny, C.nz), kB3, 0, s>>>(L.k(), b, L.x, C.b);
+        }
     }
     if (lt < nl) {
         TailK<T> K;
@@ -1519,14 +1732,13 @@ static void vcycle(Hier<T> &H, const T *fineb, const Cycle &cy, Prof *prof, hipS
         K.nu_pre = nu_pre;
         K.nu_post = nu_post;
         for (int a = 0; a < 3; ++a) K.per[a] = H.per[a];
-        for (int l = lt; l < nl; ++l) {
-            K.L[l - lt] = H.lv[l].k();
-            K.b[l - lt] = H.lv[l].b;
-            K.x[l - lt] = H.lv[l].x;
-        }
+        for (int l = lt; l < nl; ++l) K.L[l - lt] = H.lv[l].k();
+        K.b0 = H.lv[lt].b;
+        K.x0 = H.lv[lt].x;
         K.minv = H.minv.template as<T>();
-        if (dgi) k_tail<T, true><<<1, 1024, 0, s>>>(K);
-        else k_tail<T, false><<<1, 1024, 0, s>>>(K);
+        const size_t lds = tail_layout(H, lt, dgi, &K);
+        if (dgi) k_tail<T, true><<<1, 1024, lds, s>>>(K);
+        else k_tail<T, false><<<1, 1024, lds, s>>>(K);
     } else {
         Level<T> &Bt = H.lv.back();
         const T *b = nl == 1 ? fineb : Bt.b;
@@ -1536,11 +1748,12 @@ static void vcycle(Hier<T> &H, const T *fineb, const Cycle &cy, Prof *prof, hipS
         Level<T> &L = H.lv[l];
         const Level<T> &C = H.lv[l + 1];
         const T *b = l == 0 ? fineb : L.b;
-        k_prolong_add<T><<<cgrid3(L.nx, L.ny, L.nz), kB3, 0, s>>>(L.nx, L.ny, L.nz, H.per[0], H.per[1],
-                                                                  H.per[2], L.x, C.x);
+        // the first fused post-sweep adds the correction as it loads x
+        const bool fold = nu_post > 0 && sweep_fused(L, cy);
+        if (!fold) k_prolong_add<T><<<cgrid3(C.nx, C.ny, C.nz), kB3, 0, s>>>(L.nx, L.ny, L.nz, L.x, C.x);
         for (int q = 0; q < nu_post; ++q) {
             if (fused) {
-                rb_sweep(L, l, b, 1, false, cy, prof, s);
+                rb_sweep(L, l, b, 1, false, cy, prof, s, (fold && q == 0) ? (const T *)C.x : nullptr);
             } else {
                 rb_pass(L, l, b, 1, false, dgi, prof, s);
                 rb_pass(L, l, b, 0, false, dgi, prof, s);
@@ -1569,6 +1782,19 @@ static int fast_ensure(pps_handle *h) {
     PPS_TRY(f->red.ensure(sizeof(double) * (4 * kRedBlocks + 64)));
     PPS_TRY(f->flags.ensure(64));
     if (!h->pinned) PPS_HIP(hipHostMalloc((void **)&h->pinned, 16 * sizeof(double), 0));
+    if (!f->built) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device) == hipSuccess && v > 0)
+            f->lds_max = (size_t)v;
+        // the tail kernel may use all of it
+        const int mx = (int)f->lds_max;
+        (void)hipFuncSetAttribute((const void *)k_tail<float, true>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
+        (void)hipFuncSetAttribute((const void *)k_tail<float, false>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
+        (void)hipFuncSetAttribute((const void *)k_tail<double, true>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
+        (void)hipFuncSetAttribute((const void *)k_tail<double, false>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
+        (void)hipGetLastError();
+        f->built = 1;
+    }
     return PPS_OK;
 }
 
@@ -1622,6 +1848,7 @@ static int fast_setup(pps_handle *h, const double *dA, const double *dp, bool mi
     if (fl & 1) return PPS_ERR_NAN_OR_TINY_DIAG;
     if (fl & 2) return PPS_ERR_NONSYMMETRIC;
     f->singular = (fl & 4) ? 0 : 1;
+    f->exact = (fl & 8) ? 0 : 1;
     f->have_r = 1;
 
     if (mixed) {
@@ -1667,10 +1894,12 @@ static int outer_iterate(pps_handle *h, Hier<T> &H, bool mixed, double tol_abs, 
     cy.rows = h->opt.sweep_rows;
     cy.planes = h->opt.sweep_planes;
     cy.tail_cells = h->opt.tail_cells == 0 ? 4096 : (h->opt.tail_cells < 0 ? 0 : h->opt.tail_cells);
+    cy.lds_max = f->lds_max;
 
     // r = b - M x (unless the setup pass just computed it)
     if (!f->have_r) {
-        k_outer_resid<float><<<zg.grid, zb, 0, s>>>(O, zg.zc, V.b, V.x, V.r, b32, part);
+        if (f->exact) k_outer_resid<float, true><<<zg.grid, zb, 0, s>>>(O, zg.zc, V.b, V.x, V.r, b32, part);
+        else k_outer_resid<float, false><<<zg.grid, zb, 0, s>>>(O, zg.zc, V.b, V.x, V.r, b32, part);
         k_scal_rr<<<1, 1024, 0, s>>>(part, zg.blocks, 1, S);
     }
     f->have_r = 0;
@@ -1689,7 +1918,12 @@ static int outer_iterate(pps_handle *h, Hier<T> &H, bool mixed, double tol_abs, 
         if (cg) {
             k_dot_rz<T><<<nb, 256, 0, s>>>(n, V.r, z, part);
             k_scal_beta<<<1, 1024, 0, s>>>(part, nb, n, f->singular, S);
-            k_dq_tiled<T><<<tg.blocks, dim3(64, 16, 1), 0, s>>>(O, tg, z, V.d[cur], V.d[cur ^ 1], V.q, S, part);
+            if (f->exact)
+                k_dq_tiled<T, true><<<tg.blocks, dim3(64, 16, 1), 0, s>>>(O, tg, z, V.d[cur], V.d[cur ^ 1], V.q, S,
+                                                                          part);
+            else
+                k_dq_tiled<T, false><<<tg.blocks, dim3(64, 16, 1), 0, s>>>(O, tg, z, V.d[cur], V.d[cur ^ 1], V.q, S,
+                                                                           part);
             k_scal_alpha<<<1, 1024, 0, s>>>(part, tg.blocks, S);
             k_update<float><<<nb, 256, 0, s>>>(n, V.x, V.r, V.d[cur ^ 1], V.q, b32, S, part);
             k_scal_rr<<<1, 1024, 0, s>>>(part, nb, 0, S);
@@ -1697,7 +1931,8 @@ static int outer_iterate(pps_handle *h, Hier<T> &H, bool mixed, double tol_abs, 
         } else {
             k_zsum<T><<<nb, 256, 0, s>>>(n, z, part);
             k_add_z<T><<<nb, 256, 0, s>>>(n, V.x, z, part, nb, f->singular);
-            k_outer_resid<float><<<zg.grid, zb, 0, s>>>(O, zg.zc, V.b, V.x, V.r, b32, part);
+            if (f->exact) k_outer_resid<float, true><<<zg.grid, zb, 0, s>>>(O, zg.zc, V.b, V.x, V.r, b32, part);
+            else k_outer_resid<float, false><<<zg.grid, zb, 0, s>>>(O, zg.zc, V.b, V.x, V.r, b32, part);
             k_scal_rr<<<1, 1024, 0, s>>>(part, zg.blocks, 0, S);
         }
         PPS_HIP(hipGetLastError());
@@ -1742,7 +1977,8 @@ int fast_solve(pps_handle *h, const double *dA, double *dp, double maxError, int
             const ZGrid zg = zgrid(d.nx, d.ny, d.nz);
             double *part = f->red.as<double>();
             double *S = part + 4 * kRedBlocks;
-            k_check_unpack<<<zg.grid, dim3(64, 4, 1), 0, s>>>(opk(f), packk(d), zg.zc, vecs(f).b,
+            auto kcu = f->exact ? k_check_unpack<true> : k_check_unpack<false>;
+            kcu<<<zg.grid, dim3(64, 4, 1), 0, s>>>(opk(f), packk(d), zg.zc, vecs(f).b,
                                                                vecs(f).x, dp, part);
             k_scal_rr<<<1, 1024, 0, s>>>(part, zg.blocks, 0, S);
             PPS_HIP(hipGetLastError());
@@ -1792,11 +2028,7 @@ int fast_adjoint(pps_handle *h, const double *dA, const double *dg, double *dlam
         double *part = f->red.as<double>();
         const int nb = nblk(n);
         k_zsum<double><<<nb, 256, 0, s>>>(n, V.b, part);
-        // b -= mean(b): reuse k_add_z with z = b and x = b would double it;
-        // use a negated add on a zero vector instead
-        k_fill<<<nb, 256, 0, s>>>(V.x, n, 0.0);
-        k_add_z<double><<<nb, 256, 0, s>>>(n, V.x, V.b, part, nb, 1);
-        PPS_HIP(hipMemcpyAsync(V.b, V.x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        k_sub_mean<<<nb, 256, 0, s>>>(n, V.b, part, nb);
     }
     k_fill<<<nblk(n), 256, 0, s>>>(V.x, n, 0.0);
     // absolute tolerance from the right-hand side norm
@@ -1804,7 +2036,8 @@ int fast_adjoint(pps_handle *h, const double *dA, const double *dg, double *dlam
     double *S = part + 4 * kRedBlocks;
     const OpK O = opk(f);
     const ZGrid zg = zgrid(d.nx, d.ny, d.nz);
-    k_outer_resid<float><<<zg.grid, dim3(64, 4, 1), 0, s>>>(O, zg.zc, V.b, V.x, V.r, nullptr, part);
+    if (f->exact) k_outer_resid<float, true><<<zg.grid, dim3(64, 4, 1), 0, s>>>(O, zg.zc, V.b, V.x, V.r, nullptr, part);
+    else k_outer_resid<float, false><<<zg.grid, dim3(64, 4, 1), 0, s>>>(O, zg.zc, V.b, V.x, V.r, nullptr, part);
     k_scal_rr<<<1, 1024, 0, s>>>(part, zg.blocks, 1, S);
     PPS_HIP(hipMemcpyAsync(h->pinned, S + S_BB, sizeof(double), hipMemcpyDeviceToHost, s));
     PPS_HIP(hipStreamSynchronize(s));
@@ -1813,8 +2046,9 @@ int fast_adjoint(pps_handle *h, const double *dA, const double *dg, double *dlam
     int k = 0;
     double last = 0.0;
     int rc;
-    if (mixed) rc = outer_iterate<float>(h, f->h32, true, tol, maxit, &k, &last, false, s);
-    else rc = outer_iterate<double>(h, f->h64, false, tol, maxit, &k, &last, false, s);
+    h->hist.clear();
+    if (mixed) rc = outer_iterate<float>(h, f->h32, true, tol, maxit, &k, &last, true, s);
+    else rc = outer_iterate<double>(h, f->h64, false, tol, maxit, &k, &last, true, s);
     if (rc != PPS_OK) return rc;
     // padded output, ghosts: wrapped (periodic) or zero (walls)
     PPS_HIP(hipMemsetAsync(dlam, 0, sizeof(double) * d.padded(), s));

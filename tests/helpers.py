@@ -112,3 +112,74 @@ def exact_solution(g, A):
     rhs = np.concatenate([A[7].ravel(), [0.0]])
     x = spla.spsolve(K, rhs)
     return x[:n].reshape(g.nz, g.ny, g.nx)
+
+
+# ---------------------------------------------------------------------------
+# adjoint helpers (torch tensors on the solver's device)
+# ---------------------------------------------------------------------------
+def faces_from_rho(grid, rho):
+    """c_f = 2/(rho_L+rho_R) on every +x/+y/+z face; closed (0) on walls."""
+    import torch
+    ng = grid.ng
+    R = rho[ng - 1:ng + grid.nz + 1, ng - 1:ng + grid.ny + 1, ng - 1:ng + grid.nx + 1]
+    c = R[1:-1, 1:-1, 1:-1]
+    cx = 2.0 / (c + R[1:-1, 1:-1, 2:])
+    cy = 2.0 / (c + R[1:-1, 2:, 1:-1])
+    cz = 2.0 / (c + R[2:, 1:-1, 1:-1])
+    per = grid.periodic()
+    if not per[0]:
+        cx[:, :, -1] = 0.0
+    if not per[1]:
+        cy[:, -1, :] = 0.0
+    if not per[2]:
+        cz[-1, :, :] = 0.0
+    return torch.stack([cx, cy, cz]).contiguous()
+
+
+def sine_target(grid, seed=3, device="cuda"):
+    """p_target: 8 random-amplitude sine modes, wavenumbers 1..4."""
+    import torch
+    rng = np.random.default_rng(seed)
+    ax = [(torch.arange(m, dtype=torch.float64, device=device) + 0.5) / m
+          for m in (grid.nz, grid.ny, grid.nx)]
+    z = ax[0].view(-1, 1, 1)
+    y = ax[1].view(1, -1, 1)
+    x = ax[2].view(1, 1, -1)
+    t = torch.zeros((grid.nz, grid.ny, grid.nx), dtype=torch.float64, device=device)
+    for _ in range(8):
+        k = rng.integers(1, 5, size=3)
+        a = rng.uniform(-1, 1)
+        ph = rng.uniform(0, 2 * np.pi)
+        t += a * torch.sin(2 * np.pi * (int(k[0]) * x + int(k[1]) * y + int(k[2]) * z) + ph)
+    return t
+
+
+def objective(p, pt):
+    """J = 0.5 ||(p - mean p) - p_target||^2."""
+    p0 = p - p.mean()
+    return 0.5 * ((p0 - pt) ** 2).sum()
+
+
+def objective_diff(pp, pm, pt):
+    """J(p+) - J(p-) evaluated difference-first, so its rounding is relative
+    to the difference and not to J."""
+    d = pp - pm
+    d0 = d - d.mean()
+    sm = 0.5 * (pp + pm)
+    s0 = sm - sm.mean()
+    return float((d0 * (s0 - pt)).sum())
+
+
+def random_faces(grid, count, seed):
+    """Random (direction, k, j, i) faces, skipping closed wall faces."""
+    rng = np.random.default_rng(seed)
+    per = grid.periodic()
+    faces = []
+    while len(faces) < count:
+        d = int(rng.integers(0, 3))
+        k, j, i = (int(rng.integers(0, m)) for m in (grid.nz, grid.ny, grid.nx))
+        last = (i == grid.nx - 1, j == grid.ny - 1, k == grid.nz - 1)[d]
+        if last and not per[d]:
+            continue
+        faces.append((d, k, j, i))
+    return faces
