@@ -43,27 +43,31 @@ def level_dims(n3):
     return dims
 
 
-def solve_bytes(n3, iters, S, nu_pre=2, nu_post=2):
-    """Algorithmic bytes of one fast solve (every array element counted once
-    per launch that must touch it)."""
+def solve_bytes(n3, iters, S, nu_pre=2, nu_post=2, tail_cells=4096):
+    """Algorithmic bytes of one fast solve of a singular system whose A7 is
+    the coupling sum (every array element counted once per launch that must
+    touch it; S = bytes of the V-cycle type)."""
     dims = level_dims(n3)
     n0 = n3[0] * n3[1] * n3[2]
     v = 0.0
-    for (a, b, c) in dims[:-1]:
+    for li, (a, b, c) in enumerate(dims[:-1]):
         n = a * b * c
-        passes_first = 1
-        passes = (2 * nu_pre - 1) + 2 * nu_post
-        v += n * S * (4.5 * passes_first + 5.5 * passes)    # colour passes
-        v += n * S * (6 + 1 / 8)                            # residual + restriction
-        v += n * S * (2 + 1 / 8)                            # prolongation + correction
-    nb = dims[-1][0] * dims[-1][1] * dims[-1][2]
-    v += nb * nb * S + 2 * nb * S                           # dense bottom solve
+        if li > 0 and n <= tail_cells:
+            v += sum(x * y * z for (x, y, z) in dims[li:]) * S * 5    # LDS tail: staged once
+            break
+        v += n * S * (5 + 6 * (nu_pre - 1))                 # pre-smoothing (first: no x read)
+        v += n * S * ((6 + 1 / 8) + 6 * (nu_post - 1))      # post (first adds the correction)
+        v += n * S * (5 + 1 / 8)                            # residual + restriction
+    else:
+        nb = dims[-1][0] * dims[-1][1] * dims[-1][2]
+        v += nb * nb * S + 2 * nb * S                       # dense bottom solve
     mixed = S == 4
-    outer = n0 * ((8 + S) + (56 + S) + (48 + (4 if mixed else 0)))
-    setup = n0 * (120 + (48 if mixed else 0))               # pack (+ FP32 copy)
+    sz = S
+    outer = n0 * ((8 + sz) + (48 + sz) + (48 + (4 if mixed else 0)))  # dot, d & A d, update
+    setup = n0 * (72 + 56 + (20 if mixed else 0))           # A, p in; op, b, x, r (+ FP32) out
     for (a, b, c) in dims[1:]:
-        setup += a * b * c * S * (8 * 4 + 4)               # coarsening
-    final = n0 * (56 + (4 if mixed else 0)) + n0 * (72 + 16)  # initial residual, check, unpack
+        setup += a * b * c * S * (8 * 4 + 4)                # coarsening
+    final = n0 * (40 + 8)                                   # true residual + write-back of p
     return setup + iters * (v + outer) + final
 
 
@@ -231,7 +235,8 @@ def main():
     f64 = res["fp64"]
     n_cells = grid.cells
     roof = {
-        "kernel": "k_rb_sweep<float,false,1,true> (finest-level fused red-black Gauss-Seidel sweep, FP32)",
+        "kernel": "k_rb_sweep<float, false, 1, true, true, true, false> (finest-level fused red-black "
+                  "Gauss-Seidel sweep, FP32)",
         "bound": "hbm",
         "achieved": round(m["smoother_GBps"], 1),
         "peak": HBM_PEAK_GBPS,
