@@ -13,10 +13,11 @@ p = 0 to ||b - A p||_2 <= 1e-10 ||b||_2 on the device-resident 8-slot system.
 Also reported on the same line:
   * roofline: live HIP-event timing of every finest-level smoother launch
     during the timed region (algorithmic bytes / kernel time vs 8 TB/s);
-  * cpu_baseline: the reference algorithm (NewSolverMG restated in C, one
-    thread) timed on this host for a bounded number of V-cycles, scaled to
-    a whole solve with the iteration count of the same algorithm (counted by
-    its GPU restatement, whose residual history is the reference's).
+  * cpu_baseline: the reference algorithm (NewSolverMG restated in C, on
+    all OpenMP threads and on one) timed on this host for a bounded number
+    of V-cycles, scaled to a whole solve with the iteration count of the
+    same algorithm (counted by its GPU restatement, whose residual history
+    is the reference's).
 """
 import argparse
 import json
@@ -134,15 +135,13 @@ def reference_iterations(pps, torch, grid, A, tol, cap):
 
 
 def cpu_baseline(A_host, grid, cycles, n_solve_cycles, extrapolated):
+    """The reference algorithm (the oracle's NewSolverMG restatement) on this
+    host: `cycles` V-cycles on all cores the OpenMP runtime offers
+    (OMP_NUM_THREADS) and on one core, scaled to a whole solve."""
     import numpy as np
     import __graft_entry__ as ge
     orc = ge.load_oracle()
     og = orc.OrcGrid.make(grid.nx, grid.ny, grid.nz, bc=tuple(grid.bdry_cond))
-    p = np.zeros(og.shape_p)
-    t0 = time.perf_counter()
-    orc.newsolver_mg(og, A_host, p, 1e-300, 1.2, cycles)
-    dt = time.perf_counter() - t0
-    per_cycle = dt / cycles
     cpu = platform.processor() or platform.machine()
     try:
         with open("/proc/cpuinfo") as fh:
@@ -152,18 +151,32 @@ def cpu_baseline(A_host, grid, cycles, n_solve_cycles, extrapolated):
                     break
     except OSError:
         pass
+
+    def per_cycle(threads):
+        orc.set_threads(threads)
+        p = np.zeros(og.shape_p)
+        t0 = time.perf_counter()
+        orc.newsolver_mg(og, A_host, p, 1e-300, 1.2, cycles)
+        dt = time.perf_counter() - t0
+        orc.set_threads(1)
+        return dt / cycles
+
+    nt = orc.max_threads()
+    t_all = per_cycle(nt)
+    t_one = per_cycle(1)
+    how = "extrapolated" if extrapolated else "counted"
     return {
-        "value": per_cycle * n_solve_cycles * 1e3,
+        "value": t_all * n_solve_cycles * 1e3,
         "unit": "ms",
-        "cores": 1,
+        "cores": nt,
         "kind": "port",
         "sample": (f"{cycles} V-cycles of the reference NewSolverMG algorithm (oracle C port, "
-                   f"1 thread, {cpu}) on the same system: {per_cycle * 1e3:.1f} ms/V-cycle x "
-                   f"{n_solve_cycles:.0f} V-cycles to reach 1e-10 "
-                   f"({'extrapolated' if extrapolated else 'counted'} by the GPU restatement "
-                   f"of the same algorithm)"),
-        "ms_per_vcycle": per_cycle * 1e3,
+                   f"OpenMP, {nt} threads, {cpu}) on the same system: {t_all * 1e3:.1f} ms/V-cycle x "
+                   f"{n_solve_cycles:.0f} V-cycles to reach 1e-10 ({how} by the GPU restatement "
+                   f"of the same algorithm); one thread: {t_one * 1e3:.1f} ms/V-cycle"),
+        "ms_per_vcycle": t_all * 1e3,
         "vcycles_to_solve": n_solve_cycles,
+        "one_core": {"value": t_one * n_solve_cycles * 1e3, "ms_per_vcycle": t_one * 1e3, "cores": 1},
     }
 
 

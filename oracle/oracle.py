@@ -84,6 +84,8 @@ def lib():
         "orc_get_mgtest_err": (_D, [G, _P]),
         "orc_project_velocity": (None, [G, _P, _P, _P, _P, _P, _P, _P, _P, _D]),
         "orc_pcg_solve": (_I, [G, _P, _P, _D, _I, DP]),
+        "orc_set_threads": (None, [_I]),
+        "orc_max_threads": (_I, []),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -197,6 +199,15 @@ def get_mgtest_err(g, p):
 def project_velocity(g, u, v, w, um, vm, wm, rho, p, dt=1.0):
     lib().orc_project_velocity(ctypes.byref(g), _p(u), _p(v), _p(w), _p(um), _p(vm), _p(wm),
                                _p(rho), _p(p), dt)
+
+
+def set_threads(n):
+    """Threads of the NewSolverMG restatement (1 = serial)."""
+    lib().orc_set_threads(int(n))
+
+
+def max_threads():
+    return int(lib().orc_max_threads())
 
 
 def pcg_solve(g, A, p, rtol=1e-14, maxit=100000):

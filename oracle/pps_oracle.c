@@ -18,6 +18,9 @@
 #include "pps_oracle.h"
 
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdlib.h>
 #include <string.h>
 
@@ -44,6 +47,21 @@ static inline long ic(const lvl_t *L, int i, int j, int k) {
 static inline long ip(const lvl_t *L, int i, int j, int k) {
     return (long)(i + L->ng) + (long)L->X * ((long)(j + L->ng) + (long)L->Y * (k + L->ng));
 }
+/* Threads of the MG restatement's loops (1: serial, the order every test
+ * pins).  Colour passes, residuals and transfers are independent per cell,
+ * so more threads change only the summation order of the residual norms.
+ * Used to time the reference algorithm on all host cores. */
+static int orc_nthreads = 1;
+
+void orc_set_threads(int n) { orc_nthreads = n < 1 ? 1 : n; }
+int orc_max_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
 static inline long ncell(const lvl_t *L) { return (long)L->n[0] * L->n[1] * L->n[2]; }
 static inline long npad(const lvl_t *L) { return (long)L->X * L->Y * L->Z; }
 
@@ -279,6 +297,7 @@ static double compute_residual_lvl(const lvl_t *L, const double *A,
                                    const double *p, int norm) {
     long n = ncell(L), sx = 1, sy = L->X, sz = (long)L->X * L->Y;
     double acc = 0.0;
+#pragma omp parallel for if (orc_nthreads > 1) num_threads(orc_nthreads) reduction(+ : acc) schedule(static)
     for (int k = 0; k < L->n[2]; ++k)
         for (int j = 0; j < L->n[1]; ++j)
             for (int i = 0; i < L->n[0]; ++i) {
@@ -355,6 +374,8 @@ void orc_apply_bc(const orc_grid *g, double *p) {
 static void rb_relax_lvl(const lvl_t *L, const double *A, double *p, double beta) {
     long n = ncell(L), sx = 1, sy = L->X, sz = (long)L->X * L->Y;
     for (int pass = 0; pass < 2; ++pass) {
+        /* the cells of one colour are independent */
+#pragma omp parallel for if (orc_nthreads > 1) num_threads(orc_nthreads) schedule(static)
         for (int k = 0; k < L->n[2]; ++k)
             for (int j = 0; j < L->n[1]; ++j)
                 for (int i = (j + k + pass) & 1; i < L->n[0]; i += 2) {
@@ -379,6 +400,7 @@ void orc_redblack_relax(const orc_grid *g, const double *A, double *p,
 /* get_residual, in place on slot 8: MGsolver.f90:84-104 */
 static void get_residual_lvl(const lvl_t *L, double *A, const double *p) {
     long n = ncell(L), sx = 1, sy = L->X, sz = (long)L->X * L->Y;
+#pragma omp parallel for if (orc_nthreads > 1) num_threads(orc_nthreads) schedule(static)
     for (int k = 0; k < L->n[2]; ++k)
         for (int j = 0; j < L->n[1]; ++j)
             for (int i = 0; i < L->n[0]; ++i) {
@@ -397,6 +419,7 @@ void orc_fill_coefficients(const int nf[3], const double *Af, double *Ac) {
     long n_f = (long)nf[0] * nf[1] * nf[2], n_c = (long)nc[0] * nc[1] * nc[2];
 #define FI(i, j, k) ((long)(i) + (long)nf[0] * ((long)(j) + (long)nf[1] * (k)))
     for (int s = 0; s < 8; ++s)
+#pragma omp parallel for if (orc_nthreads > 1) num_threads(orc_nthreads) schedule(static)
         for (int k = 0; k < nc[2]; ++k)
             for (int j = 0; j < nc[1]; ++j)
                 for (int i = 0; i < nc[0]; ++i) {
@@ -416,6 +439,7 @@ void orc_coarse_from_fine(const int nf[3], const double *Af, double *Ac) {
     long n_f = (long)nf[0] * nf[1] * nf[2], n_c = (long)nc[0] * nc[1] * nc[2];
     const double *a = Af + 7 * n_f;
     double *o = Ac + 7 * n_c;
+#pragma omp parallel for if (orc_nthreads > 1) num_threads(orc_nthreads) schedule(static)
     for (int i = 0; i < nc[0]; ++i)
         for (int j = 0; j < nc[1]; ++j)
             for (int k = 0; k < nc[2]; ++k) {
@@ -435,6 +459,7 @@ void orc_coarse_fine_interp(const int nf[3], int ng, int sym_mg, double *ef,
     int nc[3] = {nf[0] / 2, nf[1] / 2, nf[2] / 2};
     int per0[3] = {0, 0, 0};
     lvl_t F = lvl_make(nf, ng, per0), C = lvl_make(nc, ng, per0);
+#pragma omp parallel for if (orc_nthreads > 1) num_threads(orc_nthreads) schedule(static)
     for (int i = 0; i < nc[0]; ++i)
         for (int j = 0; j < nc[1]; ++j)
             for (int k = 0; k < nc[2]; ++k) {
@@ -507,8 +532,12 @@ static int mg_init(mg_t *m, const orc_grid *g) {
     return ORC_OK;
 }
 
-static void vzero(double *a, long n) { memset(a, 0, sizeof(double) * n); }
+static void vzero(double *a, long n) {
+#pragma omp parallel for if (orc_nthreads > 1) num_threads(orc_nthreads) schedule(static)
+    for (long q = 0; q < n; ++q) a[q] = 0.0;
+}
 static void vadd(double *a, const double *b, long n) {
+#pragma omp parallel for if (orc_nthreads > 1) num_threads(orc_nthreads) schedule(static)
     for (long q = 0; q < n; ++q) a[q] = a[q] + b[q];
 }
 

@@ -121,6 +121,11 @@ void orc_project_velocity(const orc_grid *g, double *u, double *v, double *w,
 int orc_pcg_solve(const orc_grid *g, const double *A, double *p, double rtol,
                   int maxit, double *relres);
 
+/* Threads used by the NewSolverMG restatement (default 1 = serial, the
+ * order the tests pin); orc_max_threads() is OpenMP's maximum. */
+void orc_set_threads(int n);
+int orc_max_threads(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -274,3 +274,21 @@ def test_pcg_matches_direct(orc):
     its, rel = orc.pcg_solve(g, A, p, 1e-14, 100000)
     assert rel < 1e-13
     assert np.abs(_mf(g.interior(p)) - _mf(x)).max() < 1e-10 * np.abs(x).max()
+
+
+def test_newsolver_mg_threads_do_not_change_the_algorithm(orc):
+    """The OpenMP loops of the restatement (used to time the reference
+    algorithm on all host cores) update independent cells only: the iterate
+    is bit-identical to the serial run, the residual history equal to
+    summation-order rounding."""
+    g = orc.OrcGrid.make(32, bc=(1, 0, 0, 1, 0, 0))
+    A, _ = system(orc, g, [(0.3, 0.5, 0.5, 0.2)], rho2=1000.0, seed=4)
+    out = []
+    for nt in (1, max(2, orc.max_threads())):
+        orc.set_threads(nt)
+        p = np.zeros(g.shape_p)
+        rc, it, t2, hist = orc.newsolver_mg(g, A, p, 1e-300, 1.2, 6)
+        out.append((p, hist))
+    orc.set_threads(1)
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.allclose(out[0][1], out[1][1], rtol=1e-12, atol=0)
