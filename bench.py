@@ -180,6 +180,58 @@ def cpu_baseline(A_host, grid, cycles, n_solve_cycles, extrapolated):
     }
 
 
+def autograd_run(pps, torch, args):
+    """Forward (mixed solve from face values) + backward (adjoint solve and
+    face kernel) of PressureSolve, J = 0.5 ||p - mean(p) - p_target||^2."""
+    import numpy as np
+    print(f"[bench] building {args.workload}", file=sys.stderr, flush=True)
+    grid, A, f = pps.make_problem(args.workload)
+    ng = grid.ng
+    R = f["rho"][ng - 1:ng + grid.nz + 1, ng - 1:ng + grid.ny + 1, ng - 1:ng + grid.nx + 1]
+    c = R[1:-1, 1:-1, 1:-1]
+    cf = torch.stack([2.0 / (c + R[1:-1, 1:-1, 2:]), 2.0 / (c + R[1:-1, 2:, 1:-1]),
+                      2.0 / (c + R[2:, 1:-1, 1:-1])]).contiguous()
+    per = grid.periodic()
+    if not per[0]:
+        cf[0, :, :, -1] = 0.0
+    if not per[1]:
+        cf[1, :, -1, :] = 0.0
+    if not per[2]:
+        cf[2, -1, :, :] = 0.0
+    b = A[7].clone()
+    w = pps.face_weights(grid)
+    rng = np.random.default_rng(21)
+    ax = [(torch.arange(m, dtype=torch.float64, device="cuda") + 0.5) / m for m in (grid.nz, grid.ny, grid.nx)]
+    pt = torch.zeros((grid.nz, grid.ny, grid.nx), dtype=torch.float64, device="cuda")
+    for _ in range(8):
+        k = rng.integers(1, 5, size=3)
+        pt += rng.uniform(-1, 1) * torch.sin(2 * np.pi * (int(k[0]) * ax[2].view(1, 1, -1) + int(k[1]) * ax[1].view(1, -1, 1)
+                                                          + int(k[2]) * ax[0].view(-1, 1, 1)) + rng.uniform(0, 2 * np.pi))
+    PS = pps.PressureSolve
+    fwd, bwd = [], []
+    with pps.Solver(grid, method="mixed") as s:
+        for step in range(args.warmup + args.steps):
+            cc = cf.clone().requires_grad_(True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            p = PS.apply(cc, b, s, w, 1e-10, 200)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            it_f = len(s.history())
+            J = 0.5 * (((p - p.mean()) - pt) ** 2).sum()
+            J.backward()
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            it_b = len(s.history())
+            if step >= args.warmup:
+                fwd.append((t1 - t0) * 1e3)
+                bwd.append((t2 - t1) * 1e3)
+    print(json.dumps({"mode": "autograd", "workload": args.workload, "cells": grid.cells,
+                      "forward_ms": round(float(np.median(fwd)), 3), "backward_ms": round(float(np.median(bwd)), 3),
+                      "iterations": {"forward": it_f, "adjoint": it_b}, "steps": args.steps,
+                      "warmup": args.warmup, "dtype": "f32 V-cycle / f64 refinement"}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -193,6 +245,9 @@ def main():
                     help="time one form only (for counter runs)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="run one device copy of MIB MiB (counter calibration) and exit")
+    ap.add_argument("--autograd", action="store_true",
+                    help="time forward + backward of the differentiable solve on --workload "
+                         "(mixed form) instead of the solve benchmark")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the solver runs on a single MI355X "
@@ -217,6 +272,9 @@ def main():
         print(json.dumps({"copy_bytes_each_way": n * 8}))
         return
     pps = ge.load_package()
+    if args.autograd:
+        autograd_run(pps, torch, args)
+        return
 
     print(f"[bench] building {args.workload}", file=sys.stderr, flush=True)
     grid, A, _ = pps.make_problem(args.workload)

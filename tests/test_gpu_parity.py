@@ -184,6 +184,32 @@ def test_fast_mg_relative_1e10_iterations(pps, orc, method):
     assert t2 < 1e-10 * brms and it <= 20
 
 
+def test_sphere128_forms_agree(pps, orc):
+    """The 128^3 single-sphere workload (walls, density ratio 1000) to 1e-10
+    in both forms: the oracle's residual of each returned pressure meets the
+    tolerance, and the mixed and FP64 pressures agree in the max norm after
+    removing the null-space mean."""
+    _say("sphere128 fp64 vs mixed")
+    g, At, _ = pps.make_problem("sphere128")
+    og = orc.OrcGrid.make(128)
+    A = _host(At)
+    brms = np.sqrt((A[7] ** 2).mean())
+    ps = {}
+    for method in ("fp64", "mixed"):
+        with pps.Solver(g, method=method) as s:
+            p = torch.zeros(g.shape_p, dtype=torch.float64, device="cuda")
+            it, t2 = s.solve(At, p, 1e-10 * brms, 1.2, 100)
+        ph = _host(p)
+        r = orc.residual_field(og, A, ph)
+        rel = np.sqrt((r ** 2).mean()) / brms
+        _say(f"   {method}: it={it} oracle rel. residual {rel:.2e}")
+        assert it <= 20 and rel < 1e-10
+        ps[method] = _mf(og.interior(ph))
+    d = np.abs(ps["mixed"] - ps["fp64"]).max() / np.abs(ps["fp64"]).max()
+    _say(f"   max-norm difference mixed vs fp64 {d:.2e}")
+    assert d <= 1e-8
+
+
 @pytest.mark.parametrize("method,tol", [("fp64", 1e-12), ("mixed", 2e-5)])
 @pytest.mark.parametrize("case", ["walls", "perx", "allper", "aniso"])
 def test_fused_sweep_equals_colour_passes(pps, method, tol, case):
