@@ -40,6 +40,13 @@ template <class T> struct LevelK {
     const T *cx, *cy, *cz, *dg;
 };
 
+// sum over the 64 lanes of a wavefront (result in lane 0)
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
 __device__ __forceinline__ int wrap_m(int i, int n) { return i == 0 ? n - 1 : i - 1; }
 __device__ __forceinline__ int wrap_p(int i, int n) { return i == n - 1 ? 0 : i + 1; }
 
@@ -629,8 +636,10 @@ __device__ __forceinline__ double excess_of(const LevelK<T> &L, int i, int j, in
 }
 
 // one coarse cell from its children; the coarse -x/-y/-z couplings are
-// recomputed from the fine faces of the neighbouring coarse cell
-template <class Tf, class Tc>
+// recomputed from the fine faces of the neighbouring coarse cell.  DGI: the
+// V-cycle forms every diagonal from the couplings (singular systems), so
+// the coarse diagonal is not written.
+template <class Tf, class Tc, bool DGI>
 __global__ void __launch_bounds__(256)
 k_coarsen(LevelK<Tf> F, Tc *__restrict__ ccx, Tc *__restrict__ ccy, Tc *__restrict__ ccz,
           Tc *__restrict__ cdg) {
@@ -659,6 +668,9 @@ k_coarsen(LevelK<Tf> F, Tc *__restrict__ ccx, Tc *__restrict__ ccy, Tc *__restri
     };
     const double s = 1.0 / 16.0;
     const double vx = s * cpx(I, J, K), vy = s * cpy(I, J, K), vz = s * cpz(I, J, K);
+    const long long c = I + (long long)nx * (J + (long long)ny * K);
+    ccx[c] = (Tc)vx; ccy[c] = (Tc)vy; ccz[c] = (Tc)vz;
+    if (DGI) return;
     const double wx = s * cpx(wrap_m(I, nx), J, K), wy = s * cpy(I, wrap_m(J, ny), K),
                  wz = s * cpz(I, J, wrap_m(K, nz));
     double ex = 0.0;
@@ -668,8 +680,6 @@ k_coarsen(LevelK<Tf> F, Tc *__restrict__ ccx, Tc *__restrict__ ccy, Tc *__restri
                 const int i = 2 * I + di, j = 2 * J + dj, k = 2 * K + dk;
                 ex += excess_of(F, i, j, k, fi(i, j, k));
             }
-    const long long c = I + (long long)nx * (J + (long long)ny * K);
-    ccx[c] = (Tc)vx; ccy[c] = (Tc)vy; ccz[c] = (Tc)vz;
     cdg[c] = (Tc)(((vx + wx) + (vy + wy)) + (vz + wz) + 0.125 * ex);
 }
 
@@ -691,7 +701,10 @@ k_bottom_build(LevelK<T> L, int singular, double *__restrict__ M, T *__restrict_
     for (int e = tid; e < nb * nb; e += 1024) M[e] = 0.0;
     __syncthreads();
     double dsum = 0.0;
-    for (int c = 0; c < nb; ++c) dsum += (double)L.dg[c];
+    for (int c = 0; c < nb; ++c) {
+        const int i = c % L.nx, j = (c / L.nx) % L.ny, k = c / (L.nx * L.ny);
+        dsum += singular ? (double)diag_of<T, true>(L, i, j, k, c) : (double)L.dg[c];
+    }
     const double alpha = singular ? dsum / ((double)nb * (double)nb) : 0.0;
     for (int c = tid; c < nb; c += 1024) {
         const int i = c % L.nx, j = (c / L.nx) % L.ny, k = c / (L.nx * L.ny);
@@ -734,6 +747,72 @@ k_bottom_build(LevelK<T> L, int singular, double *__restrict__ M, T *__restrict_
         __syncthreads();
     }
     for (int e = tid; e < nb * nb; e += 1024) Minv[e] = (T)M[e];
+}
+
+// The same inverse in LDS for a coarsest level of nb <= 90 cells (the
+// matrix and copies of the pivot row and column fit 64 KB): per pivot, the
+// pivot row and column are copied, then every entry is updated from the
+// copies (the arithmetic of k_bottom_build), two barriers per pivot.
+// Requires nb <= 1024 threads (true for nb <= 90).
+template <class T>
+__global__ void __launch_bounds__(1024)
+k_bottom_build_lds(LevelK<T> L, int singular, T *__restrict__ Minv) {
+    extern __shared__ double Ms[];
+    __shared__ double red[16];
+    const int nb = L.nx * L.ny * L.nz;
+    const int tid = threadIdx.x;
+    constexpr int NT = 1024;
+    double *M = Ms, *rowp = Ms + nb * nb, *colp = rowp + nb;
+    for (int e = tid; e < nb * nb; e += NT) M[e] = 0.0;
+    // alpha = mean diagonal / nb (block reduction of the diagonals)
+    double dpart = 0.0;
+    for (int c = tid; c < nb; c += NT) {
+        const int i = c % L.nx, j = (c / L.nx) % L.ny, k = c / (L.nx * L.ny);
+        dpart += singular ? (double)diag_of<T, true>(L, i, j, k, c) : (double)L.dg[c];
+    }
+    dpart = wsum(dpart);
+    if ((tid & 63) == 0) red[tid >> 6] = dpart;
+    __syncthreads();
+    double dsum = 0.0;
+    for (int w = 0; w < NT / 64; ++w) dsum += red[w];
+    const double alpha = singular ? dsum / ((double)nb * (double)nb) : 0.0;
+    for (int c = tid; c < nb; c += NT) {
+        const int i = c % L.nx, j = (c / L.nx) % L.ny, k = c / (L.nx * L.ny);
+        const int sy = L.nx, sz = L.nx * L.ny;
+        const int ip = wrap_p(i, L.nx), jp = wrap_p(j, L.ny), kp = wrap_p(k, L.nz);
+        const int im = wrap_m(i, L.nx), jm = wrap_m(j, L.ny), km = wrap_m(k, L.nz);
+        double *row = M + c * nb;
+        row[c] += singular ? (double)diag_of<T, true>(L, i, j, k, c) : (double)L.dg[c];
+        row[c + (ip - i)] -= (double)L.cx[c];
+        row[c + (im - i)] -= (double)L.cx[c + (im - i)];
+        row[c + (jp - j) * sy] -= (double)L.cy[c];
+        row[c + (jm - j) * sy] -= (double)L.cy[c + (jm - j) * sy];
+        row[c + (kp - k) * sz] -= (double)L.cz[c];
+        row[c + (km - k) * sz] -= (double)L.cz[c + (km - k) * sz];
+        for (int e = 0; e < nb; ++e) row[e] += alpha;
+    }
+    __syncthreads();
+    // thread tid owns column tid % nb of rows tid / nb, tid / nb + rs, ...
+    const int col = tid % nb, r0 = tid / nb, rs = NT / nb;
+    for (int p = 0; p < nb; ++p) {
+        if (tid < nb) {
+            rowp[tid] = M[p * nb + tid];
+            colp[tid] = M[tid * nb + p];
+        }
+        __syncthreads();
+        const double piv = 1.0 / rowp[p];
+        if (r0 < rs) {
+            const double rc = rowp[col] * piv;
+            for (int r = r0; r < nb; r += rs) {
+                double *m = M + r * nb + col;
+                if (r == p) *m = col == p ? piv : rc;
+                else if (col == p) *m = -colp[r] * piv;
+                else *m -= colp[r] * rc;
+            }
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < nb * nb; e += NT) Minv[e] = (T)M[e];
 }
 
 template <class T>
@@ -896,11 +975,6 @@ k_tail(TailK<T> K) {
 // ---------------------------------------------------------------------
 enum { S_RZ = 0, S_ZMEAN, S_BETA, S_DQ, S_ALPHA, S_RR, S_BB, S_FIRST, S_N };
 
-__device__ __forceinline__ double wsum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    return v;
-}
 // Writes the block sum of v to part[block linear id].  Any block of at most
 // 1024 threads whose x extent is a multiple of 64 (each wavefront is 64
 // consecutive linear thread ids); wavefront sums are added in a fixed order.
@@ -1522,10 +1596,15 @@ static void hier_build(Hier<T> &H, int singular, hipStream_t s) {
     for (int l = 1; l < nl; ++l) {
         const Level<T> &F = H.lv[l - 1];
         Level<T> &C = H.lv[l];
-        k_coarsen<T, T><<<cgrid3(C.nx, C.ny, C.nz), kB3, 0, s>>>(F.k(), C.cx, C.cy, C.cz, C.dg);
+        if (singular) k_coarsen<T, T, true><<<cgrid3(C.nx, C.ny, C.nz), kB3, 0, s>>>(F.k(), C.cx, C.cy, C.cz, C.dg);
+        else k_coarsen<T, T, false><<<cgrid3(C.nx, C.ny, C.nz), kB3, 0, s>>>(F.k(), C.cx, C.cy, C.cz, C.dg);
     }
-    k_bottom_build<T><<<1, 1024, 0, s>>>(H.lv.back().k(), singular, H.work.template as<double>(),
-                                         H.minv.template as<T>());
+    const size_t lds = sizeof(double) * ((size_t)H.nb * H.nb + 2 * (size_t)H.nb);
+    if (lds <= 65536)
+        k_bottom_build_lds<T><<<1, 1024, lds, s>>>(H.lv.back().k(), singular, H.minv.template as<T>());
+    else
+        k_bottom_build<T><<<1, 1024, 0, s>>>(H.lv.back().k(), singular, H.work.template as<double>(),
+                                             H.minv.template as<T>());
 }
 
 // One colour pass of level l.  Algorithmic bytes of a pass on n cells:
