@@ -846,7 +846,6 @@ k_bottom_solve(int nb, const T *__restrict__ Minv, const T *__restrict__ b, T *_
 constexpr int kTailMax = 8;
 template <class T> struct TailK {
     int nl, nb, nu_pre, nu_post;
-    int per[3];
     LevelK<T> L[kTailMax];   // operators in global memory
     const T *b0;             // right-hand side of level 0 (global)
     T *x0;                   // iterate of level 0 (global, written at the end)
@@ -1498,7 +1497,6 @@ template <class T> struct Hier {
     std::vector<DevBuf> bufs;
     DevBuf minv, work;
     int nb = 0;
-    int per[3] = {0, 0, 0};
 };
 
 
@@ -1559,9 +1557,8 @@ static int plan_levels(const Dims &d, std::vector<int> &dims) {
 }
 
 template <class T>
-static int hier_alloc(Hier<T> &H, const std::vector<int> &dims, bool own_fine, const int per[3]) {
+static int hier_alloc(Hier<T> &H, const std::vector<int> &dims, bool own_fine) {
     const int nl = (int)dims.size() / 3;
-    for (int a = 0; a < 3; ++a) H.per[a] = per[a];
     H.lv.resize(nl);
     H.bufs.resize(nl);
     for (int l = 0; l < nl; ++l) {
@@ -1810,7 +1807,6 @@ static void vcycle(Hier<T> &H, const T *fineb, const Cycle &cy, Prof *prof, hipS
         K.nb = H.nb;
         K.nu_pre = nu_pre;
         K.nu_post = nu_post;
-        for (int a = 0; a < 3; ++a) K.per[a] = H.per[a];
         for (int l = lt; l < nl; ++l) K.L[l - lt] = H.lv[l].k();
         K.b0 = H.lv[lt].b;
         K.x0 = H.lv[lt].x;
@@ -1906,7 +1902,7 @@ static int fast_setup(pps_handle *h, const double *dA, const double *dp, bool mi
     PPS_TRY(plan_levels(d, dims));
     float *f32 = nullptr, *r32 = nullptr;
     if (mixed) {
-        if (!f->mixed_built) { PPS_TRY(hier_alloc(f->h32, dims, true, d.per)); f->mixed_built = 1; }
+        if (!f->mixed_built) { PPS_TRY(hier_alloc(f->h32, dims, true)); f->mixed_built = 1; }
         PPS_TRY(f->b32.ensure(sizeof(float) * n));
         f32 = f->h32.lv[0].cx;   // cx, cy, cz, dg are contiguous
         r32 = f->b32.as<float>();
@@ -1934,7 +1930,7 @@ static int fast_setup(pps_handle *h, const double *dA, const double *dp, bool mi
         hier_build(f->h32, f->singular, s);
     } else {
         Hier<double> &H = f->h64;
-        if (!f->fp64_built) { PPS_TRY(hier_alloc(H, dims, false, d.per)); f->fp64_built = 1; }
+        if (!f->fp64_built) { PPS_TRY(hier_alloc(H, dims, false)); f->fp64_built = 1; }
         Level<double> &L0 = H.lv[0];
         L0.cx = (double *)O.cx; L0.cy = (double *)O.cy; L0.cz = (double *)O.cz; L0.dg = (double *)O.dg;
         L0.b = nullptr;
