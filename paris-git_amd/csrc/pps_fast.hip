@@ -169,7 +169,7 @@ __device__ __forceinline__ CellOp<T> pick(const PairOp<T> &P, int e) {
 // FINE does not change the code; it gives the finest level's launches
 // their own symbol so profilers can separate them from coarse levels.
 // DGI: the diagonal is the coupling sum (not read).
-template <class T, bool FIRST, int PX, bool FINE, bool DGI, bool KEEP, bool PROL>
+template <class T, bool FIRST, int PX, bool FINE, bool DGI, bool PROL>
 __global__ void __launch_bounds__(1024)
 k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *__restrict__ xout,
            int c1, int zc, int oy, int ytiles, const T *__restrict__ ec) {
@@ -220,56 +220,52 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
         const int kw = wrapz(k);
         return xin[col(jx, kw, m) + e2x] + (PROL ? ec[par(jx, kw, m)] : (T)0);
     };
-    auto stage = [&](int k, int m, T a0, T a1, T ex) {
+    auto stage = [&](int k, int m, T xsec, T ex) {
         const int buf = k & 1;
-        const int e2 = (c1 + j + k + 1) & 1;   // offset of the second colour
-        X(buf, jj + 1, m) = e2 ? a1 : a0;
+        X(buf, jj + 1, m) = xsec;
         if (ring_lo) X(buf, 0, m) = ex;
         if (ring_hi) X(buf, rb + 1, m) = ex;
     };
 
-    // per pair: xin at planes s-1 .. s+2 (cells a = 2m, b = 2m+1), first
-    // colour new at s-2 .. s, operator of planes s (cur) and s+1 (nxt).
-    // KEEP: the operator of plane s-1 (prv) and the +z couplings of plane
-    // s-2 (pza, pzb) stay in registers for the second-colour update and the
-    // -z couplings; otherwise they are read again from memory, where they
-    // were loaded one step earlier (fewer registers, about 1.4x the fetched
-    // bytes of the FP32 sweep at 256^3).
-    T xa0[PX], xb0[PX], xa1[PX], xb1[PX], xa2[PX], xb2[PX], xa3[PX], xb3[PX], ex2[PX], ex3[PX];
-    T r_m2[PX], r_m1[PX], r_0[PX];
-    PairOp<T> cur[PX], nxt[PX], prv[PX];
-    T pza[PX], pzb[PX];
+    // xin of the second colour of plane q for pair m (the only xin value a
+    // sweep reads: the first colour's old value is overwritten unread), plus
+    // the coarse correction under PROL
+    auto sc = [&](int q, int m) -> T {
+        if (FIRST) return (T)0;
+        const int e2 = (c1 + j + q + 1) & 1;
+        const int kw = wrapz(q);
+        return xin[col(j, kw, m) + e2] + (PROL ? ec[par(j, kw, m)] : (T)0);
+    };
 
-    // prologue: xin planes k0-2 .. k0 (+ extra row of k0), operator of
-    // k0-1 and the +z couplings of k0-2; stage plane k0-1
+    // per pair: second-colour xin at planes s-1 .. s+2 (xs0..xs3), first
+    // colour new at s-2 .. s, operator of planes s (cur) and s+1 (nxt), the
+    // second-colour cell's operator of plane s-1 (kop: its -z coupling is
+    // the first colour's at s), and the first colour's +z coupling of
+    // planes s-1 and s-2 (zr1, zr2: the second colour's -z coupling one
+    // plane up).
+    T xs0[PX], xs1[PX], xs2[PX], xs3[PX], ex2[PX], ex3[PX];
+    T r_m2[PX], r_m1[PX], r_0[PX];
+    PairOp<T> cur[PX], nxt[PX];
+    CellOp<T> kop[PX];
+    T zr1[PX], zr2[PX];
+
+    // prologue: planes k0-2 .. k0 (+ extra row of k0), operator of k0-1,
+    // the +z couplings of k0-2; stage plane k0-1
 #pragma unroll
     for (int p = 0; p < PX; ++p) {
         const int m = threadIdx.x + p * tpr;
-        if (FIRST) {
-            xa0[p] = xb0[p] = xa1[p] = xb1[p] = xa2[p] = xb2[p] = (T)0;
-        } else {
-            const int ka = wrapz(k0 - 2), kb = wrapz(k0 - 1), kc = wrapz(k0);
-            const long long ca = col(j, ka, m), cb = col(j, kb, m), cc = col(j, kc, m);
-            xa0[p] = xin[ca]; xb0[p] = xin[ca + 1];
-            xa1[p] = xin[cb]; xb1[p] = xin[cb + 1];
-            xa2[p] = xin[cc]; xb2[p] = xin[cc + 1];
-            if (PROL) {
-                const T ea = ec[par(j, ka, m)], eb = ec[par(j, kb, m)], ecc = ec[par(j, kc, m)];
-                xa0[p] += ea; xb0[p] += ea;
-                xa1[p] += eb; xb1[p] += eb;
-                xa2[p] += ecc; xb2[p] += ecc;
-            }
-        }
+        xs0[p] = sc(k0 - 2, m);
+        xs1[p] = sc(k0 - 1, m);
+        xs2[p] = sc(k0, m);
         ex2[p] = load_extra(k0, m);
         load_op(k0 - 1, m, cur[p]);
-        if (KEEP) {
-            const long long cz2 = col(j, wrapz(k0 - 2), m);
-            prv[p].cza = L.cz[cz2];
-            prv[p].czb = L.cz[cz2 + 1];
-            pza[p] = pzb[p] = (T)0;
-        }
+        const int e1p = (c1 + j + k0 - 1) & 1;   // first colour at k0-1
+        const long long cz2 = col(j, wrapz(k0 - 2), m);
+        kop[p].cz = L.cz[cz2 + e1p];
+        zr1[p] = L.cz[cz2 + (1 - e1p)];
+        zr2[p] = (T)0;
         r_m2[p] = r_m1[p] = r_0[p] = (T)0;
-        stage(k0 - 1, m, xa1[p], xb1[p], load_extra(k0 - 1, m));
+        stage(k0 - 1, m, xs1[p], load_extra(k0 - 1, m));
     }
     lds_barrier();
 
@@ -281,42 +277,28 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
         for (int p = 0; p < PX; ++p) {
             const int m = threadIdx.x + p * tpr;
             if (more) {
-                if (FIRST) {
-                    xa3[p] = xb3[p] = (T)0;
-                } else {
-                    const int kw = wrapz(s + 2);
-                    const long long cc = col(j, kw, m);
-                    xa3[p] = xin[cc];
-                    xb3[p] = xin[cc + 1];
-                    if (PROL) {
-                        const T e = ec[par(j, kw, m)];
-                        xa3[p] += e;
-                        xb3[p] += e;
-                    }
-                }
+                xs3[p] = sc(s + 2, m);
                 ex3[p] = load_extra(s + 2, m);
                 load_op(s + 1, m, nxt[p]);
             }
         }
         // R: first colour at plane s
         const int bs = s & 1;
+        T zrs[PX];
 #pragma unroll
         for (int p = 0; p < PX; ++p) {
             const int m = threadIdx.x + p * tpr;
             const CellOp<T> o = pick(cur[p], e1);
+            const T zm = kop[p].cz;
+            zrs[p] = o.cz;
             T v = o.b;
-            T zm = (T)0;
-            if (KEEP) zm = e1 ? prv[p].czb : prv[p].cza;
-            else if (!FIRST || DGI) zm = L.cz[col(j, wrapz(s - 1), m) + e1];
             if (!FIRST) {
                 const int mm = m == 0 ? np - 1 : m - 1, mp = m == np - 1 ? 0 : m + 1;
-                const T xw = e1 == 0 ? X(bs, jj + 1, mm) : xa1[p];
-                const T xe = e1 == 0 ? xb1[p] : X(bs, jj + 1, mp);
-                const T xbz = e1 ? xb0[p] : xa0[p];
-                const T xtz = e1 ? xb2[p] : xa2[p];
+                const T xw = e1 == 0 ? X(bs, jj + 1, mm) : xs1[p];
+                const T xe = e1 == 0 ? xs1[p] : X(bs, jj + 1, mp);
                 v += o.cxm * xw + o.cx * xe;
                 v += o.cym * X(bs, jj, m) + o.cy * X(bs, jj + 2, m);
-                v += zm * xbz + o.cz * xtz;
+                v += zm * xs0[p] + o.cz * xs2[p];
             }
             v /= DGI ? diag_sum(o.cxm, o.cx, o.cym, o.cy, zm, o.cz) : o.dg;
             R(bs, jj, m) = v;
@@ -333,28 +315,16 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
             for (int p = 0; p < PX; ++p) {
                 const int m = threadIdx.x + p * tpr;
                 const int mm = m == 0 ? np - 1 : m - 1, mp = m == np - 1 ? 0 : m + 1;
-                CellOp<T> o;
-                T zm;
-                if (KEEP) {
-                    o = pick(prv[p], e2);
-                    zm = e2 ? pzb[p] : pza[p];
-                } else {
-                    const long long c = col(j, wrapz(t), m) + e2;
-                    const long long cxm = (m == 0 && e2 == 0) ? c + (nx - 1) : c - 1;
-                    o.cx = L.cx[c]; o.cxm = L.cx[cxm];
-                    o.cy = L.cy[c]; o.cym = L.cy[c + dym];
-                    o.cz = L.cz[c]; o.b = b[c];
-                    if (!DGI) o.dg = L.dg[c];
-                    zm = L.cz[col(j, wrapz(t - 1), m) + e2];
-                }
-                if (DGI) o.dg = diag_sum(o.cxm, o.cx, o.cym, o.cy, zm, o.cz);
+                const CellOp<T> &o = kop[p];
+                const T zm = zr2[p];
+                const T d = DGI ? diag_sum(o.cxm, o.cx, o.cym, o.cy, zm, o.cz) : o.dg;
                 const T xw = e2 == 0 ? R(bt, jj, mm) : r_m1[p];
                 const T xe = e2 == 0 ? r_m1[p] : R(bt, jj, mp);
                 T v = o.b;
                 v += o.cxm * xw + o.cx * xe;
                 v += o.cym * R(bt, jj - 1, m) + o.cy * R(bt, jj + 1, m);
                 v += zm * r_m2[p] + o.cz * r_0[p];
-                v /= o.dg;
+                v /= d;
                 const long long c0 = col(j, wrapz(t), m);
                 xout[c0] = e2 ? r_m1[p] : v;
                 xout[c0 + 1] = e2 ? v : r_m1[p];
@@ -364,16 +334,14 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
 #pragma unroll
         for (int p = 0; p < PX; ++p) {
             const int m = threadIdx.x + p * tpr;
-            if (more) stage(s + 1, m, xa2[p], xb2[p], ex2[p]);
-            if (KEEP) {
-                pza[p] = prv[p].cza;
-                pzb[p] = prv[p].czb;
-                prv[p] = cur[p];
-            }
+            if (more) stage(s + 1, m, xs2[p], ex2[p]);
+            kop[p] = pick(cur[p], 1 - e1);   // second colour of plane s
+            zr2[p] = zr1[p];
+            zr1[p] = zrs[p];
             cur[p] = nxt[p];
-            xa0[p] = xa1[p]; xb0[p] = xb1[p];
-            xa1[p] = xa2[p]; xb1[p] = xb2[p];
-            xa2[p] = xa3[p]; xb2[p] = xb3[p];
+            xs0[p] = xs1[p];
+            xs1[p] = xs2[p];
+            xs2[p] = xs3[p];
             ex2[p] = ex3[p];
         }
         lds_barrier();
@@ -1636,7 +1604,8 @@ static void rb_pass(const Level<T> &L, int l, const T *b, int colour, bool first
 
 // Launch shape of the fused sweep on a level: pairs per thread, thread
 // rows per block, planes per block.  Blocks hold at most 1024 threads and
-// 64 KB of LDS; about 1024 blocks fill the chip.
+// 64 KB of LDS; about 1024 blocks fill the chip.  (Two pairs per thread
+// halve the ring-row overhead but ran at half the speed at 256^3.)
 struct SweepShape {
     int px, tpr, rb, oy, zc, ytiles;
     dim3 grid, block;
@@ -1708,23 +1677,23 @@ struct Cycle {
 // swap).  Algorithmic bytes: the operator planes (3 couplings, plus the
 // diagonal unless dgi), b, the old iterate (unless FIRST) and the new
 // iterate, once each.
-template <class T, bool FIRST, bool DGI, bool KEEP, bool PROL>
+template <class T, bool FIRST, bool DGI, bool PROL>
 static void sweep_launch1(const SweepShape &sh, const Level<T> &L, int l, const T *b, int c1,
                           const T *ec, hipStream_t s) {
     const LevelK<T> K = L.k();
     if (l == 0)
-        k_rb_sweep<T, FIRST, 1, true, DGI, KEEP, PROL><<<sh.grid, sh.block, sh.lds, s>>>(
+        k_rb_sweep<T, FIRST, 1, true, DGI, PROL><<<sh.grid, sh.block, sh.lds, s>>>(
             K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec);
     else
-        k_rb_sweep<T, FIRST, 1, false, DGI, KEEP, PROL><<<sh.grid, sh.block, sh.lds, s>>>(
+        k_rb_sweep<T, FIRST, 1, false, DGI, PROL><<<sh.grid, sh.block, sh.lds, s>>>(
             K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec);
 }
-template <class T, bool DGI, bool KEEP>
+template <class T, bool DGI>
 static void sweep_launch(const SweepShape &sh, const Level<T> &L, int l, const T *b, int c1, bool first,
                          const T *ec, hipStream_t s) {
-    if (first) sweep_launch1<T, true, DGI, KEEP, false>(sh, L, l, b, c1, nullptr, s);
-    else if (ec) sweep_launch1<T, false, DGI, KEEP, true>(sh, L, l, b, c1, ec, s);
-    else sweep_launch1<T, false, DGI, KEEP, false>(sh, L, l, b, c1, nullptr, s);
+    if (first) sweep_launch1<T, true, DGI, false>(sh, L, l, b, c1, nullptr, s);
+    else if (ec) sweep_launch1<T, false, DGI, true>(sh, L, l, b, c1, ec, s);
+    else sweep_launch1<T, false, DGI, false>(sh, L, l, b, c1, nullptr, s);
 }
 
 // Whether level l's sweeps run as one fused kernel (else two colour passes).
@@ -1751,8 +1720,8 @@ static void rb_sweep(Level<T> &L, int l, const T *b, int c1, bool first, const C
     const double bytes = (double)L.n * sizeof(T) * (dgi ? 6.0 : 7.0);
     hipEvent_t *e = (l == 0 && !first && !ec && prof) ? prof->begin(bytes) : nullptr;
     if (e) (void)hipEventRecord(e[0], s);
-    if (dgi) sweep_launch<T, true, true>(sh, L, l, b, c1, first, ec, s);
-    else sweep_launch<T, false, true>(sh, L, l, b, c1, first, ec, s);
+    if (dgi) sweep_launch<T, true>(sh, L, l, b, c1, first, ec, s);
+    else sweep_launch<T, false>(sh, L, l, b, c1, first, ec, s);
     if (e) (void)hipEventRecord(e[1], s);
     T *t = L.x;
     L.x = L.xt;
