@@ -118,27 +118,38 @@ k_rb(LevelK<T> L, const B *__restrict__ b, T *__restrict__ x, int colour) {
 // values for the neighbours of the rb-2 output rows) and marches through a
 // chunk of planes.  At plane step s it
 //   R: updates the first colour of plane s (all rows, ring included) from
-//      the second-colour xin of plane s staged in LDS and the thread's own
-//      column at s-1 / s+1, staging the results in LDS,
+//      the second-colour xin of planes s-1, s, s+1 staged in LDS, staging
+//      the results in LDS,
 //   K: updates the second colour of plane s-1 (output rows) from the
 //      first-colour values of planes s-2, s-1 (LDS) and s, and stores both
 //      cells of the pair at plane s-1,
-//   S: stages the second-colour xin of plane s+1 for the next step.
+//   S: stages the second-colour xin of plane s+2.
 // Every first-colour value uses only old second-colour values and every
 // second-colour value only new first-colour values, so the result equals
-// the two colour passes of k_rb.  Global loads run one plane ahead
-// (xin of plane s+2 and the operator of plane s+1 are requested at step s
-// and consumed at step s+1) and the per-plane barrier orders LDS only, so
-// the prefetches stay in flight across it.  Redundant work: the ring rows
-// and one extra first-colour plane at each end of a chunk.
+// the two colour passes of k_rb.  Only the second colour's xin is read (the
+// first colour's old value is overwritten unread).
+// PF: loads run ahead of their use: step s requests the operator of plane
+// s+1 and the xin of plane s+3, which steps s+1 and s+2 consume.  The
+// operator registers alternate between two sets by step parity (the loop
+// is unrolled by two), so no register copy forces a loaded value to arrive
+// within the step that requested it, and every load is unconditional and
+// in bounds (past the chunk's end the last needed plane is loaded again;
+// threads off the ring load their own row twice), so the waits the
+// compiler places stay one step behind the requests.  Without PF a step
+// loads the operator of plane s and the xin of plane s+2 itself.  The
+// per-plane barrier orders LDS only.  Redundant work: the ring rows and one extra
+// first-colour plane at each end of a chunk.
 // c1: first colour ((x+y+z)&1 of the cells updated first).  FIRST: xin is
 // identically zero (not read).  PROL: the coarse correction ec is added to
-// every xin value as it is loaded (xin + P ec, P the piecewise-constant
+// every xin value before it is staged (xin + P ec, P the piecewise-constant
 // prolongation: both cells of a pair share the parent m), so the sweep
 // that follows a coarse-grid correction needs no separate prolongation
 // pass.  PX pairs per thread.
-// LDS: X[2][rb+2][np] (second-colour xin), R[2][rb][np] (first colour new).
+// LDS: X[4][rb+2][np] (second-colour xin, plane k in slot k & 3),
+// R[2][rb][np] (first colour new).
 // ---------------------------------------------------------------------
+template <int I> struct Ic { static constexpr int value = I; };
+
 __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
     __builtin_amdgcn_s_barrier();
@@ -169,7 +180,7 @@ __device__ __forceinline__ CellOp<T> pick(const PairOp<T> &P, int e) {
 // FINE does not change the code; it gives the finest level's launches
 // their own symbol so profilers can separate them from coarse levels.
 // DGI: the diagonal is the coupling sum (not read).
-template <class T, bool FIRST, int PX, bool FINE, bool DGI, bool PROL>
+template <class T, bool FIRST, int PX, bool FINE, bool DGI, bool PROL, bool PF>
 __global__ void __launch_bounds__(1024)
 k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *__restrict__ xout,
            int c1, int zc, int oy, int ytiles, const T *__restrict__ ec) {
@@ -181,10 +192,10 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
     const int rb = blockDim.y;              // thread rows (oy + 2)
     const int jj = threadIdx.y;
     const long long sy = nx, sz = (long long)nx * ny;
-    T *Xs = sm;                             // [2][rb+2][np]
-    T *Rs = sm + 2 * (rb + 2) * np;         // [2][rb][np]
-    auto X = [&](int buf, int row, int m) -> T & { return Xs[(buf * (rb + 2) + row) * np + m]; };
-    auto R = [&](int buf, int row, int m) -> T & { return Rs[(buf * rb + row) * np + m]; };
+    T *Xs = sm;                             // [4][rb+2][np]
+    T *Rs = sm + 4 * (rb + 2) * np;         // [2][rb][np]
+    auto X = [&](int k, int row, int m) -> T & { return Xs[((k & 3) * (rb + 2) + row) * np + m]; };
+    auto R = [&](int k, int row, int m) -> T & { return Rs[((k & 1) * rb + row) * np + m]; };
     const int tile = blockIdx.x % ytiles, chunk = blockIdx.x / ytiles;
     const int y0 = tile * oy;
     const int k0 = chunk * zc;
@@ -196,7 +207,9 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
     const long long dym = (long long)(jm - j) * sy;
     const bool out_row = jj >= 1 && jj <= oy && (y0 - 1 + jj) < ny;
     const bool ring_lo = jj == 0, ring_hi = jj == rb - 1;
-    const int jx = ring_lo ? wrapy(j - 1) : wrapy(j + 1);   // extra staged row
+    // the row of a thread's second xin load: the staged row beyond the ring
+    // for ring threads, its own row (a duplicate load) otherwise
+    const int jl = ring_lo ? wrapy(j - 1) : (ring_hi ? wrapy(j + 1) : j);
     auto col = [&](int jrow, int k, int m) { return 2LL * m + sy * jrow + sz * k; };
 
     auto load_op = [&](int k, int m, PairOp<T> &P) {
@@ -209,99 +222,107 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
         if (!DGI) { P.dga = L.dg[c]; P.dgb = L.dg[c + 1]; }
         P.ba = b[c]; P.bb = b[c + 1];
     };
-    // second-colour xin of the extra row (ring threads only)
     // parent of pair m of row jrow, plane k (wrapped) in the coarse grid
     auto par = [&](int jrow, int k, int m) {
         return (long long)m + (long long)(nx >> 1) * ((jrow >> 1) + (long long)(ny >> 1) * (k >> 1));
     };
-    auto load_extra = [&](int k, int m) -> T {
-        if (FIRST || !(ring_lo || ring_hi)) return (T)0;
-        const int e2x = (c1 + jx + k + 1) & 1;
-        const int kw = wrapz(k);
-        return xin[col(jx, kw, m) + e2x] + (PROL ? ec[par(jx, kw, m)] : (T)0);
+    // second-colour xin of row jrow, plane q; its coarse correction
+    auto ldx = [&](int jrow, int q, int m) -> T {
+        return xin[col(jrow, wrapz(q), m) + ((c1 + jrow + q + 1) & 1)];
     };
-    auto stage = [&](int k, int m, T xsec, T ex) {
-        const int buf = k & 1;
-        X(buf, jj + 1, m) = xsec;
-        if (ring_lo) X(buf, 0, m) = ex;
-        if (ring_hi) X(buf, rb + 1, m) = ex;
+    auto lde = [&](int jrow, int q, int m) -> T { return ec[par(jrow, wrapz(q), m)]; };
+    auto stage = [&](int k, int m, T xv, T ev) {
+        X(k, jj + 1, m) = xv;
+        if (ring_lo) X(k, 0, m) = ev;
+        if (ring_hi) X(k, rb + 1, m) = ev;
     };
 
-    // xin of the second colour of plane q for pair m (the only xin value a
-    // sweep reads: the first colour's old value is overwritten unread), plus
-    // the coarse correction under PROL
-    auto sc = [&](int q, int m) -> T {
-        if (FIRST) return (T)0;
-        const int e2 = (c1 + j + q + 1) & 1;
-        const int kw = wrapz(q);
-        return xin[col(j, kw, m) + e2] + (PROL ? ec[par(j, kw, m)] : (T)0);
-    };
-
-    // per pair: second-colour xin at planes s-1 .. s+2 (xs0..xs3), first
-    // colour new at s-2 .. s, operator of planes s (cur) and s+1 (nxt), the
-    // second-colour cell's operator of plane s-1 (kop: its -z coupling is
-    // the first colour's at s), and the first colour's +z coupling of
-    // planes s-1 and s-2 (zr1, zr2: the second colour's -z coupling one
-    // plane up).
-    T xs0[PX], xs1[PX], xs2[PX], xs3[PX], ex2[PX], ex3[PX];
-    T r_m2[PX], r_m1[PX], r_0[PX];
-    PairOp<T> cur[PX], nxt[PX];
+    // per pair: operator of planes s and s+1 (op[P], op[1-P] at a step of
+    // parity P), the second-colour cell's operator of plane s-1 (kop: its
+    // -z coupling is the first colour's at s), the first colour's +z
+    // coupling of planes s-1 and s-2 (zr1, zr2: the second colour's -z
+    // coupling one plane up), first colour new at s-2 .. s, and the xin of
+    // own and extra row of plane s+2 (set P) and s+3 (set 1-P), with their
+    // coarse corrections under PROL.
+    PairOp<T> op[2][PX];
     CellOp<T> kop[PX];
-    T zr1[PX], zr2[PX];
+    T zr1[PX], zr2[PX], r_m2[PX], r_m1[PX], r_0[PX];
+    T xr[2][PX], er[2][PX], xq[2][PX], eq[2][PX];
+    const int s0 = k0 - 1;
+    const int klast = k1 + 1;   // last xin plane a step reads
 
-    // prologue: planes k0-2 .. k0 (+ extra row of k0), operator of k0-1,
-    // the +z couplings of k0-2; stage plane k0-1
+    // prologue: operator of plane s0, the +z couplings of s0-1; stage the
+    // xin of planes s0-1 .. s0+1, request s0+2
 #pragma unroll
     for (int p = 0; p < PX; ++p) {
         const int m = threadIdx.x + p * tpr;
-        xs0[p] = sc(k0 - 2, m);
-        xs1[p] = sc(k0 - 1, m);
-        xs2[p] = sc(k0, m);
-        ex2[p] = load_extra(k0, m);
-        load_op(k0 - 1, m, cur[p]);
-        const int e1p = (c1 + j + k0 - 1) & 1;   // first colour at k0-1
-        const long long cz2 = col(j, wrapz(k0 - 2), m);
+        if (PF) load_op(s0, m, op[0][p]);
+        if (!FIRST) {
+            for (int q = s0 - 1; q <= s0 + 1; ++q) {
+                const T xv = ldx(j, q, m) + (PROL ? lde(j, q, m) : (T)0);
+                const T ev = ldx(jl, q, m) + (PROL ? lde(jl, q, m) : (T)0);
+                stage(q, m, xv, ev);
+            }
+            if (PF) {
+                xr[0][p] = ldx(j, s0 + 2, m);
+                er[0][p] = ldx(jl, s0 + 2, m);
+                if (PROL) {
+                    xq[0][p] = lde(j, s0 + 2, m);
+                    eq[0][p] = lde(jl, s0 + 2, m);
+                }
+            }
+        }
+        const int e1p = (c1 + j + s0) & 1;   // first colour at s0
+        const long long cz2 = col(j, wrapz(s0 - 1), m);
         kop[p].cz = L.cz[cz2 + e1p];
         zr1[p] = L.cz[cz2 + (1 - e1p)];
         zr2[p] = (T)0;
         r_m2[p] = r_m1[p] = r_0[p] = (T)0;
-        stage(k0 - 1, m, xs1[p], load_extra(k0 - 1, m));
     }
     lds_barrier();
 
-    for (int s = k0 - 1; s <= k1; ++s) {
+    auto step = [&](auto PC, int s) {
+        // PF: this step uses set PU and requests into set PN for later
+        // steps; else it loads what it uses into set 0
+        constexpr int PU = PF ? decltype(PC)::value : 0;
+        constexpr int PN = PF ? 1 - PU : 0;
         const int e1 = (c1 + j + s) & 1;   // first-colour offset at plane s
-        const bool more = s < k1;
-        // prefetch: xin of plane s+2, operator of plane s+1
+        // requests: operator of plane s+1 and xin of plane s+3 (PF), or
+        // operator of plane s and xin of plane s+2
+        const int kn = PF ? min(s + 1, k1) : s;
+        const int qn = min(PF ? s + 3 : s + 2, klast);
 #pragma unroll
         for (int p = 0; p < PX; ++p) {
             const int m = threadIdx.x + p * tpr;
-            if (more) {
-                xs3[p] = sc(s + 2, m);
-                ex3[p] = load_extra(s + 2, m);
-                load_op(s + 1, m, nxt[p]);
+            load_op(kn, m, op[PN][p]);
+            if (!FIRST) {
+                xr[PN][p] = ldx(j, qn, m);
+                er[PN][p] = ldx(jl, qn, m);
+                if (PROL) {
+                    xq[PN][p] = lde(j, qn, m);
+                    eq[PN][p] = lde(jl, qn, m);
+                }
             }
         }
         // R: first colour at plane s
-        const int bs = s & 1;
         T zrs[PX];
 #pragma unroll
         for (int p = 0; p < PX; ++p) {
             const int m = threadIdx.x + p * tpr;
-            const CellOp<T> o = pick(cur[p], e1);
+            const CellOp<T> o = pick(op[PU][p], e1);
             const T zm = kop[p].cz;
             zrs[p] = o.cz;
             T v = o.b;
             if (!FIRST) {
                 const int mm = m == 0 ? np - 1 : m - 1, mp = m == np - 1 ? 0 : m + 1;
-                const T xw = e1 == 0 ? X(bs, jj + 1, mm) : xs1[p];
-                const T xe = e1 == 0 ? xs1[p] : X(bs, jj + 1, mp);
+                const T xw = X(s, jj + 1, e1 == 0 ? mm : m);
+                const T xe = X(s, jj + 1, e1 == 0 ? m : mp);
                 v += o.cxm * xw + o.cx * xe;
-                v += o.cym * X(bs, jj, m) + o.cy * X(bs, jj + 2, m);
-                v += zm * xs0[p] + o.cz * xs2[p];
+                v += o.cym * X(s, jj, m) + o.cy * X(s, jj + 2, m);
+                v += zm * X(s - 1, jj + 1, m) + o.cz * X(s + 1, jj + 1, m);
             }
             v /= DGI ? diag_sum(o.cxm, o.cx, o.cym, o.cy, zm, o.cz) : o.dg;
-            R(bs, jj, m) = v;
+            R(s, jj, m) = v;
             r_m2[p] = r_m1[p];
             r_m1[p] = r_0[p];
             r_0[p] = v;
@@ -309,7 +330,6 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
         // K: second colour at plane s-1 (offset e1 there), output
         const int t = s - 1;
         if (out_row && t >= k0 && t < k1) {
-            const int bt = t & 1;
             const int e2 = e1;
 #pragma unroll
             for (int p = 0; p < PX; ++p) {
@@ -318,11 +338,11 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
                 const CellOp<T> &o = kop[p];
                 const T zm = zr2[p];
                 const T d = DGI ? diag_sum(o.cxm, o.cx, o.cym, o.cy, zm, o.cz) : o.dg;
-                const T xw = e2 == 0 ? R(bt, jj, mm) : r_m1[p];
-                const T xe = e2 == 0 ? r_m1[p] : R(bt, jj, mp);
+                const T xw = e2 == 0 ? R(t, jj, mm) : r_m1[p];
+                const T xe = e2 == 0 ? r_m1[p] : R(t, jj, mp);
                 T v = o.b;
                 v += o.cxm * xw + o.cx * xe;
-                v += o.cym * R(bt, jj - 1, m) + o.cy * R(bt, jj + 1, m);
+                v += o.cym * R(t, jj - 1, m) + o.cy * R(t, jj + 1, m);
                 v += zm * r_m2[p] + o.cz * r_0[p];
                 v /= d;
                 const long long c0 = col(j, wrapz(t), m);
@@ -330,21 +350,25 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
                 xout[c0 + 1] = e2 ? v : r_m1[p];
             }
         }
-        // S: stage the second colour of plane s+1; rotate the registers
+        // S: stage the xin of plane s+2 (requested at step s-1); keep the
+        // second colour's operator of plane s
 #pragma unroll
         for (int p = 0; p < PX; ++p) {
             const int m = threadIdx.x + p * tpr;
-            if (more) stage(s + 1, m, xs2[p], ex2[p]);
-            kop[p] = pick(cur[p], 1 - e1);   // second colour of plane s
+            if (!FIRST)
+                stage(s + 2, m, xr[PU][p] + (PROL ? xq[PU][p] : (T)0),
+                      er[PU][p] + (PROL ? eq[PU][p] : (T)0));
+            kop[p] = pick(op[PU][p], 1 - e1);
             zr2[p] = zr1[p];
             zr1[p] = zrs[p];
-            cur[p] = nxt[p];
-            xs0[p] = xs1[p];
-            xs1[p] = xs2[p];
-            xs2[p] = xs3[p];
-            ex2[p] = ex3[p];
         }
         lds_barrier();
+    };
+    for (int s = s0;; s += 2) {
+        step(Ic<0>(), s);
+        if (s + 1 > k1) break;
+        step(Ic<1>(), s + 1);
+        if (s + 2 > k1) break;
     }
 }
 
@@ -1619,7 +1643,7 @@ static SweepShape sweep_shape(int nx, int ny, int nz, int max_rows, int planes) 
     sh.tpr = np;
     int rb = 1024 / sh.tpr;
     if (max_rows > 0 && rb > max_rows) rb = max_rows;
-    const int lds_rows = (int)((65536 / ((long long)np * sizeof(T)) - 4) / 4);
+    const int lds_rows = (int)((65536 / ((long long)np * sizeof(T)) - 8) / 6);
     if (rb > lds_rows) rb = lds_rows;
     if (rb > ny + 2) rb = ny + 2;
     sh.rb = rb;                      // < 3: the caller falls back to colour passes
@@ -1632,7 +1656,7 @@ static SweepShape sweep_shape(int nx, int ny, int nz, int max_rows, int planes) 
     sh.ytiles = ytiles;
     sh.grid = dim3((unsigned)(ytiles * ((nz + zc - 1) / zc)), 1, 1);
     sh.block = dim3((unsigned)sh.tpr, (unsigned)(rb > 0 ? rb : 1), 1);
-    sh.lds = sizeof(T) * (size_t)np * (2 * (rb + 2) + 2 * rb);
+    sh.lds = sizeof(T) * (size_t)np * (4 * (rb + 2) + 2 * rb);
     return sh;
 }
 
@@ -1677,16 +1701,23 @@ struct Cycle {
 // swap).  Algorithmic bytes: the operator planes (3 couplings, plus the
 // diagonal unless dgi), b, the old iterate (unless FIRST) and the new
 // iterate, once each.
-template <class T, bool FIRST, bool DGI, bool PROL>
-static void sweep_launch1(const SweepShape &sh, const Level<T> &L, int l, const T *b, int c1,
+template <class T, bool FIRST, bool DGI, bool PROL, bool PF>
+static void sweep_launch2(const SweepShape &sh, const Level<T> &L, int l, const T *b, int c1,
                           const T *ec, hipStream_t s) {
     const LevelK<T> K = L.k();
     if (l == 0)
-        k_rb_sweep<T, FIRST, 1, true, DGI, PROL><<<sh.grid, sh.block, sh.lds, s>>>(
+        k_rb_sweep<T, FIRST, 1, true, DGI, PROL, PF><<<sh.grid, sh.block, sh.lds, s>>>(
             K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec);
     else
-        k_rb_sweep<T, FIRST, 1, false, DGI, PROL><<<sh.grid, sh.block, sh.lds, s>>>(
+        k_rb_sweep<T, FIRST, 1, false, DGI, PROL, PF><<<sh.grid, sh.block, sh.lds, s>>>(
             K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec);
+}
+// Loads run ahead in FP32 only: the FP64 kernel with both operator sets
+// exceeds 128 VGPRs and spills (226 us against 198 us at 256^3).
+template <class T, bool FIRST, bool DGI, bool PROL>
+static void sweep_launch1(const SweepShape &sh, const Level<T> &L, int l, const T *b, int c1,
+                          const T *ec, hipStream_t s) {
+    sweep_launch2<T, FIRST, DGI, PROL, sizeof(T) == 4>(sh, L, l, b, c1, ec, s);
 }
 template <class T, bool DGI>
 static void sweep_launch(const SweepShape &sh, const Level<T> &L, int l, const T *b, int c1, bool first,
