@@ -63,12 +63,15 @@ def solve_bytes(n3, iters, S, nu_pre=2, nu_post=2, tail_cells=4096):
         nb = dims[-1][0] * dims[-1][1] * dims[-1][2]
         v += nb * nb * S + 2 * nb * S                       # dense bottom solve
     mixed = S == 4
-    sz = S
-    outer = n0 * ((8 + sz) + (48 + sz) + (48 + (4 if mixed else 0)))  # dot, d & A d, update
+    # the PCG step: (r, z) comes with the last sweep; d & A d reads z, d_old,
+    # the FP64 couplings and x (the previous x += alpha d), writes d (type
+    # S), q and x; the update reads r, q and writes r (+ its FP32 copy)
+    outer = n0 * ((48 + 3 * S) + (24 + (4 if mixed else 0)))
     setup = n0 * (72 + 56 + (20 if mixed else 0))           # A, p in; op, b, x, r (+ FP32) out
     for (a, b, c) in dims[1:]:
         setup += a * b * c * S * (8 * 4 + 4)                # coarsening
-    final = n0 * (40 + 8)                                   # true residual + write-back of p
+    final = n0 * (16 + S)                                   # the last x += alpha d
+    final += n0 * (40 + 8)                                  # true residual + write-back of p
     return setup + iters * (v + outer) + final
 
 
@@ -306,8 +309,8 @@ def main():
     f64 = res["fp64"]
     n_cells = grid.cells
     roof = {
-        "kernel": "k_rb_sweep<float, false, 1, true, true, true, false> (finest-level fused red-black "
-                  "Gauss-Seidel sweep, FP32)",
+        "kernel": "k_rb_sweep<float, false, 1, true, true, false, true, false> (finest-level fused "
+                  "red-black Gauss-Seidel sweep, FP32)",
         "bound": "hbm",
         "achieved": round(m["smoother_GBps"], 1),
         "peak": HBM_PEAK_GBPS,

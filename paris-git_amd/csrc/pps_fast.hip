@@ -144,7 +144,10 @@ k_rb(LevelK<T> L, const B *__restrict__ b, T *__restrict__ x, int colour) {
 // every xin value before it is staged (xin + P ec, P the piecewise-constant
 // prolongation: both cells of a pair share the parent m), so the sweep
 // that follows a coarse-grid correction needs no separate prolongation
-// pass.  PX pairs per thread.
+// pass.  PX pairs per thread.  DOT: the block's sums over its output
+// cells of b x, x and b (x the new iterate) go to part[block],
+// part[kRedBlocks + block] and part[2 kRedBlocks + block] (the block must
+// be whole wavefronts).
 // LDS: X[4][rb+2][np] (second-colour xin, plane k in slot k & 3),
 // R[2][rb][np] (first colour new).
 // ---------------------------------------------------------------------
@@ -154,6 +157,25 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
     __builtin_amdgcn_s_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Writes the block sum of v to part[block linear id].  Any block of at most
+// 1024 threads whose x extent is a multiple of 64 (each wavefront is 64
+// consecutive linear thread ids); wavefront sums are added in a fixed order.
+__device__ __forceinline__ void block_partial(double v, double *part) {
+    __shared__ double sm[16];
+    const int t = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    const int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const int nw = (blockDim.x * blockDim.y * blockDim.z) >> 6;
+    v = wsum(v);
+    if ((t & 63) == 0) sm[t >> 6] = v;
+    __syncthreads();
+    if (t == 0) {
+        double a = 0.0;
+        for (int w = 0; w < nw; ++w) a += sm[w];
+        part[b] = a;
+    }
+    __syncthreads();
 }
 
 template <class T> struct PairOp {   // operator of one pair at one plane
@@ -180,10 +202,10 @@ __device__ __forceinline__ CellOp<T> pick(const PairOp<T> &P, int e) {
 // FINE does not change the code; it gives the finest level's launches
 // their own symbol so profilers can separate them from coarse levels.
 // DGI: the diagonal is the coupling sum (not read).
-template <class T, bool FIRST, int PX, bool FINE, bool DGI, bool PROL, bool PF>
+template <class T, bool FIRST, int PX, bool FINE, bool DGI, bool PROL, bool PF, bool DOT>
 __global__ void __launch_bounds__(1024)
 k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *__restrict__ xout,
-           int c1, int zc, int oy, int ytiles, const T *__restrict__ ec) {
+           int c1, int zc, int oy, int ytiles, const T *__restrict__ ec, double *__restrict__ part) {
     extern __shared__ unsigned char smem_raw[];
     T *sm = reinterpret_cast<T *>(smem_raw);
     const int nx = L.nx, ny = L.ny, nz = L.nz;
@@ -248,6 +270,8 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
     CellOp<T> kop[PX];
     T zr1[PX], zr2[PX], r_m2[PX], r_m1[PX], r_0[PX];
     T xr[2][PX], er[2][PX], xq[2][PX], eq[2][PX];
+    T bf1[PX];                       // DOT: first colour's b at plane s-1
+    double sbx = 0.0, sx = 0.0, sb = 0.0;
     const int s0 = k0 - 1;
     const int klast = k1 + 1;   // last xin plane a step reads
 
@@ -278,6 +302,7 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
         zr1[p] = L.cz[cz2 + (1 - e1p)];
         zr2[p] = (T)0;
         r_m2[p] = r_m1[p] = r_0[p] = (T)0;
+        bf1[p] = (T)0;
     }
     lds_barrier();
 
@@ -305,13 +330,17 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
             }
         }
         // R: first colour at plane s
-        T zrs[PX];
+        T zrs[PX], bft[PX];
 #pragma unroll
         for (int p = 0; p < PX; ++p) {
             const int m = threadIdx.x + p * tpr;
             const CellOp<T> o = pick(op[PU][p], e1);
             const T zm = kop[p].cz;
             zrs[p] = o.cz;
+            if (DOT) {
+                bft[p] = bf1[p];
+                bf1[p] = o.b;
+            }
             T v = o.b;
             if (!FIRST) {
                 const int mm = m == 0 ? np - 1 : m - 1, mp = m == np - 1 ? 0 : m + 1;
@@ -348,6 +377,11 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
                 const long long c0 = col(j, wrapz(t), m);
                 xout[c0] = e2 ? r_m1[p] : v;
                 xout[c0 + 1] = e2 ? v : r_m1[p];
+                if (DOT) {
+                    sbx += (double)o.b * (double)v + (double)bft[p] * (double)r_m1[p];
+                    sx += (double)v + (double)r_m1[p];
+                    sb += (double)o.b + (double)bft[p];
+                }
             }
         }
         // S: stage the xin of plane s+2 (requested at step s-1); keep the
@@ -369,6 +403,11 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
         if (s + 1 > k1) break;
         step(Ic<1>(), s + 1);
         if (s + 2 > k1) break;
+    }
+    if (DOT) {
+        block_partial(sbx, part);
+        block_partial(sx, part + kRedBlocks);
+        block_partial(sb, part + 2 * kRedBlocks);
     }
 }
 
@@ -966,25 +1005,6 @@ k_tail(TailK<T> K) {
 // ---------------------------------------------------------------------
 enum { S_RZ = 0, S_ZMEAN, S_BETA, S_DQ, S_ALPHA, S_RR, S_BB, S_FIRST, S_N };
 
-// Writes the block sum of v to part[block linear id].  Any block of at most
-// 1024 threads whose x extent is a multiple of 64 (each wavefront is 64
-// consecutive linear thread ids); wavefront sums are added in a fixed order.
-__device__ __forceinline__ void block_partial(double v, double *part) {
-    __shared__ double sm[16];
-    const int t = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
-    const int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-    const int nw = (blockDim.x * blockDim.y * blockDim.z) >> 6;
-    v = wsum(v);
-    if ((t & 63) == 0) sm[t >> 6] = v;
-    __syncthreads();
-    if (t == 0) {
-        double a = 0.0;
-        for (int w = 0; w < nw; ++w) a += sm[w];
-        part[b] = a;
-    }
-    __syncthreads();
-}
-
 // 2.5-D launch shape for z-marching kernels: (64, 4) threads over (x, y),
 // each thread walks `zc` consecutive planes; at most kRedBlocks blocks so
 // that one partial slot per block suffices.
@@ -1100,19 +1120,23 @@ k_dot_rz(long long n, const double *__restrict__ r, const TZ *__restrict__ z, do
 
 
 
-// d_new = (z - zmean) + beta d_old ; q = M d_new ; partial sum d.q, on
-// LDS tiles: d_new of every plane (own cells plus a one-cell halo
-// ring, coordinates wrapped) is written to one of three LDS planes; the
-// in-plane neighbours of q = M d_new are read there, the vertical ones
-// from registers.  The raw z / d_old of plane s+2 and the operator of
-// plane s are requested at the start of step s, so the loads overlap the
-// arithmetic of the step; one barrier per plane (three buffers: the plane
-// written at step s+1 was last read at step s-2).
+// d_new = (z - zmean) + beta d_old, rounded to TZ (the direction is
+// stored in the precision of z) ; q = M d_new from the rounded values, so
+// that x += alpha d and r -= alpha q change r by exactly -M (alpha d) in
+// FP64 ; partial sum d.q.  The previous iteration's x += alpha_prev d_old
+// (alpha_prev = S[S_ALPHA], 0 for none) is applied here, where d_old is
+// read anyway.  On LDS tiles: d_new of every plane (own cells plus a
+// one-cell halo ring, coordinates wrapped) is written to one of three LDS
+// planes; the in-plane neighbours of q = M d_new are read there, the
+// vertical ones from registers.  The raw z / d_old of plane s+2 and the
+// operator of plane s are requested at the start of step s, so the loads
+// overlap the arithmetic of the step; one barrier per plane (three
+// buffers: the plane written at step s+1 was last read at step s-2).
 template <class TZ, bool DGI>
 __global__ void __launch_bounds__(1024)
-k_dq_tiled(OpK L, TGrid g, const TZ *__restrict__ z, const double *__restrict__ dold,
-           double *__restrict__ dnew, double *__restrict__ q, const double *__restrict__ S,
-           double *__restrict__ part) {
+k_dq_tiled(OpK L, TGrid g, const TZ *__restrict__ z, const TZ *__restrict__ dold,
+           TZ *__restrict__ dnew, double *__restrict__ q, double *__restrict__ x,
+           const double *__restrict__ S, double *__restrict__ part) {
     __shared__ double D[3][18][66];
     const int nx = L.nx, ny = L.ny, nz = L.nz;
     const int bid = blockIdx.x;
@@ -1121,8 +1145,9 @@ k_dq_tiled(OpK L, TGrid g, const TZ *__restrict__ z, const double *__restrict__ 
     const int i = tx * 64 + li, j = ty * 16 + lj;
     const int k0 = ch * g.zc, k1 = min(k0 + g.zc, nz);
     const bool in = i < nx && j < ny;
-    const double beta = S[S_BETA], zm = S[S_ZMEAN];
-    const bool use_old = beta != 0.0;
+    const double beta = S[S_BETA], zm = S[S_ZMEAN], ax = S[S_ALPHA];
+    const bool use_old = beta != 0.0, upd = ax != 0.0;
+    const bool rd_old = use_old || upd;
     const long long sy = nx, sz = (long long)nx * ny;
     auto wrapn = [](int a, int n) { a %= n; return a < 0 ? a + n : a; };
     auto wrapk = [&](int k) { return wrapn(k, nz); };
@@ -1137,29 +1162,31 @@ k_dq_tiled(OpK L, TGrid g, const TZ *__restrict__ z, const double *__restrict__ 
     else { hx = 64; hy = t - 148; }
     const bool halo = t < 164;
     const long long hcol = wrapn(tx * 64 + hx, nx) + sy * wrapn(ty * 16 + hy, ny);
-    auto dval = [&](TZ zr, double dr) {
+    auto dval = [&](TZ zr, TZ dr) -> double {
         const double v = (double)zr - zm;
-        return use_old ? v + beta * dr : v;
+        return (double)(TZ)(use_old ? v + beta * (double)dr : v);
     };
     // in-plane operator offsets of the own cell
     const long long dxm = in ? (long long)(wrap_m(i, nx) - i) : 0;
     const long long dym = in ? (long long)(wrap_m(j, ny) - j) * sy : 0;
 
-    // prologue: d_new at k0-1 (register), k0 (register + LDS); raw of k0+1
+    // prologue: d_new at k0-1 (register), k0 (register + LDS); raw of k0+1;
+    // dx0: d_old of the own cell at the plane being output
     double d_m = 0.0, d_0 = 0.0;
     TZ zo1 = 0, zh1 = 0;
-    double do1 = 0.0, dh1 = 0.0;
+    TZ do1 = 0, dh1 = 0, dx0 = 0;
     {
         const long long pm = sz * wrapk(k0 - 1), p0 = sz * wrapk(k0), p1 = sz * wrapk(k0 + 1);
-        d_m = dval(z[own + pm], use_old ? dold[own + pm] : 0.0);
-        d_0 = dval(z[own + p0], use_old ? dold[own + p0] : 0.0);
+        d_m = dval(z[own + pm], rd_old ? dold[own + pm] : (TZ)0);
+        dx0 = rd_old ? dold[own + p0] : (TZ)0;
+        d_0 = dval(z[own + p0], dx0);
         D[k0 % 3][lj + 1][li + 1] = d_0;
-        if (halo) D[k0 % 3][hy + 1][hx + 1] = dval(z[hcol + p0], use_old ? dold[hcol + p0] : 0.0);
+        if (halo) D[k0 % 3][hy + 1][hx + 1] = dval(z[hcol + p0], rd_old ? dold[hcol + p0] : (TZ)0);
         zo1 = z[own + p1];
-        if (use_old) do1 = dold[own + p1];
+        if (rd_old) do1 = dold[own + p1];
         if (halo) {
             zh1 = z[hcol + p1];
-            if (use_old) dh1 = dold[hcol + p1];
+            if (rd_old) dh1 = dold[hcol + p1];
         }
     }
     double czm = in ? L.cz[own + sz * wrapk(k0 - 1)] : 0.0;
@@ -1168,14 +1195,14 @@ k_dq_tiled(OpK L, TGrid g, const TZ *__restrict__ z, const double *__restrict__ 
     for (int s = k0; s < k1; ++s) {
         // requests: raw z / d_old of plane s+2, operator of plane s
         TZ zo2 = 0, zh2 = 0;
-        double do2 = 0.0, dh2 = 0.0;
+        TZ do2 = 0, dh2 = 0;
         if (s + 1 < k1) {
             const long long p2 = sz * wrapk(s + 2);
             zo2 = z[own + p2];
-            if (use_old) do2 = dold[own + p2];
+            if (rd_old) do2 = dold[own + p2];
             if (halo) {
                 zh2 = z[hcol + p2];
-                if (use_old) dh2 = dold[hcol + p2];
+                if (rd_old) dh2 = dold[hcol + p2];
             }
         }
         const long long c = own + sz * s;
@@ -1196,11 +1223,13 @@ k_dq_tiled(OpK L, TGrid g, const TZ *__restrict__ z, const double *__restrict__ 
             const double syv = cys * D[rb][lj][li + 1] + cyn * D[rb][lj + 2][li + 1];
             const double szv = czm * d_m + czp * d_p;
             const double qv = dgc * d_0 - ((sx + syv) + szv);
-            dnew[c] = d_0;
+            dnew[c] = (TZ)d_0;
             q[c] = qv;
             dq += d_0 * qv;
+            if (upd) x[c] += ax * (double)dx0;
         }
         d_m = d_0; d_0 = d_p; czm = czp;
+        dx0 = do1;
         zo1 = zo2; do1 = do2; zh1 = zh2; dh1 = dh2;
         __syncthreads();
     }
@@ -1208,22 +1237,31 @@ k_dq_tiled(OpK L, TGrid g, const TZ *__restrict__ z, const double *__restrict__ 
 }
 
 
-// x += alpha d ; r -= alpha q ; partial sum r^2 ; optional FP32 copy of r
+// r -= alpha q ; partial sum r^2 ; optional FP32 copy of r (x += alpha d
+// follows in the next k_dq_tiled, or in k_axpy_d after the last iteration)
 template <class TB>
 __global__ void __launch_bounds__(256)
-k_update(long long n, double *__restrict__ x, double *__restrict__ r, const double *__restrict__ d,
-         const double *__restrict__ q, TB *__restrict__ bt, const double *__restrict__ S,
-         double *__restrict__ part) {
+k_update(long long n, double *__restrict__ r, const double *__restrict__ q, TB *__restrict__ bt,
+         const double *__restrict__ S, double *__restrict__ part) {
     const double alpha = S[S_ALPHA];
     double rr = 0.0;
     for (long long c = blockIdx.x * 256LL + threadIdx.x; c < n; c += (long long)gridDim.x * 256) {
-        x[c] += alpha * d[c];
         const double v = r[c] - alpha * q[c];
         r[c] = v;
         if (bt) bt[c] = (TB)v;
         rr += v * v;
     }
     block_partial(rr, part);
+}
+
+// x += alpha d with alpha = S[S_ALPHA] (the last iteration's deferred
+// solution update)
+template <class TD>
+__global__ void __launch_bounds__(256)
+k_axpy_d(long long n, double *__restrict__ x, const TD *__restrict__ d, const double *__restrict__ S) {
+    const double alpha = S[S_ALPHA];
+    for (long long c = blockIdx.x * 256LL + threadIdx.x; c < n; c += (long long)gridDim.x * 256)
+        x[c] += alpha * (double)d[c];
 }
 
 // refinement step: x += z - zmean ; partial sums of z
@@ -1703,28 +1741,35 @@ struct Cycle {
 // iterate, once each.
 template <class T, bool FIRST, bool DGI, bool PROL, bool PF>
 static void sweep_launch2(const SweepShape &sh, const Level<T> &L, int l, const T *b, int c1,
-                          const T *ec, hipStream_t s) {
+                          const T *ec, double *dot, hipStream_t s) {
     const LevelK<T> K = L.k();
+    if constexpr (!FIRST) {
+        if (l == 0 && dot) {
+            k_rb_sweep<T, FIRST, 1, true, DGI, PROL, PF, true><<<sh.grid, sh.block, sh.lds, s>>>(
+                K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec, dot);
+            return;
+        }
+    }
     if (l == 0)
-        k_rb_sweep<T, FIRST, 1, true, DGI, PROL, PF><<<sh.grid, sh.block, sh.lds, s>>>(
-            K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec);
+        k_rb_sweep<T, FIRST, 1, true, DGI, PROL, PF, false><<<sh.grid, sh.block, sh.lds, s>>>(
+            K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec, nullptr);
     else
-        k_rb_sweep<T, FIRST, 1, false, DGI, PROL, PF><<<sh.grid, sh.block, sh.lds, s>>>(
-            K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec);
+        k_rb_sweep<T, FIRST, 1, false, DGI, PROL, PF, false><<<sh.grid, sh.block, sh.lds, s>>>(
+            K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec, nullptr);
 }
 // Loads run ahead in FP32 only: the FP64 kernel with both operator sets
 // exceeds 128 VGPRs and spills (226 us against 198 us at 256^3).
 template <class T, bool FIRST, bool DGI, bool PROL>
 static void sweep_launch1(const SweepShape &sh, const Level<T> &L, int l, const T *b, int c1,
-                          const T *ec, hipStream_t s) {
-    sweep_launch2<T, FIRST, DGI, PROL, sizeof(T) == 4>(sh, L, l, b, c1, ec, s);
+                          const T *ec, double *dot, hipStream_t s) {
+    sweep_launch2<T, FIRST, DGI, PROL, sizeof(T) == 4>(sh, L, l, b, c1, ec, dot, s);
 }
 template <class T, bool DGI>
 static void sweep_launch(const SweepShape &sh, const Level<T> &L, int l, const T *b, int c1, bool first,
-                         const T *ec, hipStream_t s) {
-    if (first) sweep_launch1<T, true, DGI, false>(sh, L, l, b, c1, nullptr, s);
-    else if (ec) sweep_launch1<T, false, DGI, true>(sh, L, l, b, c1, ec, s);
-    else sweep_launch1<T, false, DGI, false>(sh, L, l, b, c1, nullptr, s);
+                         const T *ec, double *dot, hipStream_t s) {
+    if (first) sweep_launch1<T, true, DGI, false>(sh, L, l, b, c1, nullptr, nullptr, s);
+    else if (ec) sweep_launch1<T, false, DGI, true>(sh, L, l, b, c1, ec, dot, s);
+    else sweep_launch1<T, false, DGI, false>(sh, L, l, b, c1, nullptr, dot, s);
 }
 
 // Whether level l's sweeps run as one fused kernel (else two colour passes).
@@ -1734,10 +1779,26 @@ static bool sweep_fused(const Level<T> &L, const Cycle &cy) {
     return cy.fused && !(sh.rb < 3 || sh.lds > 65536 || (L.nx & 1) || sh.tpr * sh.rb > 1024);
 }
 
-// ec: the coarse correction to add to the iterate first (fused sweeps only).
+// Whether the fused sweep of the finest level can also form the block sums
+// of b x, x and b (whole wavefronts per block, one partial slot per block).
+template <class T>
+static bool sweep_dot_ok(const Level<T> &L, const Cycle &cy) {
+    if (!sweep_fused(L, cy)) return false;
+    const SweepShape sh = sweep_shape<T>(L.nx, L.ny, L.nz, cy.rows, cy.planes);
+    return (sh.tpr * sh.rb) % 64 == 0 && (long long)sh.grid.x <= kRedBlocks;
+}
+
+// Blocks of the fused sweep of level l (its partial sums under DOT).
+template <class T>
+static int sweep_blocks(const Level<T> &L, const Cycle &cy) {
+    return (int)sweep_shape<T>(L.nx, L.ny, L.nz, cy.rows, cy.planes).grid.x;
+}
+
+// ec: the coarse correction to add to the iterate first (fused sweeps
+// only); dot: partial sums of b x, x, b of the result (sweep_dot_ok only).
 template <class T>
 static void rb_sweep(Level<T> &L, int l, const T *b, int c1, bool first, const Cycle &cy, Prof *prof,
-                     hipStream_t s, const T *ec = nullptr) {
+                     hipStream_t s, const T *ec = nullptr, double *dot = nullptr) {
     const bool dgi = cy.dgi;
     const SweepShape sh = sweep_shape<T>(L.nx, L.ny, L.nz, cy.rows, cy.planes);
     if (!sweep_fused(L, cy)) {
@@ -1749,10 +1810,10 @@ static void rb_sweep(Level<T> &L, int l, const T *b, int c1, bool first, const C
     // the profiled kernel: finest level, neither the first sweep of a cycle
     // nor the one that adds the coarse correction
     const double bytes = (double)L.n * sizeof(T) * (dgi ? 6.0 : 7.0);
-    hipEvent_t *e = (l == 0 && !first && !ec && prof) ? prof->begin(bytes) : nullptr;
+    hipEvent_t *e = (l == 0 && !first && !ec && !dot && prof) ? prof->begin(bytes) : nullptr;
     if (e) (void)hipEventRecord(e[0], s);
-    if (dgi) sweep_launch<T, true>(sh, L, l, b, c1, first, ec, s);
-    else sweep_launch<T, false>(sh, L, l, b, c1, first, ec, s);
+    if (dgi) sweep_launch<T, true>(sh, L, l, b, c1, first, ec, dot, s);
+    else sweep_launch<T, false>(sh, L, l, b, c1, first, ec, dot, s);
     if (e) (void)hipEventRecord(e[1], s);
     T *t = L.x;
     L.x = L.xt;
@@ -1760,12 +1821,17 @@ static void rb_sweep(Level<T> &L, int l, const T *b, int c1, bool first, const C
 }
 
 // one V-cycle: x_0 = V(b_0).  fineb: the fine right-hand side (the FP64
-// residual in FP64 mode, its FP32 copy in mixed mode).
+// residual in FP64 mode, its FP32 copy in mixed mode).  dot: when not null
+// and the last fine sweep is a fused one that can, that sweep writes the
+// block sums of b_0 x_0, x_0 and b_0 there (as k_dot_rz lays them out);
+// returns their block count, 0 if it did not.
 template <class T>
-static void vcycle(Hier<T> &H, const T *fineb, const Cycle &cy, Prof *prof, hipStream_t s) {
+static int vcycle(Hier<T> &H, const T *fineb, const Cycle &cy, Prof *prof, hipStream_t s,
+                  double *dot = nullptr) {
     const int nl = (int)H.lv.size();
     const int nu_pre = cy.nu_pre, nu_post = cy.nu_post;
     const bool dgi = cy.dgi, fused = cy.fused;
+    int ndot = 0;
     // levels lt .. nl-1 run in one workgroup (k_tail): the first level
     // below the finest with at most cy.tail_cells cells
     int lt = nl;
@@ -1826,15 +1892,19 @@ static void vcycle(Hier<T> &H, const T *fineb, const Cycle &cy, Prof *prof, hipS
         // the first fused post-sweep adds the correction as it loads x
         const bool fold = nu_post > 0 && sweep_fused(L, cy);
         if (!fold) k_prolong_add<T><<<cgrid3(C.nx, C.ny, C.nz), kB3, 0, s>>>(L.nx, L.ny, L.nz, L.x, C.x);
+        const bool with_dot = l == 0 && dot && fused && sweep_dot_ok(L, cy);
         for (int q = 0; q < nu_post; ++q) {
             if (fused) {
-                rb_sweep(L, l, b, 1, false, cy, prof, s, (fold && q == 0) ? (const T *)C.x : nullptr);
+                rb_sweep(L, l, b, 1, false, cy, prof, s, (fold && q == 0) ? (const T *)C.x : nullptr,
+                         (with_dot && q == nu_post - 1) ? dot : nullptr);
             } else {
                 rb_pass(L, l, b, 1, false, dgi, prof, s);
                 rb_pass(L, l, b, 0, false, dgi, prof, s);
             }
         }
+        if (with_dot && nu_post > 0) ndot = sweep_blocks(L, cy);
     }
+    return ndot;
 }
 
 static PackK packk(const Dims &d) {
@@ -1979,6 +2049,7 @@ static int outer_iterate(pps_handle *h, Hier<T> &H, bool mixed, double tol_abs, 
     }
     f->have_r = 0;
     k_fill<<<1, 64, 0, s>>>(S + S_FIRST, 1, 1.0);
+    k_fill<<<1, 64, 0, s>>>(S + S_ALPHA, 1, 0.0);   // no pending x update
     PPS_HIP(hipGetLastError());
     PPS_HIP(hipMemcpyAsync(h->pinned, S + S_RR, sizeof(double), hipMemcpyDeviceToHost, s));
     PPS_HIP(hipStreamSynchronize(s));
@@ -1988,19 +2059,27 @@ static int outer_iterate(pps_handle *h, Hier<T> &H, bool mixed, double tol_abs, 
     if (t2 == 0.0) { *it = 1; return PPS_OK; }   // exact solution: nothing to do
     for (k = 1; k <= maxit; ++k) {
         // z = V(r)
-        vcycle<T>(H, mixed ? (const T *)(const void *)b32 : (const T *)(const void *)V.r, cy, &h->prof, s);
+        const T *fb = mixed ? (const T *)(const void *)b32 : (const T *)(const void *)V.r;
+        const int ndot = vcycle<T>(H, fb, cy, &h->prof, s, cg ? part : nullptr);
         const T *z = H.lv[0].x;   // the sweeps swap buffers: read after the cycle
         if (cg) {
-            k_dot_rz<T><<<nb, 256, 0, s>>>(n, V.r, z, part);
-            k_scal_beta<<<1, 1024, 0, s>>>(part, nb, n, f->singular, S);
+            // (r, z), sum z, sum r: from the last sweep (with r as the cycle
+            // saw it: its FP32 copy in mixed mode), else a pass of their own
+            int nbz = ndot;
+            if (!nbz) {
+                k_dot_rz<T><<<nb, 256, 0, s>>>(n, V.r, z, part);
+                nbz = nb;
+            }
+            k_scal_beta<<<1, 1024, 0, s>>>(part, nbz, n, f->singular, S);
+            T *dold = (T *)(void *)V.d[cur], *dnew = (T *)(void *)V.d[cur ^ 1];
             if (f->exact)
-                k_dq_tiled<T, true><<<tg.blocks, dim3(64, 16, 1), 0, s>>>(O, tg, z, V.d[cur], V.d[cur ^ 1], V.q, S,
+                k_dq_tiled<T, true><<<tg.blocks, dim3(64, 16, 1), 0, s>>>(O, tg, z, dold, dnew, V.q, V.x, S,
                                                                           part);
             else
-                k_dq_tiled<T, false><<<tg.blocks, dim3(64, 16, 1), 0, s>>>(O, tg, z, V.d[cur], V.d[cur ^ 1], V.q, S,
+                k_dq_tiled<T, false><<<tg.blocks, dim3(64, 16, 1), 0, s>>>(O, tg, z, dold, dnew, V.q, V.x, S,
                                                                            part);
             k_scal_alpha<<<1, 1024, 0, s>>>(part, tg.blocks, S);
-            k_update<float><<<nb, 256, 0, s>>>(n, V.x, V.r, V.d[cur ^ 1], V.q, b32, S, part);
+            k_update<float><<<nb, 256, 0, s>>>(n, V.r, V.q, b32, S, part);
             k_scal_rr<<<1, 1024, 0, s>>>(part, nb, 0, S);
             cur ^= 1;
         } else {
@@ -2019,6 +2098,9 @@ static int outer_iterate(pps_handle *h, Hier<T> &H, bool mixed, double tol_abs, 
         if (!(t2 == t2) || t2 > 1e300) { *it = k; return PPS_ERR_DIVERGED; }
         if (t2 < tol_abs) break;
     }
+    // the last iteration's x += alpha d
+    if (cg) k_axpy_d<T><<<nb, 256, 0, s>>>(n, V.x, (const T *)(const void *)V.d[cur], S);
+    PPS_HIP(hipGetLastError());
     *it = k;
     return PPS_OK;
 }
