@@ -1547,6 +1547,7 @@ struct FastState {
     int mixed_built = 0, fp64_built = 0;
     int have_r = 0;  // the setup pass already wrote r (and its norms) for x
     size_t lds_max = 65536;  // LDS bytes per workgroup on this device
+    int cus = 256;           // compute units on this device
 };
 
 void fast_destroy(FastState *f) {
@@ -1666,15 +1667,20 @@ static void rb_pass(const Level<T> &L, int l, const T *b, int colour, bool first
 
 // Launch shape of the fused sweep on a level: pairs per thread, thread
 // rows per block, planes per block.  Blocks hold at most 1024 threads and
-// 64 KB of LDS; about 1024 blocks fill the chip.  (Two pairs per thread
-// halve the ring-row overhead but ran at half the speed at 256^3.)
+// 64 KB of LDS; one runs per CU.  Planes per block: a level of 2^23 cells
+// or more is bandwidth-bound and takes about 1024 blocks (256^3 FP32: 11
+// planes, within 3% of the best of 2..32); a smaller level is bound by the
+// latency of its plane steps, so the count minimises waves of blocks x
+// steps per block (128^3: 6 planes, 220 blocks, 18 us against 30 us for
+// 640 blocks of 2 planes).  (Two pairs per thread halve the ring-row
+// overhead but ran at half the speed at 256^3.)
 struct SweepShape {
     int px, tpr, rb, oy, zc, ytiles;
     dim3 grid, block;
     size_t lds;
 };
 template <class T>
-static SweepShape sweep_shape(int nx, int ny, int nz, int max_rows, int planes) {
+static SweepShape sweep_shape(int nx, int ny, int nz, int max_rows, int planes, int cus) {
     SweepShape sh;
     const int np = nx / 2;
     sh.px = 1;
@@ -1687,7 +1693,17 @@ static SweepShape sweep_shape(int nx, int ny, int nz, int max_rows, int planes) 
     sh.rb = rb;                      // < 3: the caller falls back to colour passes
     sh.oy = rb - 2 >= ny ? ny : (rb > 2 ? rb - 2 : 1);
     const int ytiles = (ny + sh.oy - 1) / sh.oy;
-    int zc = planes > 0 ? planes : (int)(((long long)nz * ytiles + 1023) / 1024);
+    int zc = planes;
+    if (zc <= 0 && (long long)nx * ny * nz >= (1LL << 23)) {
+        zc = (int)(((long long)nz * ytiles + 1023) / 1024);
+    } else if (zc <= 0) {
+        long long best = -1;
+        for (int c = 2; c <= (nz > 2 ? nz : 2); ++c) {
+            const long long blocks = (long long)ytiles * ((nz + c - 1) / c);
+            const long long cost = ((blocks + cus - 1) / cus) * (c + 2);
+            if (best < 0 || cost < best) { best = cost; zc = c; }
+        }
+    }
     if (zc < 2) zc = 2;
     if (zc > nz) zc = nz;
     sh.zc = zc;
@@ -1733,6 +1749,7 @@ struct Cycle {
     int rows = 0, planes = 0;
     long long tail_cells = 4096;
     size_t lds_max = 65536;   // LDS bytes one workgroup may use
+    int cus = 256;            // compute units (fused-sweep plane chunks)
 };
 
 // One fused red-black sweep of level l (out of place, the iterate pointers
@@ -1775,7 +1792,7 @@ static void sweep_launch(const SweepShape &sh, const Level<T> &L, int l, const T
 // Whether level l's sweeps run as one fused kernel (else two colour passes).
 template <class T>
 static bool sweep_fused(const Level<T> &L, const Cycle &cy) {
-    const SweepShape sh = sweep_shape<T>(L.nx, L.ny, L.nz, cy.rows, cy.planes);
+    const SweepShape sh = sweep_shape<T>(L.nx, L.ny, L.nz, cy.rows, cy.planes, cy.cus);
     return cy.fused && !(sh.rb < 3 || sh.lds > 65536 || (L.nx & 1) || sh.tpr * sh.rb > 1024);
 }
 
@@ -1784,14 +1801,14 @@ static bool sweep_fused(const Level<T> &L, const Cycle &cy) {
 template <class T>
 static bool sweep_dot_ok(const Level<T> &L, const Cycle &cy) {
     if (!sweep_fused(L, cy)) return false;
-    const SweepShape sh = sweep_shape<T>(L.nx, L.ny, L.nz, cy.rows, cy.planes);
+    const SweepShape sh = sweep_shape<T>(L.nx, L.ny, L.nz, cy.rows, cy.planes, cy.cus);
     return (sh.tpr * sh.rb) % 64 == 0 && (long long)sh.grid.x <= kRedBlocks;
 }
 
 // Blocks of the fused sweep of level l (its partial sums under DOT).
 template <class T>
 static int sweep_blocks(const Level<T> &L, const Cycle &cy) {
-    return (int)sweep_shape<T>(L.nx, L.ny, L.nz, cy.rows, cy.planes).grid.x;
+    return (int)sweep_shape<T>(L.nx, L.ny, L.nz, cy.rows, cy.planes, cy.cus).grid.x;
 }
 
 // ec: the coarse correction to add to the iterate first (fused sweeps
@@ -1800,7 +1817,7 @@ template <class T>
 static void rb_sweep(Level<T> &L, int l, const T *b, int c1, bool first, const Cycle &cy, Prof *prof,
                      hipStream_t s, const T *ec = nullptr, double *dot = nullptr) {
     const bool dgi = cy.dgi;
-    const SweepShape sh = sweep_shape<T>(L.nx, L.ny, L.nz, cy.rows, cy.planes);
+    const SweepShape sh = sweep_shape<T>(L.nx, L.ny, L.nz, cy.rows, cy.planes, cy.cus);
     if (!sweep_fused(L, cy)) {
         // rows too long for the LDS planes: two colour passes instead
         rb_pass(L, l, b, c1, first, dgi, prof, s);
@@ -1931,6 +1948,9 @@ static int fast_ensure(pps_handle *h) {
         int v = 0;
         if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device) == hipSuccess && v > 0)
             f->lds_max = (size_t)v;
+        v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, h->device) == hipSuccess && v > 0)
+            f->cus = v;
         // the tail kernel may use all of it
         const int mx = (int)f->lds_max;
         (void)hipFuncSetAttribute((const void *)k_tail<float, true>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
@@ -2040,6 +2060,7 @@ static int outer_iterate(pps_handle *h, Hier<T> &H, bool mixed, double tol_abs, 
     cy.planes = h->opt.sweep_planes;
     cy.tail_cells = h->opt.tail_cells == 0 ? 4096 : (h->opt.tail_cells < 0 ? 0 : h->opt.tail_cells);
     cy.lds_max = f->lds_max;
+    cy.cus = f->cus;
 
     // r = b - M x (unless the setup pass just computed it)
     if (!f->have_r) {
