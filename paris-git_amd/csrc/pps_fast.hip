@@ -1667,24 +1667,29 @@ static void rb_pass(const Level<T> &L, int l, const T *b, int colour, bool first
 
 // Launch shape of the fused sweep on a level: pairs per thread, thread
 // rows per block, planes per block.  Blocks hold at most 1024 threads and
-// 64 KB of LDS; one runs per CU.  Planes per block: a level of 2^23 cells
-// or more is bandwidth-bound and takes about 1024 blocks (256^3 FP32: 11
-// planes, within 3% of the best of 2..32); a smaller level is bound by the
-// latency of its plane steps, so the count minimises waves of blocks x
-// steps per block (128^3: 6 planes, 220 blocks, 18 us against 30 us for
-// 640 blocks of 2 planes).  (Two pairs per thread halve the ring-row
-// overhead but ran at half the speed at 256^3.)
+// 64 KB of LDS; one runs per CU.
+// Pairs per thread: 2 in FP32 when the diagonals are the coupling sums
+// (dgi) and a row has a multiple of 128 pairs (whole wavefronts per row;
+// twice the rows per block, so half the ring-row overhead), without the
+// run-ahead loads, which would not fit the registers; else 1.
+// Planes per block: with one pair per thread a level of 2^23 cells or
+// more takes about 1024 blocks (256^3 FP64: 11 planes, within 1% of the
+// best of 2..32); otherwise the count minimises waves of blocks x steps
+// per block, the sweep being bound by the latency of its plane steps
+// (128^3: 6 planes, 220 blocks, 18 us against 30 us for 640 blocks of 2
+// planes; 256^3 FP32 with two pairs per thread: 20 planes, 247 blocks,
+// 109 us against 124 us for one pair per thread).
 struct SweepShape {
     int px, tpr, rb, oy, zc, ytiles;
     dim3 grid, block;
     size_t lds;
 };
 template <class T>
-static SweepShape sweep_shape(int nx, int ny, int nz, int max_rows, int planes, int cus) {
+static SweepShape sweep_shape(int nx, int ny, int nz, int max_rows, int planes, int cus, bool dgi) {
     SweepShape sh;
     const int np = nx / 2;
-    sh.px = 1;
-    sh.tpr = np;
+    sh.px = (sizeof(T) == 4 && dgi && np % 128 == 0) ? 2 : 1;
+    sh.tpr = np / sh.px;
     int rb = 1024 / sh.tpr;
     if (max_rows > 0 && rb > max_rows) rb = max_rows;
     const int lds_rows = (int)((65536 / ((long long)np * sizeof(T)) - 8) / 6);
@@ -1694,7 +1699,7 @@ static SweepShape sweep_shape(int nx, int ny, int nz, int max_rows, int planes, 
     sh.oy = rb - 2 >= ny ? ny : (rb > 2 ? rb - 2 : 1);
     const int ytiles = (ny + sh.oy - 1) / sh.oy;
     int zc = planes;
-    if (zc <= 0 && (long long)nx * ny * nz >= (1LL << 23)) {
+    if (zc <= 0 && sh.px == 1 && (long long)nx * ny * nz >= (1LL << 23)) {
         zc = (int)(((long long)nz * ytiles + 1023) / 1024);
     } else if (zc <= 0) {
         long long best = -1;
@@ -1760,6 +1765,25 @@ template <class T, bool FIRST, bool DGI, bool PROL, bool PF>
 static void sweep_launch2(const SweepShape &sh, const Level<T> &L, int l, const T *b, int c1,
                           const T *ec, double *dot, hipStream_t s) {
     const LevelK<T> K = L.k();
+    if constexpr (sizeof(T) == 4 && DGI) {
+        // two pairs per thread (no run-ahead loads)
+        if (sh.px == 2) {
+            if constexpr (!FIRST) {
+                if (l == 0 && dot) {
+                    k_rb_sweep<T, FIRST, 2, true, DGI, PROL, false, true><<<sh.grid, sh.block, sh.lds, s>>>(
+                        K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec, dot);
+                    return;
+                }
+            }
+            if (l == 0)
+                k_rb_sweep<T, FIRST, 2, true, DGI, PROL, false, false><<<sh.grid, sh.block, sh.lds, s>>>(
+                    K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec, nullptr);
+            else
+                k_rb_sweep<T, FIRST, 2, false, DGI, PROL, false, false><<<sh.grid, sh.block, sh.lds, s>>>(
+                    K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec, nullptr);
+            return;
+        }
+    }
     if constexpr (!FIRST) {
         if (l == 0 && dot) {
             k_rb_sweep<T, FIRST, 1, true, DGI, PROL, PF, true><<<sh.grid, sh.block, sh.lds, s>>>(
@@ -1792,7 +1816,7 @@ static void sweep_launch(const SweepShape &sh, const Level<T> &L, int l, const T
 // Whether level l's sweeps run as one fused kernel (else two colour passes).
 template <class T>
 static bool sweep_fused(const Level<T> &L, const Cycle &cy) {
-    const SweepShape sh = sweep_shape<T>(L.nx, L.ny, L.nz, cy.rows, cy.planes, cy.cus);
+    const SweepShape sh = sweep_shape<T>(L.nx, L.ny, L.nz, cy.rows, cy.planes, cy.cus, cy.dgi);
     return cy.fused && !(sh.rb < 3 || sh.lds > 65536 || (L.nx & 1) || sh.tpr * sh.rb > 1024);
 }
 
@@ -1801,14 +1825,14 @@ static bool sweep_fused(const Level<T> &L, const Cycle &cy) {
 template <class T>
 static bool sweep_dot_ok(const Level<T> &L, const Cycle &cy) {
     if (!sweep_fused(L, cy)) return false;
-    const SweepShape sh = sweep_shape<T>(L.nx, L.ny, L.nz, cy.rows, cy.planes, cy.cus);
+    const SweepShape sh = sweep_shape<T>(L.nx, L.ny, L.nz, cy.rows, cy.planes, cy.cus, cy.dgi);
     return (sh.tpr * sh.rb) % 64 == 0 && (long long)sh.grid.x <= kRedBlocks;
 }
 
 // Blocks of the fused sweep of level l (its partial sums under DOT).
 template <class T>
 static int sweep_blocks(const Level<T> &L, const Cycle &cy) {
-    return (int)sweep_shape<T>(L.nx, L.ny, L.nz, cy.rows, cy.planes, cy.cus).grid.x;
+    return (int)sweep_shape<T>(L.nx, L.ny, L.nz, cy.rows, cy.planes, cy.cus, cy.dgi).grid.x;
 }
 
 // ec: the coarse correction to add to the iterate first (fused sweeps
@@ -1817,7 +1841,7 @@ template <class T>
 static void rb_sweep(Level<T> &L, int l, const T *b, int c1, bool first, const Cycle &cy, Prof *prof,
                      hipStream_t s, const T *ec = nullptr, double *dot = nullptr) {
     const bool dgi = cy.dgi;
-    const SweepShape sh = sweep_shape<T>(L.nx, L.ny, L.nz, cy.rows, cy.planes, cy.cus);
+    const SweepShape sh = sweep_shape<T>(L.nx, L.ny, L.nz, cy.rows, cy.planes, cy.cus, cy.dgi);
     if (!sweep_fused(L, cy)) {
         // rows too long for the LDS planes: two colour passes instead
         rb_pass(L, l, b, c1, first, dgi, prof, s);

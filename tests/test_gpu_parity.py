@@ -211,14 +211,18 @@ def test_sphere128_forms_agree(pps, orc):
 
 
 @pytest.mark.parametrize("method,tol", [("fp64", 1e-12), ("mixed", 2e-5)])
-@pytest.mark.parametrize("case", ["walls", "perx", "allper", "aniso"])
+@pytest.mark.parametrize("case", ["walls", "perx", "allper", "aniso", "wide", "wideper"])
 def test_fused_sweep_equals_colour_passes(pps, method, tol, case):
     """One outer iteration with the fused red-black sweep kernel gives the
     iterate of the two-colour-pass kernels up to rounding (any indexing or
-    ordering mistake in the fused kernel is an O(1) difference)."""
+    ordering mistake in the fused kernel is an O(1) difference).  The wide
+    cases (rows of 128 pairs) run the FP32 sweep with two pairs per
+    thread."""
     _say("fused vs colour passes", method, case)
     n, ny, nz, bc = {"walls": (64, 64, 64, (0,) * 6), "perx": (64, 64, 64, (1, 0, 0, 1, 0, 0)),
-                     "allper": (32, 32, 32, (1,) * 6), "aniso": (128, 32, 16, (0, 1, 0, 0, 1, 0))}[case]
+                     "allper": (32, 32, 32, (1,) * 6), "aniso": (128, 32, 16, (0, 1, 0, 0, 1, 0)),
+                     "wide": (256, 32, 16, (0,) * 6),
+                     "wideper": (256, 16, 32, (1, 0, 1, 1, 0, 1))}[case]
     grid = pps.Grid.make(n, ny, nz, bdry_cond=bc)
     drops = [(0.3, 0.6, 0.4, 0.2), (0.8, 0.3, 0.7, 0.15)]
     rho, u, v, w = pps.synth_fields(grid, drops, 1.0, 1000.0, 1.5, seed=12)
