@@ -309,7 +309,7 @@ def main():
     f64 = res["fp64"]
     n_cells = grid.cells
     roof = {
-        "kernel": "k_rb_sweep<float, false, 1, true, true, false, true, false> (finest-level fused "
+        "kernel": "k_rb_sweep<float, false, 2, true, true, false, false, false> (finest-level fused "
                   "red-black Gauss-Seidel sweep, FP32)",
         "bound": "hbm",
         "achieved": round(m["smoother_GBps"], 1),
