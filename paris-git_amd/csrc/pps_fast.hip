@@ -1046,21 +1046,36 @@ __device__ __forceinline__ double sum_partials(const double *part, int nb) {
 }
 
 
-struct OpK {  // FP64 fine operator for the outer iteration
+// FP64 fine operator for the outer iteration: +x/+y/+z couplings and the
+// diagonal, either the compact planes the setup pass writes (0 on wall
+// faces) or the caller's A slots 2, 4, 6, 7 (mixed mode), whose wall-face
+// coefficients the setup folded into b: op_x/op_y/op_z read them as 0 on
+// a wall face (px/py/pz: periodic directions), so both give the same
+// operator.
+struct OpK {
     int nx, ny, nz;
     const double *cx, *cy, *cz, *dg;
+    int px, py, pz;
 };
+__device__ __forceinline__ double op_x(const OpK &L, long long c, int i) {
+    return (i == L.nx - 1 && !L.px) ? 0.0 : L.cx[c];
+}
+__device__ __forceinline__ double op_y(const OpK &L, long long c, int j) {
+    return (j == L.ny - 1 && !L.py) ? 0.0 : L.cy[c];
+}
+__device__ __forceinline__ double op_z(const OpK &L, long long c, int k) {
+    return (k == L.nz - 1 && !L.pz) ? 0.0 : L.cz[c];
+}
 
-// Diagonal of the FP64 outer operator at cell c (cxm, cym, czm: the
-// indices of the cell's -x/-y/-z couplings): stored, or (DGI, used when the
-// setup pass found every A7 equal to it) the six couplings added in the
-// order of the assembly's sum(A(i,j,k,1:6)) (modules.f90:2667), which
-// reproduces A7 bit for bit.
+// Diagonal of the FP64 outer operator at cell c from its six couplings
+// (-x, +x, -y, +y, -z, +z): stored, or (DGI, used when the setup pass
+// found every A7 equal to it) the six added in the order of the assembly's
+// sum(A(i,j,k,1:6)) (modules.f90:2667), which reproduces A7 bit for bit.
 template <bool DGI>
-__device__ __forceinline__ double outer_diag(const OpK &L, long long c, long long cxm, long long cym,
-                                             long long czm) {
+__device__ __forceinline__ double outer_diag(const OpK &L, long long c, double vxm, double vx, double vym,
+                                             double vy, double vzm, double vz) {
     if (!DGI) return L.dg[c];
-    return ((((L.cx[cxm] + L.cx[c]) + L.cy[cym]) + L.cy[c]) + L.cz[czm]) + L.cz[c];
+    return ((((vxm + vx) + vym) + vy) + vzm) + vz;
 }
 
 // r = b - M x ; partials: 0 = sum r^2, 1 = sum b^2 ; optional FP32 copy.
@@ -1085,11 +1100,14 @@ k_outer_resid(OpK L, int zc, const double *__restrict__ b, const double *__restr
             const long long c = col + sz * k;
             const long long czm = col + sz * wrap_m(k, L.nz);
             const double xp = x[col + sz * wrap_p(k, L.nz)];
-            double s = L.cx[c + dxm] * x[c + dxm] + L.cx[c] * x[c + dxp];
-            s += L.cy[c + dym] * x[c + dym] + L.cy[c] * x[c + dyp];
-            s += L.cz[czm] * xm + L.cz[c] * xp;
+            const double vxm = op_x(L, c + dxm, wrap_m(i, L.nx)), vx = op_x(L, c, i);
+            const double vym = op_y(L, c + dym, wrap_m(j, L.ny)), vy = op_y(L, c, j);
+            const double vzm = op_z(L, czm, wrap_m(k, L.nz)), vz = op_z(L, c, k);
+            double s = vxm * x[c + dxm] + vx * x[c + dxp];
+            s += vym * x[c + dym] + vy * x[c + dyp];
+            s += vzm * xm + vz * xp;
             const double bv = b[c];
-            const double v = bv - (outer_diag<DGI>(L, c, c + dxm, c + dym, czm) * x0 - s);
+            const double v = bv - (outer_diag<DGI>(L, c, vxm, vx, vym, vy, vzm, vz) * x0 - s);
             r[c] = v;
             if (bt) bt[c] = (TB)v;
             rr += v * v;
@@ -1189,7 +1207,7 @@ k_dq_tiled(OpK L, TGrid g, const TZ *__restrict__ z, const TZ *__restrict__ dold
             if (rd_old) dh1 = dold[hcol + p1];
         }
     }
-    double czm = in ? L.cz[own + sz * wrapk(k0 - 1)] : 0.0;
+    double czm = in ? op_z(L, own + sz * wrapk(k0 - 1), wrapk(k0 - 1)) : 0.0;
     __syncthreads();
     double dq = 0.0;
     for (int s = k0; s < k1; ++s) {
@@ -1208,10 +1226,10 @@ k_dq_tiled(OpK L, TGrid g, const TZ *__restrict__ z, const TZ *__restrict__ dold
         const long long c = own + sz * s;
         double cxw = 0.0, cxe = 0.0, cys = 0.0, cyn = 0.0, czp = 0.0, dgc = 0.0;
         if (in) {
-            cxw = L.cx[c + dxm]; cxe = L.cx[c];
-            cys = L.cy[c + dym]; cyn = L.cy[c];
-            czp = L.cz[c];
-            dgc = DGI ? ((((cxw + cxe) + cys) + cyn) + czm) + czp : L.dg[c];
+            cxw = op_x(L, c + dxm, wrap_m(i, nx)); cxe = op_x(L, c, i);
+            cys = op_y(L, c + dym, wrap_m(j, ny)); cyn = op_y(L, c, j);
+            czp = op_z(L, c, s);
+            dgc = outer_diag<DGI>(L, c, cxw, cxe, cys, cyn, czm, czp);
         }
         // d_new of plane s+1 into LDS
         const int wb = (s + 1) % 3, rb = s % 3;
@@ -1433,7 +1451,7 @@ k_setup(PackK d, int zc, const double *__restrict__ A, const double *__restrict_
                 const double vzm = (k == 0 && !d.pz) ? 0.0 : a[4];
                 if (a[6] != ((((vxm + vx) + vym) + vy) + vzm) + vz) f |= 8;
             }
-            cx[c] = vx; cy[c] = vy; cz[c] = vz; dg[c] = a[6];
+            if (cx) { cx[c] = vx; cy[c] = vy; cz[c] = vz; dg[c] = a[6]; }
             bo[c] = rhs;
             x[c] = pc;
             r[c] = rv;
@@ -1475,10 +1493,13 @@ k_check_unpack(OpK L, PackK d, int zc, const double *__restrict__ b, const doubl
             const long long c = col + sz * k;
             const long long czm = col + sz * wrap_m(k, L.nz);
             const double xp = x[col + sz * wrap_p(k, L.nz)];
-            double s = L.cx[c + dxm] * x[c + dxm] + L.cx[c] * x[c + dxp];
-            s += L.cy[c + dym] * x[c + dym] + L.cy[c] * x[c + dyp];
-            s += L.cz[czm] * xm + L.cz[c] * xp;
-            const double v = b[c] - (outer_diag<DGI>(L, c, c + dxm, c + dym, czm) * x0 - s);
+            const double vxm = op_x(L, c + dxm, wrap_m(i, L.nx)), vx = op_x(L, c, i);
+            const double vym = op_y(L, c + dym, wrap_m(j, L.ny)), vy = op_y(L, c, j);
+            const double vzm = op_z(L, czm, wrap_m(k, L.nz)), vz = op_z(L, c, k);
+            double s = vxm * x[c + dxm] + vx * x[c + dxp];
+            s += vym * x[c + dym] + vy * x[c + dyp];
+            s += vzm * xm + vz * xp;
+            const double v = b[c] - (outer_diag<DGI>(L, c, vxm, vx, vym, vy, vzm, vz) * x0 - s);
             rr += v * v;
             const long long q = (i + d.ng) + Py * (j + d.ng) + Pz * (k + d.ng);
             p[q] = x0;
@@ -1545,6 +1566,9 @@ struct FastState {
     Hier<float> h32;
     DevBuf b32;      // FP32 right-hand side of the mixed V-cycle
     int mixed_built = 0, fp64_built = 0;
+    // the outer operator is read from the caller's A (mixed mode: the FP64
+    // compact operator is not written)
+    int op_in_A = 0;
     int have_r = 0;  // the setup pass already wrote r (and its norms) for x
     size_t lds_max = 65536;  // LDS bytes per workgroup on this device
     int cus = 256;           // compute units on this device
@@ -2000,8 +2024,13 @@ static Vecs vecs(FastState *f) {
 }
 static OpK opk(FastState *f) {
     const long long n = f->d.cells();
+    const int px = f->d.per[0], py = f->d.per[1], pz = f->d.per[2];
+    if (f->op_in_A) {
+        const double *A = f->built_for;   // slots 2, 4, 6, 7
+        return OpK{f->d.nx, f->d.ny, f->d.nz, A + n, A + 3 * n, A + 5 * n, A + 6 * n, px, py, pz};
+    }
     double *o = f->op.as<double>();
-    return OpK{f->d.nx, f->d.ny, f->d.nz, o, o + n, o + 2 * n, o + 3 * n};
+    return OpK{f->d.nx, f->d.ny, f->d.nz, o, o + n, o + 2 * n, o + 3 * n, px, py, pz};
 }
 
 // pack A, detect symmetry/singularity, build the hierarchy for `mixed`
@@ -2009,7 +2038,6 @@ static int fast_setup(pps_handle *h, const double *dA, const double *dp, bool mi
     FastState *f = h->fast;
     const Dims &d = f->d;
     const long long n = d.cells();
-    const OpK O = opk(f);
     Vecs V = vecs(f);
     int *flags = f->flags.as<int>();
     std::vector<int> dims;
@@ -2026,8 +2054,11 @@ static int fast_setup(pps_handle *h, const double *dA, const double *dp, bool mi
     double *S = part + 4 * kRedBlocks;
     const ZGrid zg = zgrid(d.nx, d.ny, d.nz);
     PPS_HIP(hipMemsetAsync(flags, 0, sizeof(int), s));
-    k_setup<<<zg.grid, dim3(64, 4, 1), 0, s>>>(packk(d), zg.zc, dA, dp, (double *)O.cx, (double *)O.cy,
-                                                (double *)O.cz, (double *)O.dg, V.b, V.x, V.r, f32, r32,
+    // mixed mode: the outer kernels read the operator from A itself
+    double *o64 = f->op.as<double>();
+    k_setup<<<zg.grid, dim3(64, 4, 1), 0, s>>>(packk(d), zg.zc, dA, dp, mixed ? nullptr : o64,
+                                                mixed ? nullptr : o64 + n, mixed ? nullptr : o64 + 2 * n,
+                                                mixed ? nullptr : o64 + 3 * n, V.b, V.x, V.r, f32, r32,
                                                 part, flags);
     k_scal_rr<<<1, 1024, 0, s>>>(part, zg.blocks, 1, S);
     PPS_HIP(hipGetLastError());
@@ -2046,13 +2077,14 @@ static int fast_setup(pps_handle *h, const double *dA, const double *dp, bool mi
         Hier<double> &H = f->h64;
         if (!f->fp64_built) { PPS_TRY(hier_alloc(H, dims, false)); f->fp64_built = 1; }
         Level<double> &L0 = H.lv[0];
-        L0.cx = (double *)O.cx; L0.cy = (double *)O.cy; L0.cz = (double *)O.cz; L0.dg = (double *)O.dg;
+        L0.cx = o64; L0.cy = o64 + n; L0.cz = o64 + 2 * n; L0.dg = o64 + 3 * n;
         L0.b = nullptr;
         L0.x = V.z;
         hier_build(H, f->singular, s);
     }
     PPS_HIP(hipGetLastError());
     f->built_for = dA;
+    f->op_in_A = mixed ? 1 : 0;
     return PPS_OK;
 }
 
