@@ -1695,7 +1695,9 @@ static void rb_pass(const Level<T> &L, int l, const T *b, int colour, bool first
 // Pairs per thread: 2 in FP32 when the diagonals are the coupling sums
 // (dgi) and a row has a multiple of 128 pairs (whole wavefronts per row;
 // twice the rows per block, so half the ring-row overhead), without the
-// run-ahead loads, which would not fit the registers; else 1.
+// run-ahead loads except in the first sweep, which would not fit the
+// registers; else 1.  (Two pairs per thread on rows of 64 pairs, 128^3,
+// ran at 21 against 18 us.)
 // Planes per block: with one pair per thread a level of 2^23 cells or
 // more takes about 1024 blocks (256^3 FP64: 11 planes, within 1% of the
 // best of 2..32); otherwise the count minimises waves of blocks x steps
@@ -1799,8 +1801,10 @@ static void sweep_launch2(const SweepShape &sh, const Level<T> &L, int l, const 
                     return;
                 }
             }
+            // the first sweep reads no iterate, which leaves registers for
+            // the run-ahead loads (115 VGPRs; 89 against 95 us at 256^3)
             if (l == 0)
-                k_rb_sweep<T, FIRST, 2, true, DGI, PROL, false, false><<<sh.grid, sh.block, sh.lds, s>>>(
+                k_rb_sweep<T, FIRST, 2, true, DGI, PROL, FIRST, false><<<sh.grid, sh.block, sh.lds, s>>>(
                     K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec, nullptr);
             else
                 k_rb_sweep<T, FIRST, 2, false, DGI, PROL, false, false><<<sh.grid, sh.block, sh.lds, s>>>(
