@@ -1401,19 +1401,22 @@ k_setup(PackK d, int zc, const double *__restrict__ A, const double *__restrict_
         const bool lastx = i == d.nx - 1, lasty = j == d.ny - 1;
         const long long ex_ = lastx ? -(long long)(d.nx - 1) : 1;
         const long long ey_ = lasty ? -(long long)(d.ny - 1) * sy : sy;
+        // the column's p at k-1 and k, and the -z coupling of plane k (the
+        // symmetry check of plane k-1 loaded it), carried down the march
+        const long long q0 = (i + d.ng) + Py * (j + d.ng) + Pz * (k0 + d.ng);
+        double pb = p[q0 - Pz], pc = p[q0];
+        double a4n = A[4 * n + i + sy * j + sz * k0];
         for (int k = k0; k < k1; ++k) {
             const long long c = i + sy * j + sz * k;
             const long long q = (i + d.ng) + Py * (j + d.ng) + Pz * (k + d.ng);
             double a[8];
 #pragma unroll
-            for (int s = 0; s < 8; ++s) a[s] = A[s * n + c];
+            for (int s = 0; s < 8; ++s) a[s] = s == 4 ? a4n : A[s * n + c];
             bool nan = false;
 #pragma unroll
             for (int s = 0; s < 8; ++s) nan |= a[s] != a[s];
             if (nan || a[6] < 1e-55) f |= 1;
-            const double pc = p[q];
-            const double pw = p[q - 1], pe = p[q + 1], ps = p[q - Py], pn = p[q + Py],
-                         pb = p[q - Pz], pt = p[q + Pz];
+            const double pw = p[q - 1], pe = p[q + 1], ps = p[q - Py], pn = p[q + Py], pt = p[q + Pz];
             const double rv = a[7] - a[6] * pc + a[0] * pw + a[1] * pe + a[2] * ps + a[3] * pn +
                               a[4] * pb + a[5] * pt;
             const bool lastz = k == d.nz - 1;
@@ -1439,9 +1442,11 @@ k_setup(PackK d, int zc, const double *__restrict__ A, const double *__restrict_
             }
             if (!lastx || d.px) { if (A[c + ex_] != a[1]) f |= 2; }
             if (!lasty || d.py) { if (A[2 * n + c + ey_] != a[3]) f |= 2; }
-            if (!lastz || d.pz) {
+            {
+                // the +z neighbour's -z coupling: the next plane's a[4]
                 const long long e = lastz ? c - (long long)(d.nz - 1) * sz : c + sz;
-                if (A[4 * n + e] != a[5]) f |= 2;
+                a4n = A[4 * n + e];
+                if ((!lastz || d.pz) && a4n != a[5]) f |= 2;
             }
             if (fabs(a[6] - kept) > 1e-12 * fabs(a[6])) f |= 4;
             {
@@ -1462,6 +1467,8 @@ k_setup(PackK d, int zc, const double *__restrict__ A, const double *__restrict_
             }
             rr += rv * rv;
             bb += rhs * rhs;
+            pb = pc;
+            pc = pt;
         }
     }
     if (f) atomicOr(flags, f);
