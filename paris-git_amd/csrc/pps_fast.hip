@@ -1736,13 +1736,13 @@ static SweepShape sweep_shape(int nx, int ny, int nz, int max_rows, int planes, 
         zc = (int)(((long long)nz * ytiles + 1023) / 1024);
     } else if (zc <= 0) {
         long long best = -1;
-        for (int c = 2; c <= (nz > 2 ? nz : 2); ++c) {
+        for (int c = 1; c <= nz; ++c) {
             const long long blocks = (long long)ytiles * ((nz + c - 1) / c);
             const long long cost = ((blocks + cus - 1) / cus) * (c + 2);
             if (best < 0 || cost < best) { best = cost; zc = c; }
         }
     }
-    if (zc < 2) zc = 2;
+    if (zc < 1) zc = 1;
     if (zc > nz) zc = nz;
     sh.zc = zc;
     sh.ytiles = ytiles;
