@@ -892,25 +892,50 @@ k_tail(TailK<T> K) {
     T *sm = reinterpret_cast<T *>(smem_raw);
     const int tid = threadIdx.x;
     constexpr int NT = 1024;
-    // stage operators, the right-hand side of level 0 and the inverse
+    // stage operators, the right-hand side of level 0 and the inverse: per
+    // level, every thread issues all its loads (up to 4 elements of each
+    // array) before its LDS stores, so a level costs one memory latency
+    // rather than one per element and array
     for (int l = 0; l < K.nl; ++l) {
         const LevelK<T> &G = K.L[l];
         const int n = G.nx * G.ny * G.nz;
-        T *cx = sm + K.off[l][0], *cy = sm + K.off[l][1], *cz = sm + K.off[l][2];
-        T *dg = sm + K.off[l][3];
-        for (int e = tid; e < n; e += NT) {
-            cx[e] = G.cx[e];
-            cy[e] = G.cy[e];
-            cz[e] = G.cz[e];
-            if (!DGI) dg[e] = G.dg[e];
+        const T *src[5] = {G.cx, G.cy, G.cz, DGI ? G.cx : G.dg, l == 0 ? K.b0 : G.cx};
+        T *dst[5] = {sm + K.off[l][0], sm + K.off[l][1], sm + K.off[l][2], sm + K.off[l][3],
+                     sm + K.off[0][4]};
+        const int na = 3 + (DGI ? 0 : 1);
+        for (int e0 = tid; e0 < n; e0 += 4 * NT) {
+            T v[5][4];
+#pragma unroll
+            for (int a = 0; a < 5; ++a) {
+                const bool use = a < 3 || (a == 3 && na == 4) || (a == 4 && l == 0);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int e = e0 + u * NT;
+                    v[a][u] = (use && e < n) ? src[a][e] : (T)0;
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < 5; ++a) {
+                const bool use = a < 3 || (a == 3 && na == 4) || (a == 4 && l == 0);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int e = e0 + u * NT;
+                    if (use && e < n) dst[a][e] = v[a][u];
+                }
+            }
         }
     }
     {
-        const int n0 = K.L[0].nx * K.L[0].ny * K.L[0].nz;
-        T *b = sm + K.off[0][4];
-        for (int e = tid; e < n0; e += NT) b[e] = K.b0[e];
         T *mi = sm + K.off_minv;
-        for (int e = tid; e < K.nb * K.nb; e += NT) mi[e] = K.minv[e];
+        const int nm = K.nb * K.nb;
+        for (int e0 = tid; e0 < nm; e0 += 4 * NT) {
+            T v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = e0 + u * NT < nm ? K.minv[e0 + u * NT] : (T)0;
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (e0 + u * NT < nm) mi[e0 + u * NT] = v[u];
+        }
     }
     __syncthreads();
     auto lev = [&](int l) {
