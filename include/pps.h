@@ -135,6 +135,27 @@ int pps_solve_device(pps_handle *h, const double *dA, double *dp,
 
 /* Residual history of the last solve: tres2 after each outer iteration.
  * Returns the number of entries written (<= cap). */
+/* FreeSolver (freesurface.f90:700-808; called at paris.f90:496 and 606):
+ * RB-SOR (RedBlackRelax_fs, freesurface.f90:857-895) on the active cells of
+ * the free-surface mask only: pcmask == 0 for solver_flag 1, pcmask in
+ * {1, 2} for solver_flag 2.  First p = P_gas where pcmask != 0 (flag 1) or
+ * p = 0 where pcmask == 3 (flag 2).  Stops when the max-norm residual over
+ * the active cells is below maxError (the reference's use_L_inf); tres2 =
+ * sqrt(sum r^2) / (active cells); it = maxit+1 when not converged.  The
+ * reference takes pcmask, P_gas and solver_flag from module_freesurface
+ * (modules.f90:411-420); here they are arguments: pcmask (int32) and
+ * P_gas are padded like p (vofmodules.f90:446-447); P_gas may be NULL for
+ * solver_flag 2.  Returns PPS_ERR_BAD_ARG for a solver_flag other than 1
+ * or 2 (the reference aborts, freesurface.f90:722) and PPS_ERR_DIVERGED
+ * where the reference aborts in catch_divergence_fs. */
+int pps_free_solver(pps_handle *h, const double *A, double *p, const int *pcmask,
+                    const double *P_gas, int solver_flag, double maxError, double beta,
+                    int maxit, int *it, int *ierr, double *tres2);
+/* The same on device arrays, on the given stream (NULL: the default). */
+int pps_free_solver_device(pps_handle *h, const double *dA, double *dp, const int *dpcmask,
+                           const double *dP_gas, int solver_flag, double maxError,
+                           double beta, int maxit, int *it, double *tres2, void *stream);
+
 int pps_last_history(pps_handle *h, double *hist, int cap);
 
 /* Adjoint of the last-solved system (no reference counterpart).  With

@@ -80,6 +80,7 @@ def lib():
         "orc_coarse_fine_interp": (None, [IP, _I, _I, _P, _P]),
         "orc_newsolver_mg": (_I, [G, _P, _P, _D, _D, _I, _I, _I, _I, IP, DP, _P]),
         "orc_newsolver_std": (_I, [G, _P, _P, _D, _D, _I, _I, IP, DP, _P]),
+        "orc_free_solver": (_I, [G, _P, _P, _P, _P, _I, _D, _D, _I, IP, DP, _P]),
         "orc_setup_testmg": (None, [G, _P, _P]),
         "orc_get_mgtest_err": (_D, [G, _P]),
         "orc_project_velocity": (None, [G, _P, _P, _P, _P, _P, _P, _P, _P, _D]),
@@ -183,6 +184,18 @@ def newsolver_std(g, A, p, maxError, beta=1.2, maxit=50, norm=2):
     hist = np.zeros(max(maxit, 1))
     rc = lib().orc_newsolver_std(ctypes.byref(g), _p(A), _p(p), maxError, beta, maxit, norm,
                                  ctypes.byref(it), ctypes.byref(t2), _p(hist))
+    return rc, it.value, t2.value, hist[:min(it.value, maxit)]
+
+
+def free_solver(g, A, p, pcmask, pgas, solver_flag, maxError, beta=1.2, maxit=50):
+    """FreeSolver on p (in place); pcmask int32 and pgas float64, both padded
+    like p.  Returns (rc, it, tres2, history)."""
+    assert pcmask.dtype == np.int32 and pcmask.flags["C_CONTIGUOUS"]
+    it, t2 = ctypes.c_int(), ctypes.c_double()
+    hist = np.zeros(max(maxit, 1))
+    rc = lib().orc_free_solver(ctypes.byref(g), _p(A), _p(p), ctypes.c_void_p(pcmask.ctypes.data),
+                               _p(pgas), solver_flag, maxError, beta, maxit, ctypes.byref(it),
+                               ctypes.byref(t2), _p(hist))
     return rc, it.value, t2.value, hist[:min(it.value, maxit)]
 
 

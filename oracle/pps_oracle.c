@@ -663,6 +663,83 @@ int orc_newsolver_std(const orc_grid *g, const double *A, double *p,
 }
 
 /* ------------------------------------------------------------------ */
+/* FreeSolver: freesurface.f90:700-855, RedBlackRelax_fs: 857-895        */
+/* ------------------------------------------------------------------ */
+static int fs_active(int m, int flag) {
+    return flag == 1 ? m == 0 : (m == 1 || m == 2);   /* 880-881 */
+}
+
+int orc_free_solver(const orc_grid *g, const double *A, double *p,
+                    const int *pcmask, const double *pgas, int solver_flag,
+                    double maxError, double beta, int maxit, int *it,
+                    double *tres2, double *hist) {
+    if (solver_flag != 1 && solver_flag != 2) return ORC_ERR_UNSUPPORTED;   /* 722 */
+    lvl_t L = grid_lvl(g);
+    long n = ncell(&L), sx = 1, sy = L.X, sz = (long)L.X * L.Y;
+    /* 723-726: fixed values of the cells the solve does not update */
+    for (int k = 0; k < g->nz; ++k)
+        for (int j = 0; j < g->ny; ++j)
+            for (int i = 0; i < g->nx; ++i) {
+                long q = ip(&L, i, j, k);
+                if (solver_flag == 1 && pcmask[q] != 0) p[q] = pgas[q];
+                if (solver_flag == 2 && pcmask[q] == 3) p[q] = 0.0;
+            }
+    apply_bc_lvl(&L, p);
+    double t2 = 0.0;
+    int it_ = 1;
+    for (it_ = 1; it_ <= maxit; ++it_) {
+        /* RedBlackRelax_fs: even local parity first, halo, then odd */
+        for (int pass = 0; pass < 2; ++pass) {
+            for (int k = 0; k < g->nz; ++k)
+                for (int j = 0; j < g->ny; ++j)
+                    for (int i = (j + k + pass) & 1; i < g->nx; i += 2) {
+                        long c = ic(&L, i, j, k), q = ip(&L, i, j, k);
+                        if (!fs_active(pcmask[q], solver_flag)) continue;
+                        p[q] = (1.0 - beta) * p[q] +
+                               beta / A[6 * n + c] *
+                                   (A[0 * n + c] * p[q - sx] + A[1 * n + c] * p[q + sx] +
+                                    A[2 * n + c] * p[q - sy] + A[3 * n + c] * p[q + sy] +
+                                    A[4 * n + c] * p[q - sz] + A[5 * n + c] * p[q + sz] +
+                                    A[7 * n + c]);
+                    }
+            if (pass == 0) apply_bc_lvl(&L, p);
+        }
+        apply_bc_lvl(&L, p);
+        /* 738-785: interior cells first, then the boundary shell */
+        double res2 = 0.0, resinf = 0.0, cells = 0.0;
+        int nan = 0;
+        for (int pass = 0; pass < 2; ++pass)
+            for (int k = 0; k < g->nz; ++k)
+                for (int j = 0; j < g->ny; ++j)
+                    for (int i = 0; i < g->nx; ++i) {
+                        int inner = i > 0 && i < g->nx - 1 && j > 0 && j < g->ny - 1 &&
+                                    k > 0 && k < g->nz - 1;
+                        if (inner != (pass == 0)) continue;
+                        long c = ic(&L, i, j, k), q = ip(&L, i, j, k);
+                        if (!fs_active(pcmask[q], solver_flag)) continue;
+                        double r = fabs(-p[q] * A[6 * n + c] + A[0 * n + c] * p[q - sx] +
+                                        A[1 * n + c] * p[q + sx] + A[2 * n + c] * p[q - sy] +
+                                        A[3 * n + c] * p[q + sy] + A[4 * n + c] * p[q - sz] +
+                                        A[5 * n + c] * p[q + sz] + A[7 * n + c]);
+                        res2 = res2 + r * r;
+                        if (r > resinf) resinf = r;
+                        cells = cells + 1.0;
+                        if (r != r) nan = 1;
+                        for (int s = 0; s < 8; ++s)
+                            if (A[s * n + c] != A[s * n + c]) nan = 1;
+                    }
+        /* catch_divergence_fs (811-855) with one rank */
+        if (nan || res2 > 1e18 || res2 != res2) { *it = it_; *tres2 = res2; return ORC_ERR_DIVERGED; }
+        t2 = sqrt(res2) / cells;   /* 786-791, norm = 2 */
+        if (hist) hist[it_ - 1] = t2;
+        if (resinf < maxError) break;
+    }
+    *it = it_;
+    *tres2 = t2;
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------ */
 /* Setup_testMG (modules.f90:3160-3186), get_MGtest_err (407-426)        */
 /* `real(8) :: pi=3.14159265359` is a default-real literal: the value   */
 /* stored is the single-precision rounding of pi.                        */

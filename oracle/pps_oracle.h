@@ -103,6 +103,19 @@ int orc_newsolver_std(const orc_grid *g, const double *A, double *p,
                       double maxError, double beta, int maxit, int norm,
                       int *it, double *tres2, double *hist);
 
+/* FreeSolver (freesurface.f90:700-855) with RedBlackRelax_fs (857-895):
+ * RB-SOR on the active cells of the free-surface mask, pcmask == 0 for
+ * solver_flag 1, pcmask in {1, 2} for solver_flag 2.  pcmask (int) and
+ * P_gas carry ng ghosts like p.  Stops when the max-norm residual over the
+ * active cells is below maxError (use_L_inf); tres2 = sqrt(sum r^2) /
+ * (active cells).  hist (optional, maxit) receives tres2 per iteration.
+ * Returns ORC_ERR_UNSUPPORTED for a solver_flag other than 1 or 2 and
+ * ORC_ERR_DIVERGED for NaN or sum r^2 > 1e18 (catch_divergence_fs). */
+int orc_free_solver(const orc_grid *g, const double *A, double *p,
+                    const int *pcmask, const double *pgas, int solver_flag,
+                    double maxError, double beta, int maxit, int *it,
+                    double *tres2, double *hist);
+
 /* Setup_testMG (modules.f90:3160-3186) and get_MGtest_err
  * (MGsolver.f90:407-426); both use the reference's single-precision pi. */
 void orc_setup_testmg(const orc_grid *g, const double *rho, double *A);

@@ -158,6 +158,8 @@ def _lib():
         "pps_faces_to_A_device": (_I, [_GP, _P, _P, _P, _P, _P, _P, _P]),
         "pps_profile_enable": (_I, [_P, _I]),
         "pps_profile_read": (_I, [_P, _DP, ctypes.POINTER(ctypes.c_longlong), _DP]),
+        "pps_free_solver": (_I, [_P, _P, _P, _P, _P, _I, _D, _D, _I, _IP, _IP, _DP]),
+        "pps_free_solver_device": (_I, [_P, _P, _P, _P, _P, _I, _D, _D, _I, _IP, _DP, _P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -177,7 +179,7 @@ def exported_symbols():
         "pps_remove_rhs_mean_device", "pps_calc_residuals_device",
         "pps_project_velocity_device", "pps_synth_fields_device", "pps_synth_drops",
         "pps_coords", "pps_status_string", "pps_faces_to_A_device",
-        "pps_profile_enable", "pps_profile_read",
+        "pps_profile_enable", "pps_profile_read", "pps_free_solver", "pps_free_solver_device",
     ]
 
 
@@ -197,6 +199,18 @@ def _ptr(t):
         return ctypes.c_void_p(t.ctypes.data)
     if not t.is_contiguous():
         raise ValueError("tensor must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _iptr(t):
+    """Address of a contiguous int32 numpy array / torch tensor (masks)."""
+    if isinstance(t, np.ndarray):
+        if t.dtype != np.int32 or not t.flags["C_CONTIGUOUS"]:
+            raise TypeError("mask must be a C-contiguous int32 array")
+        return ctypes.c_void_p(t.ctypes.data)
+    import torch
+    if t.dtype != torch.int32 or not t.is_contiguous():
+        raise TypeError("mask must be a contiguous int32 tensor")
     return ctypes.c_void_p(t.data_ptr())
 
 
@@ -277,6 +291,33 @@ class Solver:
                                       ctypes.byref(t2))
         _check(rc, "NewSolverMG", allow=(OK, ERR_NOT_CONVERGED))
         return it.value, ierr.value, t2.value
+
+    def free_solver(self, A, p, pcmask, P_gas, solver_flag, maxError, beta=1.2, maxit=50):
+        """FreeSolver (freesurface.f90:700-808) on host arrays: p (float64)
+        is updated in place; pcmask (int32) and P_gas (float64, or None for
+        solver_flag 2) are padded like p.  Returns (it, ierr, tres2)."""
+        it, ierr, t2 = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        rc = _lib().pps_free_solver(self._h, _ptr(A), _ptr(p), _iptr(pcmask),
+                                    _ptr(P_gas) if P_gas is not None else None, solver_flag,
+                                    maxError, beta, maxit, ctypes.byref(it), ctypes.byref(ierr),
+                                    ctypes.byref(t2))
+        _check(rc, "FreeSolver", allow=(OK, ERR_NOT_CONVERGED))
+        return it.value, ierr.value, t2.value
+
+    def free_solve(self, A, p, pcmask, P_gas, solver_flag, maxError, beta=1.2, maxit=50,
+                   stream=None):
+        """FreeSolver in place on device tensors (pcmask int32, P_gas float64
+        or None for solver_flag 2, both padded like p); returns (it, tres2)."""
+        _need_gpu(A, p)
+        if not pcmask.is_cuda:
+            raise TypeError("pcmask must be a CUDA (HIP) int32 tensor")
+        it, t2 = ctypes.c_int(), ctypes.c_double()
+        rc = _lib().pps_free_solver_device(self._h, _ptr(A), _ptr(p), _iptr(pcmask),
+                                           _ptr(P_gas) if P_gas is not None else None,
+                                           solver_flag, maxError, beta, maxit, ctypes.byref(it),
+                                           ctypes.byref(t2), _stream(stream))
+        _check(rc, "pps_free_solver_device", allow=(OK, ERR_NOT_CONVERGED))
+        return it.value, t2.value
 
     # -- device-resident form ---------------------------------------------
     def solve(self, A, p, maxError, beta=1.2, maxit=50, norm=2, stream=None, allow_maxit=True):

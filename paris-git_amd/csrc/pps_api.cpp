@@ -94,6 +94,8 @@ extern "C" int pps_finalize(pps_handle *h) {
     fast_destroy(h->fast);
     h->hA.release();
     h->hp.release();
+    h->hmask.release();
+    h->hpgas.release();
     h->red.release();
     if (h->pinned) (void)hipHostFree(h->pinned);
     for (auto &e : h->prof.ev)
@@ -196,6 +198,68 @@ extern "C" int pps_new_solver_mg(pps_handle *h, const double *A, double *p, doub
     if (!h) return PPS_ERR_BAD_ARG;
     const int method = h->opt.method == PPS_SOR ? PPS_MG_REFERENCE : h->opt.method;
     return host_solve(h, method, A, p, maxError, beta, maxit, it, ierr, norm, tres2);
+}
+
+// FreeSolver on device arrays (freesurface.f90:700-808)
+static int free_with(pps_handle *h, const double *dA, double *dp, const int *dmask, const double *dpgas,
+                     int solver_flag, double maxError, double beta, int maxit, int *it, double *tres2,
+                     hipStream_t s) {
+    if (!dA || !dp || !dmask || !it || !tres2) return PPS_ERR_BAD_ARG;
+    if (solver_flag == 1 && !dpgas) return PPS_ERR_BAD_ARG;
+    (void)hipSetDevice(h->device);
+    PPS_HIP(hipEventRecord(h->ev[0], s));
+    const int rc = free_solve(h, dA, dp, dmask, dpgas, solver_flag, maxError, beta, maxit, it, tres2, s);
+    PPS_HIP(hipEventRecord(h->ev[2], s));
+    PPS_HIP(hipEventSynchronize(h->ev[2]));
+    float ms = 0.f;
+    PPS_HIP(hipEventElapsedTime(&ms, h->ev[0], h->ev[2]));
+    h->solve_ms = ms;
+    h->setup_ms = 0.0;
+    if (rc != PPS_OK) return rc;
+    return *it > maxit ? PPS_ERR_NOT_CONVERGED : PPS_OK;
+}
+
+extern "C" int pps_free_solver_device(pps_handle *h, const double *dA, double *dp, const int *dpcmask,
+                                      const double *dP_gas, int solver_flag, double maxError,
+                                      double beta, int maxit, int *it, double *tres2, void *stream) {
+    if (!h) return PPS_ERR_BAD_ARG;
+    return free_with(h, dA, dp, dpcmask, dP_gas, solver_flag, maxError, beta, maxit, it, tres2,
+                     static_cast<hipStream_t>(stream));
+}
+
+extern "C" int pps_free_solver(pps_handle *h, const double *A, double *p, const int *pcmask,
+                               const double *P_gas, int solver_flag, double maxError, double beta,
+                               int maxit, int *it, int *ierr, double *tres2) {
+    if (!h || !A || !p || !pcmask || !it) return PPS_ERR_BAD_ARG;
+    if (solver_flag == 1 && !P_gas) return PPS_ERR_BAD_ARG;
+    (void)hipSetDevice(h->device);
+    const Dims D = dims_of(h->grid);
+    const size_t bA = sizeof(double) * 8 * (size_t)D.cells();
+    const size_t bp = sizeof(double) * (size_t)D.padded();
+    const size_t bm = sizeof(int) * (size_t)D.padded();
+    PPS_TRY(h->hA.ensure(bA));
+    PPS_TRY(h->hp.ensure(bp));
+    PPS_TRY(h->hmask.ensure(bm));
+    hipStream_t s = h->stream;
+    PPS_HIP(hipMemcpyAsync(h->hA.ptr, A, bA, hipMemcpyHostToDevice, s));
+    PPS_HIP(hipMemcpyAsync(h->hp.ptr, p, bp, hipMemcpyHostToDevice, s));
+    PPS_HIP(hipMemcpyAsync(h->hmask.ptr, pcmask, bm, hipMemcpyHostToDevice, s));
+    const double *dpg = nullptr;
+    if (P_gas) {
+        PPS_TRY(h->hpgas.ensure(bp));
+        PPS_HIP(hipMemcpyAsync(h->hpgas.ptr, P_gas, bp, hipMemcpyHostToDevice, s));
+        dpg = h->hpgas.as<double>();
+    }
+    double t2 = 0.0;
+    const int rc = free_with(h, h->hA.as<double>(), h->hp.as<double>(), h->hmask.as<int>(), dpg,
+                             solver_flag, maxError, beta, maxit, it, &t2, s);
+    if (rc == PPS_OK || rc == PPS_ERR_NOT_CONVERGED) {
+        PPS_HIP(hipMemcpyAsync(p, h->hp.ptr, bp, hipMemcpyDeviceToHost, s));
+        PPS_HIP(hipStreamSynchronize(s));
+    }
+    if (tres2) *tres2 = t2;
+    if (ierr) *ierr = (rc == PPS_OK || rc == PPS_ERR_NOT_CONVERGED) ? 0 : rc;
+    return rc;
 }
 
 extern "C" int pps_last_history(pps_handle *h, double *hist, int cap) {

@@ -161,6 +161,53 @@ int compute_residual_norm(pps_handle *h, const Dims &d, const double *A,
     return PPS_OK;
 }
 
+// FreeSolver's residual statistics over the active cells (pcmask == 0 for
+// solver_flag 1, pcmask in {1, 2} for solver_flag 2; freesurface.f90:
+// 738-785, the expression of 742-745): comp 0 = sum r^2, 1 = max |r|,
+// 2 = active cells, 3 = active cells whose residual or any A slot is NaN
+// (catch_divergence_fs, 811-855).  pcmask is padded like p.
+__global__ void __launch_bounds__(256)
+k_resid_stats_fs(DK d, const double *__restrict__ A, const double *__restrict__ p,
+                 const int *__restrict__ pcmask, int flag, double *__restrict__ part) {
+    __shared__ double sm[4];
+    const long long n = (long long)d.nx * d.ny * d.nz;
+    const long long sy = d.X, sz = (long long)d.X * d.Y;
+    double s2 = 0.0, mx = 0.0, cnt = 0.0, bad = 0.0;
+    for (long long c = blockIdx.x * 256LL + threadIdx.x; c < n; c += (long long)gridDim.x * 256) {
+        const int i = (int)(c % d.nx);
+        const long long t = c / d.nx;
+        const int j = (int)(t % d.ny), k = (int)(t / d.ny);
+        const long long q = padi(d, i, j, k);
+        const int m = pcmask[q];
+        if (!(flag == 1 ? m == 0 : (m == 1 || m == 2))) continue;
+        const double r = fabs(-p[q] * A[6 * n + c] + A[c] * p[q - 1] + A[n + c] * p[q + 1] +
+                              A[2 * n + c] * p[q - sy] + A[3 * n + c] * p[q + sy] +
+                              A[4 * n + c] * p[q - sz] + A[5 * n + c] * p[q + sz] + A[7 * n + c]);
+        bool nan = r != r;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) nan |= A[s * n + c] != A[s * n + c];
+        s2 += r * r;
+        mx = fmax(mx, r);
+        cnt += 1.0;
+        if (nan) bad += 1.0;
+    }
+    double t;
+    t = block_sum256(s2, sm); if (threadIdx.x == 0) part[0 * kRedBlocks + blockIdx.x] = t; __syncthreads();
+    t = block_max256(mx, sm); if (threadIdx.x == 0) part[1 * kRedBlocks + blockIdx.x] = t; __syncthreads();
+    t = block_sum256(cnt, sm); if (threadIdx.x == 0) part[2 * kRedBlocks + blockIdx.x] = t; __syncthreads();
+    t = block_sum256(bad, sm); if (threadIdx.x == 0) part[3 * kRedBlocks + blockIdx.x] = t;
+}
+
+int fs_residual_stats(pps_handle *h, const Dims &d, const double *A, const double *p,
+                      const int *pcmask, int flag, double out[4], hipStream_t s) {
+    PPS_TRY(ensure_red(h));
+    const int nb = red_blocks(d.cells());
+    k_resid_stats_fs<<<nb, 256, 0, s>>>(dk(d), A, p, pcmask, flag, red_part(h));
+    PPS_TRY(red_finish(h, nb, 4, 1 << 1, s));
+    for (int c = 0; c < 4; ++c) out[c] = h->pinned[c];
+    return PPS_OK;
+}
+
 int residual_stats(pps_handle *h, const Dims &d, const double *A,
                    const double *p, double out[3], hipStream_t s) {
     PPS_TRY(ensure_red(h));

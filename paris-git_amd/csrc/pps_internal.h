@@ -123,6 +123,7 @@ struct pps_handle {
     double solve_ms = 0.0, setup_ms = 0.0;
     // scratch device buffers shared by drivers (A/p staging for host calls)
     pps::DevBuf hA, hp;
+    pps::DevBuf hmask, hpgas;      // pcmask / P_gas staging for host FreeSolver calls
     pps::DevBuf red;               // reduction partials + results
     double *pinned = nullptr;      // 16 pinned host doubles for scalar readback
     pps::Prof prof;                // fine-level smoother timing
@@ -153,6 +154,11 @@ int compute_residual_norm(pps_handle *h, const Dims &d, const double *A,
                           const double *p, int norm, double *out_sum,
                           hipStream_t s);
 // {sum |r|, sum r^2, max |r|} (calcResiduals before normalisation).
+int fs_residual_stats(pps_handle *h, const Dims &d, const double *A, const double *p,
+                      const int *pcmask, int flag, double out[4], hipStream_t s);
+int free_solve(pps_handle *h, const double *dA, double *dp, const int *dmask, const double *dpgas,
+               int solver_flag, double maxError, double beta, int maxit, int *it, double *tres2,
+               hipStream_t s);
 int residual_stats(pps_handle *h, const Dims &d, const double *A,
                    const double *p, double out[3], hipStream_t s);
 

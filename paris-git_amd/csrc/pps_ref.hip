@@ -382,4 +382,89 @@ int sor_solve(pps_handle *h, const double *dA, double *dp, double maxError,
     return PPS_OK;
 }
 
+// ---------------------------------------------------------------------
+// FreeSolver (freesurface.f90:700-855): RB-SOR on the cells the
+// free-surface mask selects.  Active: pcmask == 0 (solver_flag 1), pcmask
+// in {1, 2} (solver_flag 2); pcmask and P_gas are padded like p.
+// ---------------------------------------------------------------------
+__device__ __forceinline__ bool fs_active(int m, int flag) {
+    return flag == 1 ? m == 0 : (m == 1 || m == 2);
+}
+
+// freesurface.f90:723-726: p = P_gas where pcmask != 0 (flag 1), p = 0
+// where pcmask == 3 (flag 2), on the interior
+__global__ void __launch_bounds__(256)
+k_fs_init(LvlK L, const int *__restrict__ pcmask, const double *__restrict__ pgas,
+          double *__restrict__ p, int flag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= L.nx || j >= L.ny) return;
+    const long long q = ipad(L, i, j, k);
+    const int m = pcmask[q];
+    if (flag == 1 && m != 0) p[q] = pgas[q];
+    if (flag == 2 && m == 3) p[q] = 0.0;
+}
+
+// RedBlackRelax_fs colour pass (freesurface.f90:871-893): the update of
+// k_rb_pass on the active cells only
+__global__ void __launch_bounds__(256)
+k_rb_pass_fs(LvlK L, const double *__restrict__ A, double *__restrict__ p,
+             const int *__restrict__ pcmask, int flag, double beta, int colour) {
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    const int i = 2 * (int)(blockIdx.x * blockDim.x + threadIdx.x) + ((j + k + colour) & 1);
+    if (j >= L.ny || i >= L.nx) return;
+    const long long q = ipad(L, i, j, k);
+    if (!fs_active(pcmask[q], flag)) return;
+    const long long n = lk_cells(L);
+    const long long c = icel(L, i, j, k);
+    const long long sy = L.X, sz = (long long)L.X * L.Y;
+    const double s = A[c] * p[q - 1] + A[n + c] * p[q + 1] + A[2 * n + c] * p[q - sy] +
+                     A[3 * n + c] * p[q + sy] + A[4 * n + c] * p[q - sz] +
+                     A[5 * n + c] * p[q + sz] + A[7 * n + c];
+    p[q] = (1.0 - beta) * p[q] + beta / A[6 * n + c] * s;
+}
+
+// FreeSolver: initial values, then per iteration one masked RB-SOR sweep
+// (pass 0, halo, pass 1), the halo, and the residual statistics of the
+// active cells; stops when max |r| < maxError (use_L_inf), reports
+// tres2 = sqrt(sum r^2) / (active cells) (freesurface.f90:786-803);
+// it = maxit + 1 when not converged.  NaN or sum r^2 > 1e18 returns
+// PPS_ERR_DIVERGED (catch_divergence_fs with one rank).
+int free_solve(pps_handle *h, const double *dA, double *dp, const int *dmask, const double *dpgas,
+               int solver_flag, double maxError, double beta, int maxit, int *it, double *tres2,
+               hipStream_t s) {
+    if (solver_flag != 1 && solver_flag != 2) return PPS_ERR_BAD_ARG;   // freesurface.f90:722
+    const pps_grid &g = h->grid;
+    const Dims D = dims_of(g);
+    LvlK L = lvlk(g.nx, g.ny, g.nz, g.ng, D.per);
+    h->hist.clear();
+    k_fs_init<<<cell_grid(L.nx, L.ny, L.nz), kCellBlock, 0, s>>>(L, dmask, dpgas, dp, solver_flag);
+    launch_halo(L, dp, s);
+    const dim3 grid((unsigned)(((L.nx + 1) / 2 + 63) / 64), (unsigned)((L.ny + 3) / 4), (unsigned)L.nz);
+    double t2 = 0.0;
+    int k;
+    for (k = 1; k <= maxit; ++k) {
+        k_rb_pass_fs<<<grid, kCellBlock, 0, s>>>(L, dA, dp, dmask, solver_flag, beta, 0);
+        launch_halo(L, dp, s);
+        k_rb_pass_fs<<<grid, kCellBlock, 0, s>>>(L, dA, dp, dmask, solver_flag, beta, 1);
+        launch_halo(L, dp, s);
+        PPS_HIP(hipGetLastError());
+        double st[4];
+        PPS_TRY(fs_residual_stats(h, D, dA, dp, dmask, solver_flag, st, s));
+        if (st[3] > 0.0 || !(st[0] <= 1e18)) {
+            *it = k;
+            *tres2 = st[0];
+            return PPS_ERR_DIVERGED;
+        }
+        t2 = std::sqrt(st[0]) / st[2];
+        h->hist.push_back(t2);
+        if (st[1] < maxError) break;
+    }
+    *it = k;
+    *tres2 = t2;
+    return PPS_OK;
+}
+
 }  // namespace pps

@@ -2,6 +2,8 @@
 ! libpps (include/pps.h) and wrappers with the reference's argument lists
 !   NewSolver(A,p,maxError,beta,maxit,it,ierr,norm)          newsolver.f90:34-57
 !   NewSolverMG(A,p,maxError,beta,maxit,it,ierr,norm,tres2)  MGsolver.f90:346-405
+!   FreeSolver(A,p,maxError,beta,maxit,it,ierr,iout,time,tres2)
+!                                                 freesurface.f90:700-808
 !   init_MG / finalize_MG                                    MGsolver.f90:34-82
 ! The module state the reference reads implicitly (grid sizes, Ng,
 ! bdry_cond, MultiGrid, nrelax, sym_MG) is handed to pps_setup once.
@@ -70,6 +72,20 @@ module pps_paris
        real(c_double), intent(out) :: tres2
      end function c_new_solver_mg
 
+     integer(c_int) function c_free_solver(h, A, p, pcmask, P_gas, solver_flag, maxError, beta, &
+          maxit, it, ierr, tres2) bind(C, name="pps_free_solver")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: A(*)
+       real(c_double), intent(inout) :: p(*)
+       integer(c_int), intent(in) :: pcmask(*)
+       real(c_double), intent(in) :: P_gas(*)
+       integer(c_int), value :: solver_flag, maxit
+       real(c_double), value :: maxError, beta
+       integer(c_int), intent(out) :: it, ierr
+       real(c_double), intent(out) :: tres2
+     end function c_free_solver
+
      integer(c_int) function c_last_timing(h, solve_ms, setup_ms) bind(C, name="pps_last_timing")
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: h
@@ -79,7 +95,7 @@ module pps_paris
 
   type(c_ptr), save :: handle = c_null_ptr
 
-  public :: pps_setup, pps_teardown, pps_newsolver, pps_newsolvermg, pps_timing
+  public :: pps_setup, pps_teardown, pps_newsolver, pps_newsolvermg, pps_freesolver, pps_timing
 
 contains
 
@@ -139,6 +155,26 @@ contains
     rc = c_new_solver_mg(handle, A, p, maxError, beta, maxit, it, ierr, norm, tres2)
     if (rc /= PPS_OK .and. rc /= PPS_ERR_NOT_CONVERGED) ierr = rc
   end subroutine pps_newsolvermg
+
+  ! FreeSolver with the reference's arguments; the module_freesurface
+  ! state it reads (pcmask, P_gas: arrays with p's bounds; solver_flag) is
+  ! passed explicitly.  iout and time only name the reference's convergence
+  ! file and are not used.
+  subroutine pps_freesolver(A, p, maxError, beta, maxit, it, ierr, iout, time, tres2, &
+                            pcmask, P_gas, solver_flag)
+    real(8), intent(in) :: A(*)
+    real(8), intent(inout) :: p(*)
+    real(8), intent(in) :: maxError, beta, time
+    integer, intent(in) :: maxit, iout, solver_flag
+    integer, intent(out) :: it, ierr
+    real(8), intent(out) :: tres2
+    integer, intent(in) :: pcmask(*)
+    real(8), intent(in) :: P_gas(*)
+    integer :: rc
+    rc = c_free_solver(handle, A, p, pcmask, P_gas, solver_flag, maxError, beta, maxit, it, &
+                       ierr, tres2)
+    if (rc /= PPS_OK .and. rc /= PPS_ERR_NOT_CONVERGED) ierr = rc
+  end subroutine pps_freesolver
 
   ! milliseconds of the last solve (whole, and the setup part)
   subroutine pps_timing(solve_ms, setup_ms)

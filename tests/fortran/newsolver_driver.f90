@@ -3,7 +3,10 @@
 ! p(imin:imax,jmin:jmax,kmin:kmax) with Ng=2 ghost layers, passed straight
 ! through.  The system is a 32^3 walled box with one dense sphere (density
 ! ratio 100) assembled with SetupPoisson's rule (modules.f90:2737-2746).
-! Usage: newsolver_driver <method 0..3> ; prints PASS/FAIL.
+! Usage: newsolver_driver <method 0..3> ; prints PASS/FAIL.  Method 9 calls
+! FreeSolver (freesurface.f90:700-808) instead: the x-boundary planes are
+! gas cells (pcmask 1) holding P_gas = 0 and 1, the source is zero, and the
+! max-norm residual of the liquid cells is checked.
 program newsolver_driver
   use pps_paris
   implicit none
@@ -11,6 +14,9 @@ program newsolver_driver
   integer :: is, ie, imin, imax
   real(8), allocatable :: A(:,:,:,:), p(:,:,:), rho(:,:,:), u(:,:,:), v(:,:,:), w(:,:,:)
   integer :: i, j, k, it, ierr, method, bc(6)
+  integer, allocatable :: pcmask(:,:,:)
+  real(8), allocatable :: pgas(:,:,:)
+  real(8) :: rmax
   real(8) :: h, x, y, z, d, c, maxError, res, rms_b, mean8, tres2, seed
   character(len=16) :: arg
 
@@ -53,6 +59,10 @@ program newsolver_driver
   p = 0d0
 
   bc = 0
+  if (method == 9) then
+     call free_case()
+     stop
+  end if
   call pps_setup(N, N, N, Ng, bc, 1d0, 1d0, 1d0, .true., 3, method, ierr)
   if (ierr /= 0) then
      print *, 'FAIL setup', ierr
@@ -77,6 +87,30 @@ program newsolver_driver
   end if
 
 contains
+
+  subroutine free_case()
+    allocate(pcmask(imin:imax, imin:imax, imin:imax), pgas(imin:imax, imin:imax, imin:imax))
+    pcmask = 0; pcmask(is, :, :) = 1; pcmask(ie, :, :) = 1
+    pgas = 0d0; pgas(ie, :, :) = 1d0
+    A(:,:,:,8) = 0d0
+    maxError = 1d-9 * maxval(A(:,:,:,7))
+    call pps_setup(N, N, N, Ng, bc, 1d0, 1d0, 1d0, .false., 3, PPS_SOR, ierr)
+    call pps_freesolver(A, p, maxError, 1.7d0, 50000, it, ierr, 0, 0d0, tres2, pcmask, pgas, 1)
+    call pps_teardown()
+    rmax = 0d0
+    do k = is, ie; do j = is, ie; do i = is + 1, ie - 1
+       rmax = max(rmax, abs(-p(i,j,k) * A(i,j,k,7) + A(i,j,k,1) * p(i-1,j,k) + &
+            A(i,j,k,2) * p(i+1,j,k) + A(i,j,k,3) * p(i,j-1,k) + A(i,j,k,4) * p(i,j+1,k) + &
+            A(i,j,k,5) * p(i,j,k-1) + A(i,j,k,6) * p(i,j,k+1) + A(i,j,k,8)))
+    end do; end do; end do
+    if (ierr == 0 .and. it <= 50000 .and. rmax < maxError .and. all(p(is,is:ie,is:ie) == 0d0) &
+        .and. all(p(ie,is:ie,is:ie) == 1d0)) then
+       print '(a,i6,a,es10.3)', 'PASS FreeSolver it=', it, ' max|r|/maxError=', rmax / maxError
+    else
+       print '(a,i6,a,i3,a,es10.3)', 'FAIL FreeSolver it=', it, ' ierr=', ierr, ' max|r|=', rmax
+       stop 1
+    end if
+  end subroutine free_case
 
   real(8) function frand()
     ! deterministic sequence in (-1, 1)

@@ -292,3 +292,91 @@ def test_newsolver_mg_threads_do_not_change_the_algorithm(orc):
     orc.set_threads(1)
     assert np.array_equal(out[0][0], out[1][0])
     assert np.allclose(out[0][1], out[1][1], rtol=1e-12, atol=0)
+
+
+# --- FreeSolver (freesurface.f90:700-855) -----------------------------------
+
+def _fs_mask(g, rng, values):
+    m = np.zeros(g.shape_p, dtype=np.int32)
+    g.interior(m)[...] = rng.choice(values, size=(g.nz, g.ny, g.nx)).astype(np.int32)
+    return m
+
+
+def test_free_solver_with_every_cell_active_is_newsolver_std(orc):
+    """solver_flag 1 and pcmask == 0 everywhere: the masked sweep is
+    RedBlackRelax itself, so from the same (ghost-wrapped) start a fixed
+    number of sweeps (maxError 0 never stops either solver) gives
+    NewSolver_std's p bit for bit."""
+    g = orc.OrcGrid.make(12, bc=(1, 0, 0, 1, 0, 0))
+    A, _ = system(orc, g, [(0.4, 0.5, 0.6, 0.25)], rho2=100.0)
+    rng = np.random.default_rng(3)
+    p0 = np.zeros(g.shape_p)
+    g.interior(p0)[...] = rng.standard_normal((g.nz, g.ny, g.nx))
+    orc.apply_bc(g, p0)   # FreeSolver wraps the ghosts first (freesurface.f90:727)
+    p1, p2 = p0.copy(), p0.copy()
+    m = np.zeros(g.shape_p, dtype=np.int32)
+    rc1, it1, _, _ = orc.free_solver(g, A, p1, m, np.zeros(g.shape_p), 1, 0.0, 1.3, 7)
+    rc2, it2, _, _ = orc.newsolver_std(g, A, p2, 0.0, 1.3, 7)
+    assert rc1 == rc2 == 0 and it1 == it2 == 8
+    assert np.array_equal(p1, p2)
+
+
+def test_free_solver_dirichlet_planes_closed_form(orc):
+    """Uniform density, walls, no source; the planes i = 0 and i = nx-1 are
+    gas cells (pcmask 1) holding P_gas = 0 and 1.  The discrete solution on
+    the liquid cells is exactly linear in i, p = i / (nx - 1); gas cells keep
+    P_gas, and the stop is on the max-norm residual."""
+    g = orc.OrcGrid.make(12, 6, 6)
+    A, _ = system(orc, g, [], rho2=1.0, remove_mean=False)
+    A[7] = 0.0
+    m = np.zeros(g.shape_p, dtype=np.int32)
+    ng = g.ng
+    m[ng:ng + g.nz, ng:ng + g.ny, ng] = 1
+    m[ng:ng + g.nz, ng:ng + g.ny, ng + g.nx - 1] = 1
+    pgas = np.zeros(g.shape_p)
+    pgas[..., ng + g.nx - 1] = 1.0
+    p = np.zeros(g.shape_p)
+    tol = 1e-12 * A[6].max()
+    rc, it, t2, hist = orc.free_solver(g, A, p, m, pgas, 1, tol, 1.7, 20000)
+    assert rc == 0 and it < 20000
+    x = g.interior(p)
+    lin = np.arange(g.nx) / (g.nx - 1)
+    assert np.abs(x - lin[None, None, :]).max() < 1e-9
+    assert np.all(x[..., 0] == 0.0) and np.all(x[..., -1] == 1.0)
+    r = orc.residual_field(g, A, p)[..., 1:-1]
+    assert np.abs(r).max() < tol
+    assert t2 == pytest.approx(np.sqrt((r ** 2).sum()) / r.size, rel=1e-8)
+
+
+def test_free_solver_flag2_mask_rules(orc):
+    """solver_flag 2 updates pcmask 1 and 2 cells, zeroes pcmask 3 cells and
+    leaves pcmask 0 cells (and P_gas) alone; converged active cells satisfy
+    their equations."""
+    g = orc.OrcGrid.make(10, bc=(1, 0, 0, 1, 0, 0))
+    A, _ = system(orc, g, [(0.5, 0.4, 0.5, 0.3)], rho2=50.0)
+    rng = np.random.default_rng(9)
+    m = _fs_mask(g, rng, [0, 1, 2, 3])
+    p = np.zeros(g.shape_p)
+    g.interior(p)[...] = rng.standard_normal((g.nz, g.ny, g.nx))
+    p_in = p.copy()
+    scale = np.abs(A[6]).max()
+    rc, it, t2, hist = orc.free_solver(g, A, p, m, np.full(g.shape_p, 7.0), 2, 1e-10 * scale, 1.2, 20000)
+    assert rc == 0 and it < 20000
+    mi, x, x0 = g.interior(m), g.interior(p), g.interior(p_in)
+    assert np.all(x[mi == 3] == 0.0)
+    assert np.array_equal(x[mi == 0], x0[mi == 0])
+    act = (mi == 1) | (mi == 2)
+    r = orc.residual_field(g, A, p)
+    assert np.abs(r[act]).max() < 1e-10 * scale
+    assert t2 == pytest.approx(np.sqrt((r[act] ** 2).sum()) / act.sum(), rel=1e-8)
+    assert len(hist) == it and hist[-1] == t2
+
+
+def test_free_solver_rejects_flag_and_nan(orc):
+    g = orc.OrcGrid.make(8)
+    A, _ = system(orc, g, [(0.5, 0.5, 0.5, 0.25)], rho2=10.0)
+    m = np.zeros(g.shape_p, dtype=np.int32)
+    z = np.zeros(g.shape_p)
+    assert orc.free_solver(g, A, z.copy(), m, z, 0, 1e-8)[0] == orc.ERR_UNSUPPORTED
+    B = A.copy(); B[3, 2, 2, 2] = np.nan
+    assert orc.free_solver(g, B, z.copy(), m, z, 1, 1e-8)[0] == orc.ERR_DIVERGED
