@@ -183,6 +183,35 @@ def cpu_baseline(A_host, grid, cycles, n_solve_cycles, extrapolated):
     }
 
 
+def free_run(pps, torch, args):
+    """FreeSolver (solver_flag 1): cells with density above the mean are
+    gas cells (pcmask 1, p = P_gas = 0), the rest liquid; a fixed number of
+    iterations (maxError 0), timed with HIP events inside the library.
+    Algorithmic bytes per iteration: the two colour passes read the 8 A
+    planes, pcmask and p once and write p (84 B/cell), the residual pass
+    reads the 8 planes, pcmask and p again (76 B/cell)."""
+    import numpy as np
+    print(f"[bench] building {args.workload}", file=sys.stderr, flush=True)
+    grid, A, f = pps.make_problem(args.workload)
+    rho = f["rho"]
+    pcmask = (rho > rho.mean()).to(torch.int32).contiguous()
+    pgas = torch.zeros(grid.shape_p, dtype=torch.float64, device="cuda")
+    ms = []
+    with pps.Solver(grid, method="sor") as s:
+        for step in range(args.warmup + args.steps):
+            p = torch.zeros(grid.shape_p, dtype=torch.float64, device="cuda")
+            it, t2 = s.free_solve(A, p, pcmask, pgas, 1, 0.0, 1.2, args.free)
+            if step >= args.warmup:
+                ms.append(s.timing()[0])
+    per_it = float(np.median(ms)) / args.free
+    gbps = grid.cells * (84 + 76) / (per_it * 1e-3) / 1e9
+    print(json.dumps({"mode": "free_solver", "workload": args.workload, "cells": grid.cells,
+                      "iterations": args.free, "ms_per_iteration": round(per_it, 4),
+                      "algorithmic_GBps": round(gbps, 1),
+                      "gas_cells": int(pps.Grid.interior(grid, pcmask).sum().item()),
+                      "steps": args.steps, "warmup": args.warmup, "dtype": "f64"}), flush=True)
+
+
 def autograd_run(pps, torch, args):
     """Forward (mixed solve from face values) + backward (adjoint solve and
     face kernel) of PressureSolve, J = 0.5 ||p - mean(p) - p_target||^2."""
@@ -248,6 +277,9 @@ def main():
                     help="time one form only (for counter runs)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="run one device copy of MIB MiB (counter calibration) and exit")
+    ap.add_argument("--free", type=int, default=0, metavar="ITERS",
+                    help="time ITERS FreeSolver iterations (masked RB-SOR, solver_flag 1, the "
+                         "droplets of --workload as gas cells) instead of the solve benchmark")
     ap.add_argument("--autograd", action="store_true",
                     help="time forward + backward of the differentiable solve on --workload "
                          "(mixed form) instead of the solve benchmark")
@@ -277,6 +309,9 @@ def main():
     pps = ge.load_package()
     if args.autograd:
         autograd_run(pps, torch, args)
+        return
+    if args.free:
+        free_run(pps, torch, args)
         return
 
     print(f"[bench] building {args.workload}", file=sys.stderr, flush=True)
