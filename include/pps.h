@@ -156,6 +156,22 @@ int pps_free_solver_device(pps_handle *h, const double *dA, double *dp, const in
                            const double *dP_gas, int solver_flag, double maxError,
                            double beta, int maxit, int *it, double *tres2, void *stream);
 
+/* poi_solve (modules.f90:2411-2593; called at paris.f90:490): the HYPRE
+ * entry on the same 8-slot system (matrix A7 on the diagonal, -A1..-A6 off
+ * it, right-hand side A8, modules.f90:2447-2469).  HYPRE is not part of
+ * this build: every solver type (1 SMG, 2 PFMG, 3 GMRES, 4 BiCGSTAB) runs
+ * the handle's fast MG-PCG (PPS_MG_MIXED if the handle is mixed, else
+ * PPS_MG_FP64) to ||A8 - M p||_2 / ||A8||_2 < maxError, the relative
+ * tolerance HYPRE's SetTol gives.  p is the initial guess and the result;
+ * *num_iterations = iterations performed (at most maxit).  Returns
+ * PPS_ERR_BAD_ARG for another solver type, PPS_ERR_NOT_CONVERGED after
+ * maxit iterations. */
+int pps_poi_solve(pps_handle *h, const double *A, double *p, double maxError, int maxit,
+                  int *num_iterations, int solver_type);
+/* The same on device arrays, on the given stream (NULL: the default). */
+int pps_poi_solve_device(pps_handle *h, const double *dA, double *dp, double maxError,
+                         int maxit, int *num_iterations, int solver_type, void *stream);
+
 int pps_last_history(pps_handle *h, double *hist, int cap);
 
 /* Adjoint of the last-solved system (no reference counterpart).  With

@@ -160,6 +160,8 @@ def _lib():
         "pps_profile_read": (_I, [_P, _DP, ctypes.POINTER(ctypes.c_longlong), _DP]),
         "pps_free_solver": (_I, [_P, _P, _P, _P, _P, _I, _D, _D, _I, _IP, _IP, _DP]),
         "pps_free_solver_device": (_I, [_P, _P, _P, _P, _P, _I, _D, _D, _I, _IP, _DP, _P]),
+        "pps_poi_solve": (_I, [_P, _P, _P, _D, _I, _IP, _I]),
+        "pps_poi_solve_device": (_I, [_P, _P, _P, _D, _I, _IP, _I, _P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -180,6 +182,7 @@ def exported_symbols():
         "pps_project_velocity_device", "pps_synth_fields_device", "pps_synth_drops",
         "pps_coords", "pps_status_string", "pps_faces_to_A_device",
         "pps_profile_enable", "pps_profile_read", "pps_free_solver", "pps_free_solver_device",
+        "pps_poi_solve", "pps_poi_solve_device",
     ]
 
 
@@ -318,6 +321,24 @@ class Solver:
                                            ctypes.byref(t2), _stream(stream))
         _check(rc, "pps_free_solver_device", allow=(OK, ERR_NOT_CONVERGED))
         return it.value, t2.value
+
+    def poi_solve(self, A, p, maxError, maxit=100, solver_type=2):
+        """poi_solve (modules.f90:2411) on host arrays: relative tolerance
+        ||r||/||A8|| < maxError; returns num_iterations."""
+        n = ctypes.c_int()
+        rc = _lib().pps_poi_solve(self._h, _ptr(A), _ptr(p), maxError, maxit, ctypes.byref(n),
+                                  solver_type)
+        _check(rc, "poi_solve", allow=(OK, ERR_NOT_CONVERGED))
+        return n.value
+
+    def poi_solve_device(self, A, p, maxError, maxit=100, solver_type=2, stream=None):
+        """poi_solve on device tensors; returns num_iterations."""
+        _need_gpu(A, p)
+        n = ctypes.c_int()
+        rc = _lib().pps_poi_solve_device(self._h, _ptr(A), _ptr(p), maxError, maxit,
+                                         ctypes.byref(n), solver_type, _stream(stream))
+        _check(rc, "pps_poi_solve_device", allow=(OK, ERR_NOT_CONVERGED))
+        return n.value
 
     # -- device-resident form ---------------------------------------------
     def solve(self, A, p, maxError, beta=1.2, maxit=50, norm=2, stream=None, allow_maxit=True):

@@ -262,6 +262,55 @@ extern "C" int pps_free_solver(pps_handle *h, const double *A, double *p, const 
     return rc;
 }
 
+// poi_solve (modules.f90:2411-2593) on device arrays: the handle's fast
+// method (PPS_MG_FP64 unless it is PPS_MG_MIXED) to ||r||_2/||b||_2 <
+// maxError; *num_iterations = iterations performed (at most maxit).
+static int poi_with(pps_handle *h, const double *dA, double *dp, double maxError, int maxit,
+                    int *num_iterations, int solver_type, hipStream_t s) {
+    if (!dA || !dp || !num_iterations) return PPS_ERR_BAD_ARG;
+    if (solver_type < 1 || solver_type > 4) return PPS_ERR_BAD_ARG;   // SMG, PFMG, GMRES, BiCGSTAB
+    (void)hipSetDevice(h->device);
+    *num_iterations = 0;
+    if (maxit < 1) return PPS_ERR_NOT_CONVERGED;
+    const int saved = h->opt.method;
+    h->opt.method = saved == PPS_MG_MIXED ? PPS_MG_MIXED : PPS_MG_FP64;
+    int it = 0;
+    double t2 = 0.0;
+    const int rc = fast_solve(h, dA, dp, maxError, maxit, 2, &it, &t2, s, true);
+    h->opt.method = saved;
+    if (rc != PPS_OK) return rc;
+    *num_iterations = it > maxit ? maxit : it;
+    return it > maxit ? PPS_ERR_NOT_CONVERGED : PPS_OK;
+}
+
+extern "C" int pps_poi_solve_device(pps_handle *h, const double *dA, double *dp, double maxError,
+                                    int maxit, int *num_iterations, int solver_type, void *stream) {
+    if (!h) return PPS_ERR_BAD_ARG;
+    return poi_with(h, dA, dp, maxError, maxit, num_iterations, solver_type,
+                    static_cast<hipStream_t>(stream));
+}
+
+extern "C" int pps_poi_solve(pps_handle *h, const double *A, double *p, double maxError, int maxit,
+                             int *num_iterations, int solver_type) {
+    if (!h || !A || !p || !num_iterations) return PPS_ERR_BAD_ARG;
+    (void)hipSetDevice(h->device);
+    const Dims D = dims_of(h->grid);
+    const size_t bA = sizeof(double) * 8 * (size_t)D.cells();
+    const size_t bp = sizeof(double) * (size_t)D.padded();
+    PPS_TRY(h->hA.ensure(bA));
+    PPS_TRY(h->hp.ensure(bp));
+    hipStream_t s = h->stream;
+    PPS_HIP(hipMemcpyAsync(h->hA.ptr, A, bA, hipMemcpyHostToDevice, s));
+    PPS_HIP(hipMemcpyAsync(h->hp.ptr, p, bp, hipMemcpyHostToDevice, s));
+    const int rc = poi_with(h, h->hA.as<double>(), h->hp.as<double>(), maxError, maxit, num_iterations,
+                            solver_type, s);
+    if (rc == PPS_OK || rc == PPS_ERR_NOT_CONVERGED) {
+        PPS_HIP(hipMemcpyAsync(p, h->hp.ptr, bp, hipMemcpyDeviceToHost, s));
+        PPS_HIP(hipStreamSynchronize(s));
+    }
+    return rc;
+}
+
 extern "C" int pps_last_history(pps_handle *h, double *hist, int cap) {
     if (!h || (!hist && cap > 0)) return -PPS_ERR_BAD_ARG;
     int n = (int)h->hist.size();

@@ -2218,8 +2218,10 @@ static int outer_iterate(pps_handle *h, Hier<T> &H, bool mixed, double tol_abs, 
     return PPS_OK;
 }
 
+// relative: maxError bounds ||r||_2 / ||b||_2 (b the folded right-hand
+// side) instead of sqrt(sum r^2 / N)
 int fast_solve(pps_handle *h, const double *dA, double *dp, double maxError, int maxit, int norm,
-               int *it, double *tres2, hipStream_t s) {
+               int *it, double *tres2, hipStream_t s, bool relative) {
     PPS_TRY(fast_ensure(h));
     FastState *f = h->fast;
     const bool mixed = h->opt.method == PPS_MG_MIXED;
@@ -2229,6 +2231,12 @@ int fast_solve(pps_handle *h, const double *dA, double *dp, double maxError, int
     PPS_HIP(hipEventRecord(h->ev[1], s));
     const Dims &d = f->d;
     const double N = (double)d.cells();
+    if (relative) {
+        PPS_HIP(hipMemcpyAsync(h->pinned, f->red.as<double>() + 4 * kRedBlocks + S_BB, sizeof(double),
+                               hipMemcpyDeviceToHost, s));
+        PPS_HIP(hipStreamSynchronize(s));
+        maxError *= std::sqrt(h->pinned[0] / N);
+    }
     int total = 0, k = 0;
     double t2 = 0.0, last = 0.0;
     int rc = PPS_OK;

@@ -141,7 +141,8 @@ int sor_solve(pps_handle *h, const double *dA, double *dp, double maxError,
 void ref_destroy(RefState *r);
 
 int fast_solve(pps_handle *h, const double *dA, double *dp, double maxError,
-               int maxit, int norm, int *it, double *tres2, hipStream_t s);
+               int maxit, int norm, int *it, double *tres2, hipStream_t s,
+               bool relative = false);
 int fast_adjoint(pps_handle *h, const double *dA, const double *dg,
                  double *dlam, double rtol, int maxit, int *it, hipStream_t s);
 void fast_destroy(FastState *f);

@@ -4,6 +4,8 @@
 !   NewSolverMG(A,p,maxError,beta,maxit,it,ierr,norm,tres2)  MGsolver.f90:346-405
 !   FreeSolver(A,p,maxError,beta,maxit,it,ierr,iout,time,tres2)
 !                                                 freesurface.f90:700-808
+!   poi_solve(A,p,maxError,maxit,num_iterations,HYPRESolverType)
+!                                                 modules.f90:2411-2593
 !   init_MG / finalize_MG                                    MGsolver.f90:34-82
 ! The module state the reference reads implicitly (grid sizes, Ng,
 ! bdry_cond, MultiGrid, nrelax, sym_MG) is handed to pps_setup once.
@@ -86,6 +88,17 @@ module pps_paris
        real(c_double), intent(out) :: tres2
      end function c_free_solver
 
+     integer(c_int) function c_poi_solve(h, A, p, maxError, maxit, num_iterations, solver_type) &
+          bind(C, name="pps_poi_solve")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: A(*)
+       real(c_double), intent(inout) :: p(*)
+       real(c_double), value :: maxError
+       integer(c_int), value :: maxit, solver_type
+       integer(c_int), intent(out) :: num_iterations
+     end function c_poi_solve
+
      integer(c_int) function c_last_timing(h, solve_ms, setup_ms) bind(C, name="pps_last_timing")
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: h
@@ -95,7 +108,8 @@ module pps_paris
 
   type(c_ptr), save :: handle = c_null_ptr
 
-  public :: pps_setup, pps_teardown, pps_newsolver, pps_newsolvermg, pps_freesolver, pps_timing
+  public :: pps_setup, pps_teardown, pps_newsolver, pps_newsolvermg, pps_freesolver, &
+            pps_poi_solve, pps_timing
 
 contains
 
@@ -175,6 +189,19 @@ contains
                        ierr, tres2)
     if (rc /= PPS_OK .and. rc /= PPS_ERR_NOT_CONVERGED) ierr = rc
   end subroutine pps_freesolver
+
+  ! poi_solve with the reference's arguments (the HYPRE entry); a failure
+  ! prints its status, where the reference would stop inside HYPRE
+  subroutine pps_poi_solve(A, p, maxError, maxit, num_iterations, HYPRESolverType)
+    real(8), intent(in) :: A(*)
+    real(8), intent(inout) :: p(*)
+    real(8), intent(in) :: maxError
+    integer, intent(in) :: maxit, HYPRESolverType
+    integer, intent(out) :: num_iterations
+    integer :: rc
+    rc = c_poi_solve(handle, A, p, maxError, maxit, num_iterations, HYPRESolverType)
+    if (rc /= PPS_OK .and. rc /= PPS_ERR_NOT_CONVERGED) write(*,*) 'poi_solve: status ', rc
+  end subroutine pps_poi_solve
 
   ! milliseconds of the last solve (whole, and the setup part)
   subroutine pps_timing(solve_ms, setup_ms)

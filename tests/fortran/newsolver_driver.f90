@@ -6,7 +6,8 @@
 ! Usage: newsolver_driver <method 0..3> ; prints PASS/FAIL.  Method 9 calls
 ! FreeSolver (freesurface.f90:700-808) instead: the x-boundary planes are
 ! gas cells (pcmask 1) holding P_gas = 0 and 1, the source is zero, and the
-! max-norm residual of the liquid cells is checked.
+! max-norm residual of the liquid cells is checked.  Method 8 calls
+! poi_solve (modules.f90:2411), the HYPRE entry, with a PFMG solver type.
 program newsolver_driver
   use pps_paris
   implicit none
@@ -63,12 +64,19 @@ program newsolver_driver
      call free_case()
      stop
   end if
-  call pps_setup(N, N, N, Ng, bc, 1d0, 1d0, 1d0, .true., 3, method, ierr)
+  call pps_setup(N, N, N, Ng, bc, 1d0, 1d0, 1d0, .true., 3, merge(PPS_MG_FP64, method, method == 8), &
+                 ierr)
   if (ierr /= 0) then
      print *, 'FAIL setup', ierr
      stop 1
   end if
-  call pps_newsolver(A, p, maxError, 1.2d0, 4000, it, ierr, 2)
+  if (method == 8) then
+     ! relative tolerance: ||r|| / ||A8|| < 1e-10 is the check below
+     call pps_poi_solve(A, p, 1d-10, 100, it, 2)
+     ierr = 0
+  else
+     call pps_newsolver(A, p, maxError, 1.2d0, 4000, it, ierr, 2)
+  end if
   ! the reference's own residual (MGsolver.f90:118-124)
   res = 0d0
   do k = is, ie; do j = is, ie; do i = is, ie
