@@ -846,6 +846,9 @@ k_bottom_build_lds(LevelK<T> L, int singular, T *__restrict__ Minv) {
     for (int e = tid; e < nb * nb; e += NT) Minv[e] = (T)M[e];
 }
 
+// x = Minv b.  Minv is the inverse of the symmetric coarsest operator, so
+// thread r reads column r (consecutive threads, consecutive addresses)
+// rather than row r.
 template <class T>
 __global__ void __launch_bounds__(512)
 k_bottom_solve(int nb, const T *__restrict__ Minv, const T *__restrict__ b, T *__restrict__ x) {
@@ -854,8 +857,7 @@ k_bottom_solve(int nb, const T *__restrict__ Minv, const T *__restrict__ b, T *_
     __syncthreads();
     for (int r = threadIdx.x; r < nb; r += 512) {
         T acc = 0;
-        const T *row = Minv + (long long)r * nb;
-        for (int e = 0; e < nb; ++e) acc += row[e] * sb[e];
+        for (int e = 0; e < nb; ++e) acc += Minv[(long long)e * nb + r] * sb[e];
         x[r] = acc;
     }
 }
@@ -945,6 +947,21 @@ k_tail(TailK<T> K) {
         L.dg = sm + K.off[l][3];
         return L;
     };
+    // (a, b, c) of linear index e in an na x nb x nc box: shifts and masks
+    // when na and nb are powers of two (every level of a power-of-two grid),
+    // divisions otherwise
+    auto split = [](int e, int na, int nb, int &a, int &b, int &c) {
+        if (((na & (na - 1)) | (nb & (nb - 1))) == 0) {
+            const int la = __builtin_ctz(na), lb = __builtin_ctz(nb);
+            a = e & (na - 1);
+            b = (e >> la) & (nb - 1);
+            c = e >> (la + lb);
+        } else {
+            a = e % na;
+            b = (e / na) % nb;
+            c = e / (na * nb);
+        }
+    };
     auto cpass = [&](int l, int colour, bool first) {
         const LevelK<T> L = lev(l);
         const T *b = sm + K.off[l][4];
@@ -952,7 +969,8 @@ k_tail(TailK<T> K) {
         const int nxh = (L.nx + 1) >> 1;
         const int total = nxh * L.ny * L.nz;
         for (int e = tid; e < total; e += NT) {
-            const int ih = e % nxh, j = (e / nxh) % L.ny, k = e / (nxh * L.ny);
+            int ih, j, k;
+            split(e, nxh, L.ny, ih, j, k);
             const int i = 2 * ih + ((j + k + colour) & 1);
             if (i >= L.nx) continue;
             const long long c = i + (long long)L.nx * (j + (long long)L.ny * k);
@@ -975,7 +993,8 @@ k_tail(TailK<T> K) {
         const int cxn = L.nx >> 1, cyn = L.ny >> 1, czn = L.nz >> 1;
         T *bc = sm + K.off[l + 1][4];
         for (int e = tid; e < cxn * cyn * czn; e += NT) {
-            const int I = e % cxn, J = (e / cxn) % cyn, Kz = e / (cxn * cyn);
+            int I, J, Kz;
+            split(e, cxn, cyn, I, J, Kz);
             T acc = 0;
             for (int dk = 0; dk < 2; ++dk)
                 for (int dj = 0; dj < 2; ++dj)
@@ -994,9 +1013,10 @@ k_tail(TailK<T> K) {
         const T *b = sm + K.off[K.nl - 1][4];
         T *x = sm + K.off[K.nl - 1][5];
         const T *mi = sm + K.off_minv;
+        // column r of the (symmetric) inverse: no LDS bank conflicts
         for (int r = tid; r < nb; r += NT) {
             T acc = 0;
-            for (int e = 0; e < nb; ++e) acc += mi[r * nb + e] * b[e];
+            for (int e = 0; e < nb; ++e) acc += mi[e * nb + r] * b[e];
             x[r] = acc;
         }
         __syncthreads();
@@ -1008,7 +1028,8 @@ k_tail(TailK<T> K) {
         T *xf = sm + K.off[l][5];
         const T *xc = sm + K.off[l + 1][5];
         for (int e = tid; e < fx * fy * fz; e += NT) {
-            const int i = e % fx, j = (e / fx) % fy, k = e / (fx * fy);
+            int i, j, k;
+            split(e, fx, fy, i, j, k);
             xf[e] += xc[(i >> 1) + cx * ((j >> 1) + cy * (k >> 1))];
         }
         __syncthreads();
