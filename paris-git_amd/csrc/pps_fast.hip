@@ -148,10 +148,35 @@ k_rb(LevelK<T> L, const B *__restrict__ b, T *__restrict__ x, int colour) {
 // cells of b x, x and b (x the new iterate) go to part[block],
 // part[kRedBlocks + block] and part[2 kRedBlocks + block] (the block must
 // be whole wavefronts).
+// LOP (FP32, two pairs per thread, 768 threads): the couplings cx, cy, cz
+// do not pass through registers.  Step s starts direct global-to-LDS loads
+// of plane s+1's couplings into the LDS buffer that plane s-1 used, reads
+// plane s's from the other buffer, and waits for its loads only before the
+// step's barrier; b runs one plane ahead in registers.  The couplings of
+// the next plane thus stream in while a plane computes, with no VGPRs held
+// by the loads in flight (the register form without LOP can prefetch only
+// in the first sweep of a cycle, which reads no xin).
 // LDS: X[4][rb+2][np] (second-colour xin, plane k in slot k & 3),
-// R[2][rb][np] (first colour new).
+// R[2][rb][np] (first colour new), and under LOP two static buffers of
+// 3 rb + 1 coupling rows.
 // ---------------------------------------------------------------------
 template <int I> struct Ic { static constexpr int value = I; };
+
+// values per LDS coupling buffer of the LOP sweep (2 x 50 KiB), and its
+// threads per workgroup (3 wavefronts per SIMD leave 168 VGPRs a thread;
+// 1024 threads leave 128, and the LOP form spills there)
+constexpr int kLopCap = 12800;
+constexpr int kLopThreads = 768;
+
+// Workgroups go to the 8 XCDs round-robin by id (each XCD has its own L2).
+// Renumbers ids so that each XCD's workgroups are consecutive: neighbouring
+// tiles, which read each other's edge rows, then run on one XCD and find
+// those rows in its L2 (k_dq_tiled fetches 680 MB instead of 877 MB per
+// 256^3 launch).
+__device__ __forceinline__ int xcd_order(int b, int G) {
+    const int q = G >> 3, r = G & 7, x = b & 7, s = b >> 3;
+    return x < r ? x * (q + 1) + s : r * (q + 1) + (x - r) * q + s;
+}
 
 __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
@@ -202,8 +227,9 @@ __device__ __forceinline__ CellOp<T> pick(const PairOp<T> &P, int e) {
 // FINE does not change the code; it gives the finest level's launches
 // their own symbol so profilers can separate them from coarse levels.
 // DGI: the diagonal is the coupling sum (not read).
-template <class T, bool FIRST, int PX, bool FINE, bool DGI, bool PROL, bool PF, bool DOT>
-__global__ void __launch_bounds__(1024)
+template <class T, bool FIRST, int PX, bool FINE, bool DGI, bool PROL, bool PF, bool DOT,
+          bool LOP = false>
+__global__ void __launch_bounds__(LOP ? kLopThreads : 1024)
 k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *__restrict__ xout,
            int c1, int zc, int oy, int ytiles, const T *__restrict__ ec, double *__restrict__ part) {
     extern __shared__ unsigned char smem_raw[];
@@ -218,6 +244,14 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
     T *Rs = sm + 4 * (rb + 2) * np;         // [2][rb][np]
     auto X = [&](int k, int row, int m) -> T & { return Xs[((k & 3) * (rb + 2) + row) * np + m]; };
     auto R = [&](int k, int row, int m) -> T & { return Rs[((k & 1) * rb + row) * np + m]; };
+    // LOP: the couplings of planes s and s+1 in two LDS buffers, filled by
+    // direct global-to-LDS loads so that the next plane's couplings stream
+    // in while this plane computes (no registers held by loads in flight).
+    // Rows of nx values: cx in rows 0..rb-1, cz in rb..2rb-1, cy in
+    // 2rb..3rb (row 2rb: the row below thread row 0).  Two distinct arrays,
+    // so the compiler can tell reads of one from loads into the other.
+    __shared__ T lopA[LOP ? kLopCap : 1];
+    __shared__ T lopB[LOP ? kLopCap : 1];
     const int tile = blockIdx.x % ytiles, chunk = blockIdx.x / ytiles;
     const int y0 = tile * oy;
     const int k0 = chunk * zc;
@@ -233,6 +267,30 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
     // for ring threads, its own row (a duplicate load) otherwise
     const int jl = ring_lo ? wrapy(j - 1) : (ring_hi ? wrapy(j + 1) : j);
     auto col = [&](int jrow, int k, int m) { return 2LL * m + sy * jrow + sz * k; };
+    // one row of nx values from src into LDS row lrow of buf: each lane of a
+    // wavefront moves 4 consecutive values (tpr is a multiple of 64, so a
+    // wavefront lies in one thread row and its LDS base is uniform)
+    auto dma_row = [&](T *buf, int lrow, const T *src) {
+        const int t0 = threadIdx.x & ~63;
+        __builtin_amdgcn_global_load_lds(
+            (const void *)(src + 4 * threadIdx.x),
+            (__attribute__((address_space(3))) void *)(buf + lrow * nx + 4 * t0), 16, 0, 0);
+    };
+    auto dma_op = [&](T *buf, int k) {
+        const long long pk = sz * wrapz(k);
+        dma_row(buf, jj, L.cx + pk + sy * j);
+        dma_row(buf, rb + jj, L.cz + pk + sy * j);
+        dma_row(buf, 2 * rb + 1 + jj, L.cy + pk + sy * j);
+        if (jj == 0) dma_row(buf, 2 * rb, L.cy + pk + sy * jm);
+    };
+    auto lds_op = [&](const T *buf, int m, PairOp<T> &P) {
+        const int c = 2 * m, cw = m == 0 ? nx - 1 : c - 1;
+        const T *rx = buf + jj * nx, *rz = buf + (rb + jj) * nx, *ry = buf + (2 * rb + 1 + jj) * nx;
+        P.cxa = rx[c]; P.cxb = rx[c + 1]; P.cxw = rx[cw];
+        P.cya = ry[c]; P.cyb = ry[c + 1];
+        P.csa = ry[c - nx]; P.csb = ry[c - nx + 1];
+        P.cza = rz[c]; P.czb = rz[c + 1];
+    };
 
     auto load_op = [&](int k, int m, PairOp<T> &P) {
         const long long c = col(j, wrapz(k), m);
@@ -269,8 +327,12 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
     PairOp<T> op[2][PX];
     CellOp<T> kop[PX];
     T zr1[PX], zr2[PX], r_m2[PX], r_m1[PX], r_0[PX];
+    // xin run-ahead by one more plane (PF without LOP: LOP waits for all of
+    // a step's loads at its end anyway)
+    constexpr bool XRA = PF && !LOP;
     T xr[2][PX], er[2][PX], xq[2][PX], eq[2][PX];
     T bf1[PX];                       // DOT: first colour's b at plane s-1
+    T bq[2][PX][2];                  // LOP: b of the pair, planes s and s+1
     double sbx = 0.0, sx = 0.0, sb = 0.0;
     const int s0 = k0 - 1;
     const int klast = k1 + 1;   // last xin plane a step reads
@@ -280,14 +342,20 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
 #pragma unroll
     for (int p = 0; p < PX; ++p) {
         const int m = threadIdx.x + p * tpr;
-        if (PF) load_op(s0, m, op[0][p]);
+        if (LOP) {
+            const long long c = col(j, wrapz(s0), m);
+            bq[0][p][0] = b[c];
+            bq[0][p][1] = b[c + 1];
+        } else if (PF) {
+            load_op(s0, m, op[0][p]);
+        }
         if (!FIRST) {
             for (int q = s0 - 1; q <= s0 + 1; ++q) {
                 const T xv = ldx(j, q, m) + (PROL ? lde(j, q, m) : (T)0);
                 const T ev = ldx(jl, q, m) + (PROL ? lde(jl, q, m) : (T)0);
                 stage(q, m, xv, ev);
             }
-            if (PF) {
+            if (XRA) {
                 xr[0][p] = ldx(j, s0 + 2, m);
                 er[0][p] = ldx(jl, s0 + 2, m);
                 if (PROL) {
@@ -304,6 +372,10 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
         r_m2[p] = r_m1[p] = r_0[p] = (T)0;
         bf1[p] = (T)0;
     }
+    if constexpr (LOP) {
+        dma_op(lopA, s0);
+        __builtin_amdgcn_s_waitcnt(0);
+    }
     lds_barrier();
 
     auto step = [&](auto PC, int s) {
@@ -315,17 +387,25 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
         // requests: operator of plane s+1 and xin of plane s+3 (PF), or
         // operator of plane s and xin of plane s+2
         const int kn = PF ? min(s + 1, k1) : s;
-        const int qn = min(PF ? s + 3 : s + 2, klast);
+        const int qn = min(XRA ? s + 3 : s + 2, klast);
+        constexpr int XU = XRA ? PU : 0, XN = XRA ? PN : 0;
+        if constexpr (LOP) dma_op(PN ? lopB : lopA, kn);
 #pragma unroll
         for (int p = 0; p < PX; ++p) {
             const int m = threadIdx.x + p * tpr;
-            load_op(kn, m, op[PN][p]);
+            if constexpr (LOP) {
+                const long long c = col(j, wrapz(kn), m);
+                bq[PN][p][0] = b[c];
+                bq[PN][p][1] = b[c + 1];
+            } else {
+                load_op(kn, m, op[PN][p]);
+            }
             if (!FIRST) {
-                xr[PN][p] = ldx(j, qn, m);
-                er[PN][p] = ldx(jl, qn, m);
+                xr[XN][p] = ldx(j, qn, m);
+                er[XN][p] = ldx(jl, qn, m);
                 if (PROL) {
-                    xq[PN][p] = lde(j, qn, m);
-                    eq[PN][p] = lde(jl, qn, m);
+                    xq[XN][p] = lde(j, qn, m);
+                    eq[XN][p] = lde(jl, qn, m);
                 }
             }
         }
@@ -334,6 +414,11 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
 #pragma unroll
         for (int p = 0; p < PX; ++p) {
             const int m = threadIdx.x + p * tpr;
+            if constexpr (LOP) {
+                lds_op(PU ? lopB : lopA, m, op[PU][p]);
+                op[PU][p].ba = bq[PU][p][0];
+                op[PU][p].bb = bq[PU][p][1];
+            }
             const CellOp<T> o = pick(op[PU][p], e1);
             const T zm = kop[p].cz;
             zrs[p] = o.cz;
@@ -390,12 +475,15 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
         for (int p = 0; p < PX; ++p) {
             const int m = threadIdx.x + p * tpr;
             if (!FIRST)
-                stage(s + 2, m, xr[PU][p] + (PROL ? xq[PU][p] : (T)0),
-                      er[PU][p] + (PROL ? eq[PU][p] : (T)0));
+                stage(s + 2, m, xr[XU][p] + (PROL ? xq[XU][p] : (T)0),
+                      er[XU][p] + (PROL ? eq[XU][p] : (T)0));
             kop[p] = pick(op[PU][p], 1 - e1);
             zr2[p] = zr1[p];
             zr1[p] = zrs[p];
         }
+        // LOP: this wavefront's loads into the other buffer have landed; the
+        // barrier then publishes every wavefront's
+        if constexpr (LOP) __builtin_amdgcn_s_waitcnt(0);
         lds_barrier();
     };
     for (int s = s0;; s += 2) {
@@ -447,7 +535,7 @@ k_resid_restrict_tiled(LevelK<T> L, TGrid g, const T *__restrict__ b, const T *_
     __shared__ T Rz[16][64];
     const int nx = L.nx, ny = L.ny, nz = L.nz;
     const int cnx = nx >> 1, cny = ny >> 1;
-    const int bid = blockIdx.x;
+    const int bid = xcd_order(blockIdx.x, gridDim.x);
     const int tx = bid % g.tx, ty = (bid / g.tx) % g.ty, ch = bid / (g.tx * g.ty);
     const int li = threadIdx.x, lj = threadIdx.y;
     const int i = tx * 64 + li, j = ty * 16 + lj;
@@ -1203,7 +1291,7 @@ k_dq_tiled(OpK L, TGrid g, const TZ *__restrict__ z, const TZ *__restrict__ dold
            const double *__restrict__ S, double *__restrict__ part) {
     __shared__ double D[3][18][66];
     const int nx = L.nx, ny = L.ny, nz = L.nz;
-    const int bid = blockIdx.x;
+    const int bid = xcd_order(blockIdx.x, gridDim.x);
     const int tx = bid % g.tx, ty = (bid / g.tx) % g.ty, ch = bid / (g.tx * g.ty);
     const int li = threadIdx.x, lj = threadIdx.y;
     const int i = tx * 64 + li, j = ty * 16 + lj;
@@ -1760,16 +1848,19 @@ static void rb_pass(const Level<T> &L, int l, const T *b, int colour, bool first
 // 109 us against 124 us for one pair per thread).
 struct SweepShape {
     int px, tpr, rb, oy, zc, ytiles;
+    bool lop;   // couplings staged through LDS (k_rb_sweep<..., LOP>)
     dim3 grid, block;
     size_t lds;
 };
 template <class T>
-static SweepShape sweep_shape(int nx, int ny, int nz, int max_rows, int planes, int cus, bool dgi) {
+static SweepShape sweep_shape1(int nx, int ny, int nz, int max_rows, int planes, int cus, bool dgi,
+                               bool lop) {
     SweepShape sh;
     const int np = nx / 2;
     sh.px = (sizeof(T) == 4 && dgi && np % 128 == 0) ? 2 : 1;
+    sh.lop = lop && sh.px == 2;
     sh.tpr = np / sh.px;
-    int rb = 1024 / sh.tpr;
+    int rb = (sh.lop ? kLopThreads : 1024) / sh.tpr;
     if (max_rows > 0 && rb > max_rows) rb = max_rows;
     const int lds_rows = (int)((65536 / ((long long)np * sizeof(T)) - 8) / 6);
     if (rb > lds_rows) rb = lds_rows;
@@ -1796,6 +1887,18 @@ static SweepShape sweep_shape(int nx, int ny, int nz, int max_rows, int planes, 
     sh.block = dim3((unsigned)sh.tpr, (unsigned)(rb > 0 ? rb : 1), 1);
     sh.lds = sizeof(T) * (size_t)np * (4 * (rb + 2) + 2 * rb);
     return sh;
+}
+// The LOP form where its two coupling buffers (3 rb + 1 rows each) and the
+// X / R planes fit the 160 KiB of LDS of a workgroup, else the plain one.
+template <class T>
+static SweepShape sweep_shape(int nx, int ny, int nz, int max_rows, int planes, int cus, bool dgi) {
+    if (sizeof(T) == 4) {
+        const SweepShape sh = sweep_shape1<T>(nx, ny, nz, max_rows, planes, cus, dgi, true);
+        if (sh.lop && sh.rb >= 3 && (long long)(3 * sh.rb + 1) * nx <= kLopCap &&
+            sh.lds + 2 * sizeof(float) * kLopCap <= 160 * 1024)
+            return sh;
+    }
+    return sweep_shape1<T>(nx, ny, nz, max_rows, planes, cus, dgi, false);
 }
 
 // LDS bytes of a tail that starts at level lt (and its offsets in K)
@@ -1845,7 +1948,28 @@ static void sweep_launch2(const SweepShape &sh, const Level<T> &L, int l, const 
                           const T *ec, double *dot, hipStream_t s) {
     const LevelK<T> K = L.k();
     if constexpr (sizeof(T) == 4 && DGI) {
-        // two pairs per thread (no run-ahead loads)
+        // couplings staged through LDS (two pairs per thread)
+        if (sh.lop) {
+            if constexpr (!FIRST) {
+                if (l == 0 && dot) {
+                    k_rb_sweep<T, FIRST, 2, true, DGI, PROL, true, true, true>
+                        <<<sh.grid, sh.block, sh.lds, s>>>(K, b, L.x, L.xt, c1, sh.zc, sh.oy,
+                                                           sh.ytiles, ec, dot);
+                    return;
+                }
+            }
+            if (l == 0)
+                k_rb_sweep<T, FIRST, 2, true, DGI, PROL, true, false, true>
+                    <<<sh.grid, sh.block, sh.lds, s>>>(K, b, L.x, L.xt, c1, sh.zc, sh.oy,
+                                                       sh.ytiles, ec, nullptr);
+            else
+                k_rb_sweep<T, FIRST, 2, false, DGI, PROL, true, false, true>
+                    <<<sh.grid, sh.block, sh.lds, s>>>(K, b, L.x, L.xt, c1, sh.zc, sh.oy,
+                                                       sh.ytiles, ec, nullptr);
+            return;
+        }
+        // two pairs per thread, the LDS buffers not fitting (no run-ahead
+        // loads but in the first sweep)
         if (sh.px == 2) {
             if constexpr (!FIRST) {
                 if (l == 0 && dot) {
