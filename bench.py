@@ -212,6 +212,46 @@ def free_run(pps, torch, args):
                       "steps": args.steps, "warmup": args.warmup, "dtype": "f64"}), flush=True)
 
 
+def project_run(pps, torch, args):
+    """Mixed solve followed by project_velocity: the fused entry point (the
+    correction inside the last residual pass) against the solve alone and
+    against the solve plus the standalone projection call (which also
+    uploads the grid spacing on every call).  CUDA events on the current
+    stream; medians over the timed steps."""
+    import numpy as np
+    print(f"[bench] building {args.workload}", file=sys.stderr, flush=True)
+    grid, A, f = pps.make_problem(args.workload)
+    brms = float(torch.sqrt((A[7] ** 2).mean()))
+    tol = 1e-10 * brms
+    dt = 1.0
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    res = {"solve": [], "solve_then_project": [], "fused": []}
+    with pps.Solver(grid, method="mixed") as s:
+        p = torch.zeros(grid.shape_p, dtype=torch.float64, device="cuda")
+        for step in range(args.warmup + args.steps):
+            for mode in res:
+                p.zero_()
+                u = f["u"].clone()
+                torch.cuda.synchronize()
+                ev[0].record()
+                if mode == "fused":
+                    s.solve_project(A, p, f["u"], f["v"], f["w"], f["um"], f["vm"], f["wm"], f["rho"],
+                                    dt, tol, 1.2, 100)
+                else:
+                    s.solve(A, p, tol, 1.2, 100)
+                    if mode == "solve_then_project":
+                        pps.project_velocity(grid, u, f["v"].clone(), f["w"].clone(), f["um"],
+                                             f["vm"], f["wm"], f["rho"], p, dt)
+                ev[1].record()
+                torch.cuda.synchronize()
+                if step >= args.warmup:
+                    res[mode].append(ev[0].elapsed_time(ev[1]))
+    out = {k: round(float(np.median(v)), 3) for k, v in res.items()}
+    print(json.dumps({"mode": "solve_project", "workload": args.workload, "cells": grid.cells,
+                      "ms": out, "steps": args.steps, "warmup": args.warmup,
+                      "dtype": "f32 V-cycle / f64 refinement"}), flush=True)
+
+
 def autograd_run(pps, torch, args):
     """Forward (mixed solve from face values) + backward (adjoint solve and
     face kernel) of PressureSolve, J = 0.5 ||p - mean(p) - p_target||^2."""
@@ -280,6 +320,8 @@ def main():
     ap.add_argument("--free", type=int, default=0, metavar="ITERS",
                     help="time ITERS FreeSolver iterations (masked RB-SOR, solver_flag 1, the "
                          "droplets of --workload as gas cells) instead of the solve benchmark")
+    ap.add_argument("--project", action="store_true",
+                    help="time solve + project_velocity, fused against separate")
     ap.add_argument("--autograd", action="store_true",
                     help="time forward + backward of the differentiable solve on --workload "
                          "(mixed form) instead of the solve benchmark")
@@ -309,6 +351,9 @@ def main():
     pps = ge.load_package()
     if args.autograd:
         autograd_run(pps, torch, args)
+        return
+    if args.project:
+        project_run(pps, torch, args)
         return
     if args.free:
         free_run(pps, torch, args)

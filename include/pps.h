@@ -132,6 +132,19 @@ int pps_new_solver_mg(pps_handle *h, const double *A, double *p,
 int pps_solve_device(pps_handle *h, const double *dA, double *dp,
                      double maxError, double beta, int maxit, int norm,
                      int *it, double *tres2, void *stream);
+/* NewSolver followed by project_velocity, staggered branch (the main
+ * projection, paris.f90:499 then paris.f90:898-908): on return p holds the
+ * solve's result and u_out, v_out, w_out (padded like p, not aliasing u, v,
+ * w) the corrected velocities, ghosts copied from u, v, w.  The fast methods
+ * with norm 2 apply the correction inside their last residual pass. */
+int pps_solve_project_device(pps_handle *h, const double *dA, double *dp,
+                             double maxError, double beta, int maxit, int norm,
+                             int *it, double *tres2, const double *u,
+                             const double *v, const double *w,
+                             const double *umask, const double *vmask,
+                             const double *wmask, const double *rho, double dt,
+                             double *u_out, double *v_out, double *w_out,
+                             void *stream);
 
 /* Residual history of the last solve: tres2 after each outer iteration.
  * Returns the number of entries written (<= cap). */

@@ -140,6 +140,8 @@ def _lib():
         "pps_new_solver": (_I, [_P, _P, _P, _D, _D, _I, _IP, _IP, _I]),
         "pps_new_solver_mg": (_I, [_P, _P, _P, _D, _D, _I, _IP, _IP, _I, _DP]),
         "pps_solve_device": (_I, [_P, _P, _P, _D, _D, _I, _I, _IP, _DP, _P]),
+        "pps_solve_project_device": (_I, [_P, _P, _P, _D, _D, _I, _I, _IP, _DP, _P, _P, _P, _P, _P,
+                                          _P, _P, _D, _P, _P, _P, _P]),
         "pps_last_history": (_I, [_P, _DP, _I]),
         "pps_last_timing": (_I, [_P, _DP, _DP]),
         "pps_adjoint_device": (_I, [_P, _P, _P, _P, _D, _I, _IP, _P]),
@@ -175,7 +177,8 @@ def exported_symbols():
     """Names of the C-ABI functions declared in include/pps.h."""
     return [
         "pps_default_options", "pps_init", "pps_finalize", "pps_set_options",
-        "pps_new_solver", "pps_new_solver_mg", "pps_solve_device", "pps_last_history",
+        "pps_new_solver", "pps_new_solver_mg", "pps_solve_device", "pps_solve_project_device",
+        "pps_last_history",
         "pps_last_timing", "pps_adjoint_device", "pps_face_gradient_device",
         "pps_pressure_masks_device", "pps_setup_poisson_device", "pps_check_setup_device",
         "pps_remove_rhs_mean_device", "pps_calc_residuals_device",
@@ -350,6 +353,23 @@ class Solver:
         _check(rc, "pps_solve_device",
                allow=(OK, ERR_NOT_CONVERGED) if allow_maxit else (OK,))
         return it.value, t2.value
+
+    def solve_project(self, A, p, u, v, w, umask, vmask, wmask, rho, dt, maxError, beta=1.2,
+                      maxit=50, norm=2, stream=None, allow_maxit=True):
+        """Solve in place on p, then project_velocity (paris.f90:898-908) into
+        new tensors; returns (it, tres2, u_out, v_out, w_out).  The fast
+        methods apply the correction inside their last residual pass."""
+        import torch
+        _need_gpu(A, p, u, v, w, umask, vmask, wmask, rho)
+        uo, vo, wo = torch.empty_like(u), torch.empty_like(v), torch.empty_like(w)
+        it, t2 = ctypes.c_int(), ctypes.c_double()
+        rc = _lib().pps_solve_project_device(
+            self._h, _ptr(A), _ptr(p), maxError, beta, maxit, norm, ctypes.byref(it),
+            ctypes.byref(t2), _ptr(u), _ptr(v), _ptr(w), _ptr(umask), _ptr(vmask), _ptr(wmask),
+            _ptr(rho), dt, _ptr(uo), _ptr(vo), _ptr(wo), _stream(stream))
+        _check(rc, "pps_solve_project_device",
+               allow=(OK, ERR_NOT_CONVERGED) if allow_maxit else (OK,))
+        return it.value, t2.value, uo, vo, wo
 
     def adjoint(self, A, g, lam, rtol=1e-12, maxit=100, stream=None):
         """Solve M lam = g (g, lam padded device tensors); returns iterations."""

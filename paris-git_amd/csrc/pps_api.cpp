@@ -96,6 +96,7 @@ extern "C" int pps_finalize(pps_handle *h) {
     h->hp.release();
     h->hmask.release();
     h->hpgas.release();
+    h->spacing.release();
     h->red.release();
     if (h->pinned) (void)hipHostFree(h->pinned);
     for (auto &e : h->prof.ev)
@@ -115,7 +116,8 @@ extern "C" int pps_set_options(pps_handle *h, const pps_options *opt) {
 }
 
 static int solve_with(pps_handle *h, int method, const double *dA, double *dp, double maxError,
-                      double beta, int maxit, int norm, int *it, double *tres2, hipStream_t s) {
+                      double beta, int maxit, int norm, int *it, double *tres2, hipStream_t s,
+                      const ProjK *proj = nullptr) {
     if (!dA || !dp || !it || !tres2 || norm < 1) return PPS_ERR_BAD_ARG;
     (void)hipSetDevice(h->device);
     const Dims D = dims_of(h->grid);
@@ -135,7 +137,7 @@ static int solve_with(pps_handle *h, int method, const double *dA, double *dp, d
     if (method == PPS_MG_FP64 || method == PPS_MG_MIXED) {
         const int saved = h->opt.method;
         h->opt.method = method;
-        rc = fast_solve(h, dA, dp, maxError, maxit, norm, it, tres2, s);
+        rc = fast_solve(h, dA, dp, maxError, maxit, norm, it, tres2, s, false, proj);
         h->opt.method = saved;
     } else {
         PPS_HIP(hipEventRecord(h->ev[0], s));
@@ -157,6 +159,51 @@ extern "C" int pps_solve_device(pps_handle *h, const double *dA, double *dp, dou
     if (!h) return PPS_ERR_BAD_ARG;
     hipStream_t s = static_cast<hipStream_t>(stream);   // NULL: the default stream
     return solve_with(h, h->opt.method, dA, dp, maxError, beta, maxit, norm, it, tres2, s);
+}
+
+// NewSolver followed by project_velocity (paris.f90:499, 898-908).  The fast
+// methods with norm 2 apply the correction inside their last residual
+// pass; the others solve, copy and project.  The outputs are separate from
+// u, v, w so that a restart of the fast solve can rewrite them.
+extern "C" int pps_solve_project_device(pps_handle *h, const double *dA, double *dp,
+                                        double maxError, double beta, int maxit, int norm, int *it,
+                                        double *tres2, const double *u, const double *v,
+                                        const double *w, const double *umask, const double *vmask,
+                                        const double *wmask, const double *rho, double dt,
+                                        double *u_out, double *v_out, double *w_out, void *stream) {
+    if (!h || !u || !v || !w || !umask || !vmask || !wmask || !rho || !u_out || !v_out || !w_out)
+        return PPS_ERR_BAD_ARG;
+    if (u_out == u || v_out == v || w_out == w) return PPS_ERR_BAD_ARG;
+    (void)hipSetDevice(h->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const Dims D = dims_of(h->grid);
+    const int method = h->opt.method;
+    if ((method == PPS_MG_FP64 || method == PPS_MG_MIXED) && norm == 2 && maxit >= 1) {
+        ProjK P;
+        const double *dh[3];
+        PPS_TRY(handle_spacing(h, dh, s));
+        P.u = u; P.v = v; P.w = w;
+        P.um = umask; P.vm = vmask; P.wm = wmask; P.rho = rho;
+        P.dxh = dh[0]; P.dyh = dh[1]; P.dzh = dh[2];
+        P.uo = u_out; P.vo = v_out; P.wo = w_out;
+        P.dt = dt;
+        P.lx = h->grid.bdry_cond[0] != 1;
+        P.ly = h->grid.bdry_cond[1] != 1;
+        P.lz = h->grid.bdry_cond[2] != 1;
+        const double *const src[3] = {u, v, w};
+        double *const dst[3] = {u_out, v_out, w_out};
+        PPS_TRY(copy_ghosts(D, src, dst, s));
+        return solve_with(h, method, dA, dp, maxError, beta, maxit, norm, it, tres2, s, &P);
+    }
+    const int rc = solve_with(h, method, dA, dp, maxError, beta, maxit, norm, it, tres2, s);
+    if (rc != PPS_OK && rc != PPS_ERR_NOT_CONVERGED) return rc;
+    const size_t bytes = sizeof(double) * (size_t)D.padded();
+    PPS_HIP(hipMemcpyAsync(u_out, u, bytes, hipMemcpyDeviceToDevice, s));
+    PPS_HIP(hipMemcpyAsync(v_out, v, bytes, hipMemcpyDeviceToDevice, s));
+    PPS_HIP(hipMemcpyAsync(w_out, w, bytes, hipMemcpyDeviceToDevice, s));
+    PPS_TRY(pps_project_velocity_device(&h->grid, u_out, v_out, w_out, umask, vmask, wmask, rho,
+                                        dp, dt, stream));
+    return rc;
 }
 
 // host-array staging shared by NewSolver / NewSolverMG

@@ -623,6 +623,54 @@ extern "C" int pps_calc_residuals_device(const pps_grid *g, const double *A, con
     return PPS_OK;
 }
 
+namespace pps {
+
+// ghost layers of three padded arrays: dst = src (one thread a padded cell)
+__global__ void __launch_bounds__(256)
+k_copy_ghosts(DK d, int Z, const double *__restrict__ s0, const double *__restrict__ s1,
+              const double *__restrict__ s2, double *__restrict__ d0, double *__restrict__ d1,
+              double *__restrict__ d2) {
+    const long long n = (long long)d.X * d.Y * Z;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int i = (int)(t % d.X), j = (int)((t / d.X) % d.Y), k = (int)(t / ((long long)d.X * d.Y));
+    const bool ghost = i < d.ng || i >= d.nx + d.ng || j < d.ng || j >= d.ny + d.ng || k < d.ng ||
+                       k >= d.nz + d.ng;
+    if (ghost) {
+        d0[t] = s0[t];
+        d1[t] = s1[t];
+        d2[t] = s2[t];
+    }
+}
+
+int copy_ghosts(const Dims &d, const double *const src[3], double *const dst[3], hipStream_t s) {
+    const long long n = d.padded();
+    k_copy_ghosts<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(dk(d), d.Z(), src[0], src[1], src[2],
+                                                              dst[0], dst[1], dst[2]);
+    PPS_HIP(hipGetLastError());
+    return PPS_OK;
+}
+
+int handle_spacing(pps_handle *h, const double *dh[3], hipStream_t s) {
+    const pps_grid &g = h->grid;
+    const int nt[3] = {g.nx + 2 * g.ng, g.ny + 2 * g.ng, g.nz + 2 * g.ng};
+    if (!h->spacing.ptr) {
+        Coords co;
+        PPS_TRY(co.make(g, s));
+        PPS_TRY(h->spacing.ensure(co.buf.bytes));
+        PPS_HIP(hipMemcpyAsync(h->spacing.ptr, co.buf.ptr, co.buf.bytes, hipMemcpyDeviceToDevice, s));
+        PPS_HIP(hipStreamSynchronize(s));
+    }
+    // per axis: dx, dxh, x, each nt long (Coords layout)
+    const double *b = h->spacing.as<double>();
+    dh[0] = b + nt[0];
+    dh[1] = b + 3 * nt[0] + nt[1];
+    dh[2] = b + 3 * nt[0] + 3 * nt[1] + nt[2];
+    return PPS_OK;
+}
+
+}  // namespace pps
+
 extern "C" int pps_project_velocity_device(const pps_grid *g, double *u, double *v, double *w,
                                            const double *um, const double *vm, const double *wm,
                                            const double *rho, const double *p, double dt,

@@ -1610,13 +1610,25 @@ k_setup(PackK d, int zc, const double *__restrict__ A, const double *__restrict_
     block_partial(bb, part + kRedBlocks);
 }
 
+// project_velocity's face update (paris.f90:899-909), operand order and
+// rounding as the reference (no contraction)
+__device__ __forceinline__ double proj_face(double vel, double mask, double dh, double pe, double pc,
+                                            double re, double rc, double dt) {
+#pragma clang fp contract(off)
+    return vel - dt * (2.0 * mask / dh) * (pe - pc) / (re + rc);
+}
+
 // Final check fused with the write-back (z-marching): r = b - M x with the
 // FP64 operator, partial sum r^2, and p = x on the interior with the
-// first ghost layer wrapped in periodic directions.
-template <bool DGI>
+// first ghost layer wrapped in periodic directions.  PROJ: also the
+// velocity correction of project_velocity into P.uo / P.vo / P.wo, from the
+// iterate already in registers (the +x / +y neighbours of the stencil, the
+// +z one of the march; on a periodic axis the wrapped value is the ghost
+// the reference reads); faces left alone are copied.
+template <bool DGI, bool PROJ>
 __global__ void __launch_bounds__(256)
 k_check_unpack(OpK L, PackK d, int zc, const double *__restrict__ b, const double *__restrict__ x,
-               double *__restrict__ p, double *__restrict__ part) {
+               double *__restrict__ p, double *__restrict__ part, ProjK P) {
     const int i = blockIdx.x * 64 + threadIdx.x;
     const int j = blockIdx.y * 4 + threadIdx.y;
     const int k0 = blockIdx.z * zc;
@@ -1630,6 +1642,8 @@ k_check_unpack(OpK L, PackK d, int zc, const double *__restrict__ b, const doubl
         const long long col = i + sy * j;
         double xm = x[col + sz * wrap_m(k0, L.nz)];
         double x0 = x[col + sz * k0];
+        // PROJ: the cell's density, carried up the march
+        double rc = PROJ ? P.rho[(i + d.ng) + Py * (j + d.ng) + Pz * (k0 + d.ng)] : 0.0;
         for (int k = k0; k < k1; ++k) {
             const long long c = col + sz * k;
             const long long czm = col + sz * wrap_m(k, L.nz);
@@ -1643,6 +1657,19 @@ k_check_unpack(OpK L, PackK d, int zc, const double *__restrict__ b, const doubl
             const double v = b[c] - (outer_diag<DGI>(L, c, vxm, vx, vym, vy, vzm, vz) * x0 - s);
             rr += v * v;
             const long long q = (i + d.ng) + Py * (j + d.ng) + Pz * (k + d.ng);
+            if constexpr (PROJ) {
+                const double re = P.rho[q + 1], rn = P.rho[q + Py], rt = P.rho[q + Pz];
+                const double ui = P.u[q], vi = P.v[q], wi = P.w[q];
+                P.uo[q] = i < L.nx - P.lx
+                              ? proj_face(ui, P.um[q], P.dxh[i + d.ng], x[c + dxp], x0, re, rc, P.dt)
+                              : ui;
+                P.vo[q] = j < L.ny - P.ly
+                              ? proj_face(vi, P.vm[q], P.dyh[j + d.ng], x[c + dyp], x0, rn, rc, P.dt)
+                              : vi;
+                P.wo[q] = k < L.nz - P.lz ? proj_face(wi, P.wm[q], P.dzh[k + d.ng], xp, x0, rt, rc, P.dt)
+                                          : wi;
+                rc = rt;
+            }
             p[q] = x0;
             if (d.px && i == d.nx - 1) p[q - d.nx] = x0;
             if (d.px && i == 0) p[q + d.nx] = x0;
@@ -2366,7 +2393,8 @@ static int outer_iterate(pps_handle *h, Hier<T> &H, bool mixed, double tol_abs, 
 // relative: maxError bounds ||r||_2 / ||b||_2 (b the folded right-hand
 // side) instead of sqrt(sum r^2 / N)
 int fast_solve(pps_handle *h, const double *dA, double *dp, double maxError, int maxit, int norm,
-               int *it, double *tres2, hipStream_t s, bool relative) {
+               int *it, double *tres2, hipStream_t s, bool relative, const ProjK *proj) {
+    if (proj && norm != 2) return PPS_ERR_BAD_ARG;
     PPS_TRY(fast_ensure(h));
     FastState *f = h->fast;
     const bool mixed = h->opt.method == PPS_MG_MIXED;
@@ -2386,9 +2414,11 @@ int fast_solve(pps_handle *h, const double *dA, double *dp, double maxError, int
     double t2 = 0.0, last = 0.0;
     int rc = PPS_OK;
     // iterate on the recursive residual, then confirm with the true residual
-    // b - M x (norm 2: the FP64 operator, fused with the write-back of p;
-    // other norms: the reference's expression on the caller's A); restart
-    // from the true residual if the recursion drifted
+    // b - M x (norm 2: the FP64 operator, fused with the write-back of p and,
+    // given proj, with project_velocity into separate outputs, which a
+    // restart simply rewrites; other norms: the reference's expression on
+    // the caller's A); restart from the true residual if the recursion
+    // drifted
     for (int attempt = 0; attempt < 4 && total < maxit; ++attempt) {
         const int budget = maxit - total;
         if (mixed) rc = outer_iterate<float>(h, f->h32, true, maxError, budget, &k, &last, true, s);
@@ -2400,9 +2430,15 @@ int fast_solve(pps_handle *h, const double *dA, double *dp, double maxError, int
             const ZGrid zg = zgrid(d.nx, d.ny, d.nz);
             double *part = f->red.as<double>();
             double *S = part + 4 * kRedBlocks;
-            auto kcu = f->exact ? k_check_unpack<true> : k_check_unpack<false>;
-            kcu<<<zg.grid, dim3(64, 4, 1), 0, s>>>(opk(f), packk(d), zg.zc, vecs(f).b,
-                                                               vecs(f).x, dp, part);
+            if (proj) {
+                auto kcu = f->exact ? k_check_unpack<true, true> : k_check_unpack<false, true>;
+                kcu<<<zg.grid, dim3(64, 4, 1), 0, s>>>(opk(f), packk(d), zg.zc, vecs(f).b, vecs(f).x, dp,
+                                                       part, *proj);
+            } else {
+                auto kcu = f->exact ? k_check_unpack<true, false> : k_check_unpack<false, false>;
+                kcu<<<zg.grid, dim3(64, 4, 1), 0, s>>>(opk(f), packk(d), zg.zc, vecs(f).b, vecs(f).x, dp,
+                                                       part, ProjK{});
+            }
             k_scal_rr<<<1, 1024, 0, s>>>(part, zg.blocks, 0, S);
             PPS_HIP(hipGetLastError());
             PPS_HIP(hipMemcpyAsync(h->pinned, S + S_RR, sizeof(double), hipMemcpyDeviceToHost, s));

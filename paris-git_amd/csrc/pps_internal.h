@@ -124,6 +124,7 @@ struct pps_handle {
     // scratch device buffers shared by drivers (A/p staging for host calls)
     pps::DevBuf hA, hp;
     pps::DevBuf hmask, hpgas;      // pcmask / P_gas staging for host FreeSolver calls
+    pps::DevBuf spacing;           // dx, dxh, x per axis (handle_spacing)
     pps::DevBuf red;               // reduction partials + results
     double *pinned = nullptr;      // 16 pinned host doubles for scalar readback
     pps::Prof prof;                // fine-level smoother timing
@@ -140,9 +141,21 @@ int sor_solve(pps_handle *h, const double *dA, double *dp, double maxError,
               hipStream_t s);
 void ref_destroy(RefState *r);
 
+// project_velocity (paris.f90:898-908) applied inside the fast solve's last
+// residual pass: out = in - dt (2 mask / dh) (p_e - p_c) / (rho_e + rho_c) on
+// the faces between each cell and its +x / +y / +z neighbour, the last face
+// of an axis left as it is unless the axis is periodic (l* = 1).  Padded
+// arrays like p; dxh / dyh / dzh indexed like the reference (i + ng).
+struct ProjK {
+    const double *u, *v, *w, *um, *vm, *wm, *rho, *dxh, *dyh, *dzh;
+    double *uo, *vo, *wo;
+    double dt;
+    int lx, ly, lz;
+};
+
 int fast_solve(pps_handle *h, const double *dA, double *dp, double maxError,
                int maxit, int norm, int *it, double *tres2, hipStream_t s,
-               bool relative = false);
+               bool relative = false, const ProjK *proj = nullptr);
 int fast_adjoint(pps_handle *h, const double *dA, const double *dg,
                  double *dlam, double rtol, int maxit, int *it, hipStream_t s);
 void fast_destroy(FastState *f);
@@ -154,6 +167,11 @@ int check_setup(pps_handle *h, const Dims &d, const double *A, hipStream_t s);
 int compute_residual_norm(pps_handle *h, const Dims &d, const double *A,
                           const double *p, int norm, double *out_sum,
                           hipStream_t s);
+// dxh, dyh, dzh of the handle's grid on the device (built on first use).
+int handle_spacing(pps_handle *h, const double *dh[3], hipStream_t s);
+// dst[a] = src[a] on the ghost layers of three padded arrays (interiors
+// untouched).
+int copy_ghosts(const Dims &d, const double *const src[3], double *const dst[3], hipStream_t s);
 // {sum |r|, sum r^2, max |r|} (calcResiduals before normalisation).
 int fs_residual_stats(pps_handle *h, const Dims &d, const double *A, const double *p,
                       const int *pcmask, int flag, double out[4], hipStream_t s);

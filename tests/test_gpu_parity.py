@@ -341,3 +341,50 @@ def test_fortran_caller(pps):
         r = subprocess.run([exe, str(method)], capture_output=True, text=True, timeout=100)
         _say("  ", r.stdout.strip())
         assert r.returncode == 0 and "PASS" in r.stdout, r.stdout + r.stderr
+
+
+@pytest.mark.parametrize("method", ["mixed", "fp64", "reference"])
+@pytest.mark.parametrize("bc", [(0,) * 6, (1, 0, 0, 1, 0, 0)])
+def test_solve_project_matches_solve_then_project(pps, method, bc):
+    """pps_solve_project_device (the fast methods fuse project_velocity into
+    their last residual pass) equals pps_solve_device followed by
+    pps_project_velocity_device, bit for bit, ghosts included."""
+    _say("solve-project", method, bc)
+    g = pps.Grid.make(32, bdry_cond=bc)
+    drops = [(0.3, 0.5, 0.5, 0.2), (0.8, 0.4, 0.6, 0.15)]
+    rho, u, v, w = pps.synth_fields(g, drops, 1.0, 1000.0, 1.5, seed=5)
+    um, vm, wm = pps.pressure_masks(g)
+    dt = 0.5
+    A = pps.setup_poisson(g, u, v, w, um, vm, wm, rho, dt)
+    pps.remove_rhs_mean(g, A)
+    tol = 1e-10 * float(torch.sqrt((A[7] ** 2).mean()))
+    with pps.Solver(g, method=method) as s:
+        p1 = torch.zeros(g.shape_p, dtype=torch.float64, device="cuda")
+        it1, t1 = s.solve(A, p1, tol, 1.2, 60)
+        u1, v1, w1 = u.clone(), v.clone(), w.clone()
+        pps.project_velocity(g, u1, v1, w1, um, vm, wm, rho, p1, dt)
+        p2 = torch.zeros(g.shape_p, dtype=torch.float64, device="cuda")
+        u0 = u.clone()
+        it2, t2, u2, v2, w2 = s.solve_project(A, p2, u, v, w, um, vm, wm, rho, dt, tol, 1.2, 60)
+    assert it1 == it2 and t1 == t2
+    assert torch.equal(p1, p2)
+    assert torch.equal(u, u0)   # inputs untouched
+    for a, b in ((u1, u2), (v1, v2), (w1, w2)):
+        assert torch.equal(a, b)
+    # the correction did act (the face velocities changed)
+    assert not torch.equal(u1, u)
+
+
+def test_solve_project_rejects_aliasing(pps):
+    _say("solve-project aliasing")
+    g, A, f = pps.make_problem(n=16, drops=[(0.5, 0.5, 0.5, 0.25)], rho2=10.0)
+    um, vm, wm = pps.pressure_masks(g)
+    rho = torch.ones(g.shape_p, dtype=torch.float64, device="cuda")
+    u = torch.zeros(g.shape_p, dtype=torch.float64, device="cuda")
+    with pps.Solver(g, method="mixed") as s:
+        p = torch.zeros(g.shape_p, dtype=torch.float64, device="cuda")
+        rc = pps._lib().pps_solve_project_device(
+            s._h, pps._ptr(A), pps._ptr(p), 1e-8, 1.2, 10, 2, None, None, pps._ptr(u), pps._ptr(u),
+            pps._ptr(u), pps._ptr(um), pps._ptr(vm), pps._ptr(wm), pps._ptr(rho), 1.0, pps._ptr(u),
+            pps._ptr(u), pps._ptr(u), None)
+    assert rc == pps.ERR_BAD_ARG
