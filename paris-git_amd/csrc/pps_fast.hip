@@ -272,9 +272,21 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
     // wavefront lies in one thread row and its LDS base is uniform)
     auto dma_row = [&](T *buf, int lrow, const T *src) {
         const int t0 = threadIdx.x & ~63;
-        __builtin_amdgcn_global_load_lds(
-            (const void *)(src + 4 * threadIdx.x),
-            (__attribute__((address_space(3))) void *)(buf + lrow * nx + 4 * t0), 16, 0, 0);
+        if constexpr (PX == 2) {
+            __builtin_amdgcn_global_load_lds(
+                (const void *)(src + 4 * threadIdx.x),
+                (__attribute__((address_space(3))) void *)(buf + lrow * nx + 4 * t0), 16, 0, 0);
+        } else {
+            // one pair a thread: the wavefront's 128 values in two passes of
+            // one value a lane
+            const int l = threadIdx.x & 63;
+            __builtin_amdgcn_global_load_lds(
+                (const void *)(src + 2 * t0 + l),
+                (__attribute__((address_space(3))) void *)(buf + lrow * nx + 2 * t0), 4, 0, 0);
+            __builtin_amdgcn_global_load_lds(
+                (const void *)(src + 2 * t0 + 64 + l),
+                (__attribute__((address_space(3))) void *)(buf + lrow * nx + 2 * t0 + 64), 4, 0, 0);
+        }
     };
     auto dma_op = [&](T *buf, int k) {
         const long long pk = sz * wrapz(k);
@@ -1885,7 +1897,8 @@ static SweepShape sweep_shape1(int nx, int ny, int nz, int max_rows, int planes,
     SweepShape sh;
     const int np = nx / 2;
     sh.px = (sizeof(T) == 4 && dgi && np % 128 == 0) ? 2 : 1;
-    sh.lop = lop && sh.px == 2;
+    // LOP: a wavefront must lie in one thread row (tpr a multiple of 64)
+    sh.lop = lop && dgi && sizeof(T) == 4 && (sh.px == 2 || np % 64 == 0);
     sh.tpr = np / sh.px;
     int rb = (sh.lop ? kLopThreads : 1024) / sh.tpr;
     if (max_rows > 0 && rb > max_rows) rb = max_rows;
@@ -1970,29 +1983,32 @@ struct Cycle {
 // swap).  Algorithmic bytes: the operator planes (3 couplings, plus the
 // diagonal unless dgi), b, the old iterate (unless FIRST) and the new
 // iterate, once each.
+template <class T, bool FIRST, bool DGI, bool PROL, int PX>
+static void sweep_lop(const SweepShape &sh, const LevelK<T> &K, const Level<T> &L, int l, const T *b,
+                      int c1, const T *ec, double *dot, hipStream_t s) {
+    if constexpr (!FIRST) {
+        if (l == 0 && dot) {
+            k_rb_sweep<T, FIRST, PX, true, DGI, PROL, true, true, true><<<sh.grid, sh.block, sh.lds, s>>>(
+                K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec, dot);
+            return;
+        }
+    }
+    if (l == 0)
+        k_rb_sweep<T, FIRST, PX, true, DGI, PROL, true, false, true><<<sh.grid, sh.block, sh.lds, s>>>(
+            K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec, nullptr);
+    else
+        k_rb_sweep<T, FIRST, PX, false, DGI, PROL, true, false, true><<<sh.grid, sh.block, sh.lds, s>>>(
+            K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec, nullptr);
+}
 template <class T, bool FIRST, bool DGI, bool PROL, bool PF>
 static void sweep_launch2(const SweepShape &sh, const Level<T> &L, int l, const T *b, int c1,
                           const T *ec, double *dot, hipStream_t s) {
     const LevelK<T> K = L.k();
     if constexpr (sizeof(T) == 4 && DGI) {
-        // couplings staged through LDS (two pairs per thread)
+        // couplings staged through LDS
         if (sh.lop) {
-            if constexpr (!FIRST) {
-                if (l == 0 && dot) {
-                    k_rb_sweep<T, FIRST, 2, true, DGI, PROL, true, true, true>
-                        <<<sh.grid, sh.block, sh.lds, s>>>(K, b, L.x, L.xt, c1, sh.zc, sh.oy,
-                                                           sh.ytiles, ec, dot);
-                    return;
-                }
-            }
-            if (l == 0)
-                k_rb_sweep<T, FIRST, 2, true, DGI, PROL, true, false, true>
-                    <<<sh.grid, sh.block, sh.lds, s>>>(K, b, L.x, L.xt, c1, sh.zc, sh.oy,
-                                                       sh.ytiles, ec, nullptr);
-            else
-                k_rb_sweep<T, FIRST, 2, false, DGI, PROL, true, false, true>
-                    <<<sh.grid, sh.block, sh.lds, s>>>(K, b, L.x, L.xt, c1, sh.zc, sh.oy,
-                                                       sh.ytiles, ec, nullptr);
+            if (sh.px == 2) sweep_lop<T, FIRST, DGI, PROL, 2>(sh, K, L, l, b, c1, ec, dot, s);
+            else sweep_lop<T, FIRST, DGI, PROL, 1>(sh, K, L, l, b, c1, ec, dot, s);
             return;
         }
         // two pairs per thread, the LDS buffers not fitting (no run-ahead
