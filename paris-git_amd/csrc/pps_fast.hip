@@ -679,6 +679,15 @@ k_prolong_add(int fx, int fy, int fz, T *__restrict__ xf, const T *__restrict__ 
 // operator packing: 8-slot FP64 A + padded p -> couplings, diagonal,
 // folded right-hand side, interior iterate.  flags: bit0 NaN/tiny A7,
 // bit1 asymmetric, bit2 nonzero excess (system not singular).
+// The two couplings of a face differ when SetupPoisson's spacings do: on a
+// uniform grid whose step is not a power of two, dx(i) = xh(i) - xh(i-1)
+// varies in its last bits (paris.f90:2959-3102), so A2(i) and A1(i+1)
+// differ by an ulp or so.  Such a face counts as symmetric, and the stored
+// operator takes the + coupling A2(i) for both of its rows (a change of
+// relative size 1e-15, far below any solve tolerance).
+__device__ __forceinline__ bool asym(double a, double b) {
+    return !(fabs(a - b) <= 1e-12 * fmax(fabs(a), fabs(b)));
+}
 // ---------------------------------------------------------------------
 struct PackK {
     int nx, ny, nz, ng, X, Y;
@@ -730,18 +739,19 @@ k_pack(PackK d, const double *__restrict__ A, const double *__restrict__ p,
     } else {
         kept += a[4] + a[5];
     }
-    // symmetry: my + coupling equals the - coupling of my + neighbour
+    // symmetry: my + coupling equals the - coupling of my + neighbour (to
+    // rounding: see asym)
     if (!lastx || d.px) {
         const long long e = lastx ? c - (d.nx - 1) : c + 1;
-        if (A[e] != a[1]) f |= 2;
+        if (asym(A[e], a[1])) f |= 2;
     }
     if (!lasty || d.py) {
         const long long e = lasty ? c - (long long)(d.ny - 1) * sy : c + sy;
-        if (A[2 * n + e] != a[3]) f |= 2;
+        if (asym(A[2 * n + e], a[3])) f |= 2;
     }
     if (!lastz || d.pz) {
         const long long e = lastz ? c - (long long)(d.nz - 1) * sz : c + sz;
-        if (A[4 * n + e] != a[5]) f |= 2;
+        if (asym(A[4 * n + e], a[5])) f |= 2;
     }
     const double ex = a[6] - kept;
     if (fabs(ex) > 1e-12 * fabs(a[6])) f |= 4;
@@ -1586,13 +1596,13 @@ k_setup(PackK d, int zc, const double *__restrict__ A, const double *__restrict_
             } else {
                 kept += a[4] + a[5];
             }
-            if (!lastx || d.px) { if (A[c + ex_] != a[1]) f |= 2; }
-            if (!lasty || d.py) { if (A[2 * n + c + ey_] != a[3]) f |= 2; }
+            if (!lastx || d.px) { if (asym(A[c + ex_], a[1])) f |= 2; }
+            if (!lasty || d.py) { if (asym(A[2 * n + c + ey_], a[3])) f |= 2; }
             {
                 // the +z neighbour's -z coupling: the next plane's a[4]
                 const long long e = lastz ? c - (long long)(d.nz - 1) * sz : c + sz;
                 a4n = A[4 * n + e];
-                if ((!lastz || d.pz) && a4n != a[5]) f |= 2;
+                if ((!lastz || d.pz) && asym(a4n, a[5])) f |= 2;
             }
             if (fabs(a[6] - kept) > 1e-12 * fabs(a[6])) f |= 4;
             {

@@ -388,3 +388,53 @@ def test_solve_project_rejects_aliasing(pps):
             pps._ptr(u), pps._ptr(um), pps._ptr(vm), pps._ptr(wm), pps._ptr(rho), 1.0, pps._ptr(u),
             pps._ptr(u), pps._ptr(u), None)
     assert rc == pps.ERR_BAD_ARG
+
+
+@pytest.mark.parametrize("method", ["fp64", "mixed"])
+@pytest.mark.parametrize("shape,bc", [((96, 48, 80), (0,) * 6), ((48, 40, 24), (1, 0, 0, 1, 0, 0)),
+                                      ((80, 80, 80), (0, 1, 0, 0, 1, 0))])
+def test_fast_non_power_of_two(pps, orc, method, shape, bc):
+    """Sizes the reference's multigrid refuses (not powers of two, not
+    cubic): the fast methods coarsen while every size is even and solve the
+    coarsest level (<= 512 cells) exactly.  Pinned by the oracle's residual
+    of the result and by the two forms agreeing."""
+    nx, ny, nz = shape
+    _say("fast non-power-of-two", method, shape, bc)
+    g = pps.Grid.make(nx, ny, nz, bdry_cond=bc)
+    og = orc.OrcGrid.make(nx, ny, nz, bc=bc)
+    drops = [(0.3, 0.5, 0.5, 0.2), (0.7, 0.4, 0.6, 0.15)]
+    rho, u, v, w = pps.synth_fields(g, drops, 1.0, 1000.0, 1.5, seed=3)
+    um, vm, wm = pps.pressure_masks(g)
+    A = pps.setup_poisson(g, u, v, w, um, vm, wm, rho, 1.0)
+    pps.remove_rhs_mean(g, A)
+    brms = float(torch.sqrt((A[7] ** 2).mean()))
+    sols = {}
+    for m in ("fp64", method):
+        with pps.Solver(g, method=m) as s:
+            p = torch.zeros(g.shape_p, dtype=torch.float64, device="cuda")
+            it, t2 = s.solve(A, p, 1e-12 * brms, 1.2, 100, allow_maxit=False)
+            sols[m] = _mf(og.interior(_host(p)))
+            _say(f"   {m}: it={it} rel.res={t2 / brms:.2e}")
+            assert it <= 30
+            r = orc.residual_field(og, _host(A), _host(p))
+            # the claim (rms residual < 1e-12 brms) holds with the oracle's
+            # residual, up to a factor 2 for summation order
+            assert np.sqrt((r ** 2).mean()) < 2e-12 * brms
+    d = np.abs(sols[method] - sols["fp64"]).max() / np.abs(sols["fp64"]).max()
+    assert d < 1e-9
+
+
+def test_fast_refuses_sizes_without_small_coarsest(pps):
+    """50^3 halves once to 25^3 (15,625 cells), too large for the exact
+    coarsest solve: the fast methods report a bad size (the reference and
+    SOR methods still apply)."""
+    _say("fast bad size")
+    g = pps.Grid.make(50)
+    A = torch.zeros((8,) + (50, 50, 50), dtype=torch.float64, device="cuda")
+    A[0:6] = 1.0
+    A[6] = 6.0
+    p = torch.zeros(g.shape_p, dtype=torch.float64, device="cuda")
+    with pps.Solver(g, method="mixed", check_setup=False) as s:
+        with pytest.raises(pps.PPSError) as e:
+            s.solve(A, p, 1e-8, 1.2, 10)
+    assert e.value.code == pps.ERR_BAD_SIZE
