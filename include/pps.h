@@ -117,6 +117,16 @@ int pps_init(const pps_grid *grid, const pps_options *opt, pps_handle **out);
 int pps_finalize(pps_handle *h);
 /* Change the method / options of an existing handle. */
 int pps_set_options(pps_handle *h, const pps_options *opt);
+/* Cell widths of a stretched grid (the reference's dx(is:ie), dy(js:je),
+ * dz(ks:ke) of module_grid, set by xform or read_x: paris.f90:3056-3102),
+ * host arrays of nx, ny, nz positive values; all NULL clears them.  The fast
+ * methods then weight each row of A by its cell volume, which makes
+ * SetupPoisson's system (modules.f90:2737-2746) symmetric again:
+ * dx(i) A2(i) = dx(i+1) A1(i+1).  They still stop on the caller's
+ * (unweighted) residual.  Not used by the reference / SOR methods, which
+ * take any A; the adjoint refuses weighted rows. */
+int pps_set_cell_widths(pps_handle *h, const double *dx, const double *dy,
+                        const double *dz);
 
 /* NewSolver (newsolver.f90:34-57), host arrays.  Dispatches on
  * opt.multigrid like the reference; *ierr = 0 on success (the reference's

@@ -137,6 +137,7 @@ def _lib():
         "pps_init": (_I, [_GP, ctypes.POINTER(Options), ctypes.POINTER(_P)]),
         "pps_finalize": (_I, [_P]),
         "pps_set_options": (_I, [_P, ctypes.POINTER(Options)]),
+        "pps_set_cell_widths": (_I, [_P, _P, _P, _P]),
         "pps_new_solver": (_I, [_P, _P, _P, _D, _D, _I, _IP, _IP, _I]),
         "pps_new_solver_mg": (_I, [_P, _P, _P, _D, _D, _I, _IP, _IP, _I, _DP]),
         "pps_solve_device": (_I, [_P, _P, _P, _D, _D, _I, _I, _IP, _DP, _P]),
@@ -176,7 +177,7 @@ def _lib():
 def exported_symbols():
     """Names of the C-ABI functions declared in include/pps.h."""
     return [
-        "pps_default_options", "pps_init", "pps_finalize", "pps_set_options",
+        "pps_default_options", "pps_init", "pps_finalize", "pps_set_options", "pps_set_cell_widths",
         "pps_new_solver", "pps_new_solver_mg", "pps_solve_device", "pps_solve_project_device",
         "pps_last_history",
         "pps_last_timing", "pps_adjoint_device", "pps_face_gradient_device",
@@ -278,6 +279,16 @@ class Solver:
     def set_method(self, method):
         self.options.method = METHODS[method] if isinstance(method, str) else int(method)
         _check(_lib().pps_set_options(self._h, ctypes.byref(self.options)), "pps_set_options")
+
+    def set_cell_widths(self, dx=None, dy=None, dz=None):
+        """Cell widths of a stretched grid (numpy, nx / ny / nz values), or
+        None to clear: the fast methods then weight each row by its cell
+        volume, which symmetrises SetupPoisson's system."""
+        if dx is None and dy is None and dz is None:
+            _check(_lib().pps_set_cell_widths(self._h, None, None, None), "pps_set_cell_widths")
+            return
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (dx, dy, dz)]
+        _check(_lib().pps_set_cell_widths(self._h, *[_ptr(a) for a in arrs]), "pps_set_cell_widths")
 
     # -- reference signatures on host arrays ---------------------------
     def new_solver(self, A, p, maxError, beta=1.2, maxit=50, norm=2):

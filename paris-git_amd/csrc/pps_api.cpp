@@ -2,8 +2,10 @@
 // finalize_MG, MGsolver.f90:34-82), NewSolver / NewSolverMG
 // (newsolver.f90:34-57, MGsolver.f90:346-405) on host or device arrays,
 // and the adjoint entry point.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "pps_internal.h"
 
@@ -97,6 +99,7 @@ extern "C" int pps_finalize(pps_handle *h) {
     h->hmask.release();
     h->hpgas.release();
     h->spacing.release();
+    h->widths.release();
     h->red.release();
     if (h->pinned) (void)hipHostFree(h->pinned);
     for (auto &e : h->prof.ev)
@@ -204,6 +207,45 @@ extern "C" int pps_solve_project_device(pps_handle *h, const double *dA, double 
     PPS_TRY(pps_project_velocity_device(&h->grid, u_out, v_out, w_out, umask, vmask, wmask, rho,
                                         dp, dt, stream));
     return rc;
+}
+
+// Cell widths for the fast methods: the rows are weighted by the cell
+// volume (normalised), which symmetrises SetupPoisson's system on a
+// stretched grid (xform, read_x: paris.f90:3056-3102).  NULL clears them.
+extern "C" int pps_set_cell_widths(pps_handle *h, const double *dx, const double *dy,
+                                   const double *dz) {
+    if (!h) return PPS_ERR_BAD_ARG;
+    (void)hipSetDevice(h->device);
+    if (!dx && !dy && !dz) {
+        h->have_widths = 0;
+        h->wmin = 1.0;
+        fast_invalidate(h->fast);
+        return PPS_OK;
+    }
+    if (!dx || !dy || !dz) return PPS_ERR_BAD_ARG;
+    const int N[3] = {h->grid.nx, h->grid.ny, h->grid.nz};
+    const double *src[3] = {dx, dy, dz};
+    std::vector<double> w((size_t)N[0] + N[1] + N[2]);
+    double wmin = 1.0;
+    size_t o = 0;
+    for (int a = 0; a < 3; ++a) {
+        double mx = 0.0, mn = 0.0;
+        for (int q = 0; q < N[a]; ++q) {
+            const double v = src[a][q];
+            if (!(v > 0.0) || !std::isfinite(v)) return PPS_ERR_BAD_ARG;
+            mx = q == 0 ? v : std::max(mx, v);
+            mn = q == 0 ? v : std::min(mn, v);
+        }
+        for (int q = 0; q < N[a]; ++q) w[o + q] = src[a][q] / mx;
+        wmin *= mn / mx;
+        o += (size_t)N[a];
+    }
+    PPS_TRY(h->widths.ensure(sizeof(double) * w.size()));
+    PPS_HIP(hipMemcpy(h->widths.ptr, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice));
+    h->have_widths = 1;
+    h->wmin = wmin;
+    fast_invalidate(h->fast);   // rebuild the operator on the next solve
+    return PPS_OK;
 }
 
 // host-array staging shared by NewSolver / NewSolverMG

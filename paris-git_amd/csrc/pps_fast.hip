@@ -1538,12 +1538,19 @@ k_unpack(PackK d, const double *__restrict__ x, double *__restrict__ p) {
 // on the padded p, MGsolver.f90:97-100), and in mixed mode the FP32 copies
 // of the operator and of r.  Partials: 0 = sum r^2, 1 = sum b^2.  flags as
 // k_pack, plus bit3: some A7 differs from the coupling sum outer_diag forms.
+// W (wx, wy, wz not null): every row is first multiplied by the weight
+// w = wx[i] wy[j] wz[k] (normalised cell volumes), which makes the system
+// of a stretched grid symmetric: SetupPoisson's couplings of a face are
+// 2 dt mask / (dx dxh (rho + rho)) with the width dx of the row's own cell
+// (modules.f90:2737-2742), so dx(i) A2(i) = dx(i+1) A1(i+1).  b, r and the
+// operator are those of the weighted rows; the b^2 partial is unweighted.
 __global__ void __launch_bounds__(256)
 k_setup(PackK d, int zc, const double *__restrict__ A, const double *__restrict__ p,
         double *__restrict__ cx, double *__restrict__ cy, double *__restrict__ cz,
         double *__restrict__ dg, double *__restrict__ bo, double *__restrict__ x,
         double *__restrict__ r, float *__restrict__ f32, float *__restrict__ r32,
-        double *__restrict__ part, int *__restrict__ flags) {
+        double *__restrict__ part, int *__restrict__ flags, const double *__restrict__ wx,
+        const double *__restrict__ wy, const double *__restrict__ wz) {
     const int i = blockIdx.x * 64 + threadIdx.x;
     const int j = blockIdx.y * 4 + threadIdx.y;
     const int k0 = blockIdx.z * zc;
@@ -1572,6 +1579,17 @@ k_setup(PackK d, int zc, const double *__restrict__ A, const double *__restrict_
 #pragma unroll
             for (int s = 0; s < 8; ++s) nan |= a[s] != a[s];
             if (nan || a[6] < 1e-55) f |= 1;
+            // row weight and the weights of the +x, +y, +z neighbours
+            double wc = 1.0, wxn = 1.0, wyn = 1.0, wzn = 1.0;
+            if (wx) {
+                const double yz = wy[j] * wz[k];
+                wc = wx[i] * yz;
+                wxn = wx[lastx ? 0 : i + 1] * yz;
+                wyn = wx[i] * wy[lasty ? 0 : j + 1] * wz[k];
+                wzn = wx[i] * wy[j] * wz[k == d.nz - 1 ? 0 : k + 1];
+#pragma unroll
+                for (int s = 0; s < 8; ++s) a[s] *= wc;
+            }
             const double pw = p[q - 1], pe = p[q + 1], ps = p[q - Py], pn = p[q + Py], pt = p[q + Pz];
             const double rv = a[7] - a[6] * pc + a[0] * pw + a[1] * pe + a[2] * ps + a[3] * pn +
                               a[4] * pb + a[5] * pt;
@@ -1596,13 +1614,13 @@ k_setup(PackK d, int zc, const double *__restrict__ A, const double *__restrict_
             } else {
                 kept += a[4] + a[5];
             }
-            if (!lastx || d.px) { if (asym(A[c + ex_], a[1])) f |= 2; }
-            if (!lasty || d.py) { if (asym(A[2 * n + c + ey_], a[3])) f |= 2; }
+            if (!lastx || d.px) { if (asym(A[c + ex_] * wxn, a[1])) f |= 2; }
+            if (!lasty || d.py) { if (asym(A[2 * n + c + ey_] * wyn, a[3])) f |= 2; }
             {
                 // the +z neighbour's -z coupling: the next plane's a[4]
                 const long long e = lastz ? c - (long long)(d.nz - 1) * sz : c + sz;
                 a4n = A[4 * n + e];
-                if ((!lastz || d.pz) && asym(a4n, a[5])) f |= 2;
+                if ((!lastz || d.pz) && asym(a4n * wzn, a[5])) f |= 2;
             }
             if (fabs(a[6] - kept) > 1e-12 * fabs(a[6])) f |= 4;
             {
@@ -1622,7 +1640,7 @@ k_setup(PackK d, int zc, const double *__restrict__ A, const double *__restrict_
                 r32[c] = (float)rv;
             }
             rr += rv * rv;
-            bb += rhs * rhs;
+            bb += wx ? (rhs / wc) * (rhs / wc) : rhs * rhs;
             pb = pc;
             pc = pt;
         }
@@ -1650,7 +1668,8 @@ __device__ __forceinline__ double proj_face(double vel, double mask, double dh, 
 template <bool DGI, bool PROJ>
 __global__ void __launch_bounds__(256)
 k_check_unpack(OpK L, PackK d, int zc, const double *__restrict__ b, const double *__restrict__ x,
-               double *__restrict__ p, double *__restrict__ part, ProjK P) {
+               double *__restrict__ p, double *__restrict__ part, ProjK P, const double *__restrict__ wx,
+               const double *__restrict__ wy, const double *__restrict__ wz) {
     const int i = blockIdx.x * 64 + threadIdx.x;
     const int j = blockIdx.y * 4 + threadIdx.y;
     const int k0 = blockIdx.z * zc;
@@ -1676,7 +1695,8 @@ k_check_unpack(OpK L, PackK d, int zc, const double *__restrict__ b, const doubl
             double s = vxm * x[c + dxm] + vx * x[c + dxp];
             s += vym * x[c + dym] + vy * x[c + dyp];
             s += vzm * xm + vz * xp;
-            const double v = b[c] - (outer_diag<DGI>(L, c, vxm, vx, vym, vy, vzm, vz) * x0 - s);
+            double v = b[c] - (outer_diag<DGI>(L, c, vxm, vx, vym, vy, vzm, vz) * x0 - s);
+            if (wx) v /= wx[i] * wy[j] * wz[k];   // the caller's (unweighted) row
             rr += v * v;
             const long long q = (i + d.ng) + Py * (j + d.ng) + Pz * (k + d.ng);
             if constexpr (PROJ) {
@@ -1763,6 +1783,10 @@ struct FastState {
     size_t lds_max = 65536;  // LDS bytes per workgroup on this device
     int cus = 256;           // compute units on this device
 };
+
+void fast_invalidate(FastState *f) {
+    if (f) f->built_for = nullptr;
+}
 
 void fast_destroy(FastState *f) {
     if (!f) return;
@@ -2288,12 +2312,17 @@ static int fast_setup(pps_handle *h, const double *dA, const double *dp, bool mi
     double *S = part + 4 * kRedBlocks;
     const ZGrid zg = zgrid(d.nx, d.ny, d.nz);
     PPS_HIP(hipMemsetAsync(flags, 0, sizeof(int), s));
-    // mixed mode: the outer kernels read the operator from A itself
+    // mixed mode: the outer kernels read the operator from A itself, unless
+    // the rows are weighted (cell widths set)
     double *o64 = f->op.as<double>();
-    k_setup<<<zg.grid, dim3(64, 4, 1), 0, s>>>(packk(d), zg.zc, dA, dp, mixed ? nullptr : o64,
-                                                mixed ? nullptr : o64 + n, mixed ? nullptr : o64 + 2 * n,
-                                                mixed ? nullptr : o64 + 3 * n, V.b, V.x, V.r, f32, r32,
-                                                part, flags);
+    const double *w = h->widths.as<double>();
+    const double *wx = h->have_widths ? w : nullptr;
+    const double *wy = wx ? w + d.nx : nullptr, *wz = wx ? w + d.nx + d.ny : nullptr;
+    const bool own = !mixed || wx;
+    k_setup<<<zg.grid, dim3(64, 4, 1), 0, s>>>(packk(d), zg.zc, dA, dp, own ? o64 : nullptr,
+                                                own ? o64 + n : nullptr, own ? o64 + 2 * n : nullptr,
+                                                own ? o64 + 3 * n : nullptr, V.b, V.x, V.r, f32, r32,
+                                                part, flags, wx, wy, wz);
     k_scal_rr<<<1, 1024, 0, s>>>(part, zg.blocks, 1, S);
     PPS_HIP(hipGetLastError());
     int fl = 0;
@@ -2318,7 +2347,7 @@ static int fast_setup(pps_handle *h, const double *dA, const double *dp, bool mi
     }
     PPS_HIP(hipGetLastError());
     f->built_for = dA;
-    f->op_in_A = mixed ? 1 : 0;
+    f->op_in_A = own ? 0 : 1;
     return PPS_OK;
 }
 
@@ -2447,8 +2476,10 @@ int fast_solve(pps_handle *h, const double *dA, double *dp, double maxError, int
     // drifted
     for (int attempt = 0; attempt < 4 && total < maxit; ++attempt) {
         const int budget = maxit - total;
-        if (mixed) rc = outer_iterate<float>(h, f->h32, true, maxError, budget, &k, &last, true, s);
-        else rc = outer_iterate<double>(h, f->h64, false, maxError, budget, &k, &last, true, s);
+        // weighted rows: the recursion sees W r, and |r| <= |W r| / min w
+        const double tol_rec = h->have_widths ? maxError * h->wmin : maxError;
+        if (mixed) rc = outer_iterate<float>(h, f->h32, true, tol_rec, budget, &k, &last, true, s);
+        else rc = outer_iterate<double>(h, f->h64, false, tol_rec, budget, &k, &last, true, s);
         if (rc != PPS_OK && rc != PPS_ERR_NOT_CONVERGED) break;
         const bool exhausted = k > budget;
         total += exhausted ? budget : k;
@@ -2456,14 +2487,17 @@ int fast_solve(pps_handle *h, const double *dA, double *dp, double maxError, int
             const ZGrid zg = zgrid(d.nx, d.ny, d.nz);
             double *part = f->red.as<double>();
             double *S = part + 4 * kRedBlocks;
+            const double *w = h->widths.as<double>();
+            const double *wx = h->have_widths ? w : nullptr;
+            const double *wy = wx ? w + d.nx : nullptr, *wz = wx ? w + d.nx + d.ny : nullptr;
             if (proj) {
                 auto kcu = f->exact ? k_check_unpack<true, true> : k_check_unpack<false, true>;
                 kcu<<<zg.grid, dim3(64, 4, 1), 0, s>>>(opk(f), packk(d), zg.zc, vecs(f).b, vecs(f).x, dp,
-                                                       part, *proj);
+                                                       part, *proj, wx, wy, wz);
             } else {
                 auto kcu = f->exact ? k_check_unpack<true, false> : k_check_unpack<false, false>;
                 kcu<<<zg.grid, dim3(64, 4, 1), 0, s>>>(opk(f), packk(d), zg.zc, vecs(f).b, vecs(f).x, dp,
-                                                       part, ProjK{});
+                                                       part, ProjK{}, wx, wy, wz);
             }
             k_scal_rr<<<1, 1024, 0, s>>>(part, zg.blocks, 0, S);
             PPS_HIP(hipGetLastError());
@@ -2496,6 +2530,8 @@ int fast_solve(pps_handle *h, const double *dA, double *dp, double maxError, int
 
 int fast_adjoint(pps_handle *h, const double *dA, const double *dg, double *dlam, double rtol,
                  int maxit, int *it, hipStream_t s) {
+    // the adjoint of a row-weighted system is not the system itself
+    if (h->have_widths) return PPS_ERR_BAD_ARG;
     PPS_TRY(fast_ensure(h));
     FastState *f = h->fast;
     const bool mixed = h->opt.method == PPS_MG_MIXED;

@@ -125,6 +125,11 @@ struct pps_handle {
     pps::DevBuf hA, hp;
     pps::DevBuf hmask, hpgas;      // pcmask / P_gas staging for host FreeSolver calls
     pps::DevBuf spacing;           // dx, dxh, x per axis (handle_spacing)
+    // row weights of the fast methods (pps_set_cell_widths): wx | wy | wz on
+    // the device, normalised to a maximum of 1, and min wx * min wy * min wz
+    pps::DevBuf widths;
+    int have_widths = 0;
+    double wmin = 1.0;
     pps::DevBuf red;               // reduction partials + results
     double *pinned = nullptr;      // 16 pinned host doubles for scalar readback
     pps::Prof prof;                // fine-level smoother timing
@@ -159,6 +164,8 @@ int fast_solve(pps_handle *h, const double *dA, double *dp, double maxError,
 int fast_adjoint(pps_handle *h, const double *dA, const double *dg,
                  double *dlam, double rtol, int maxit, int *it, hipStream_t s);
 void fast_destroy(FastState *f);
+// forget the built operator (the next solve sets it up again)
+void fast_invalidate(FastState *f);
 
 // shared helpers (pps_assemble.hip); results are read back through the
 // handle's pinned scalars, so each call synchronises the stream.

@@ -99,6 +99,12 @@ module pps_paris
        integer(c_int), intent(out) :: num_iterations
      end function c_poi_solve
 
+     integer(c_int) function c_set_cell_widths(h, dx, dy, dz) bind(C, name="pps_set_cell_widths")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: dx(*), dy(*), dz(*)
+     end function c_set_cell_widths
+
      integer(c_int) function c_last_timing(h, solve_ms, setup_ms) bind(C, name="pps_last_timing")
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: h
@@ -109,7 +115,7 @@ module pps_paris
   type(c_ptr), save :: handle = c_null_ptr
 
   public :: pps_setup, pps_teardown, pps_newsolver, pps_newsolvermg, pps_freesolver, &
-            pps_poi_solve, pps_timing
+            pps_poi_solve, pps_timing, pps_cell_widths
 
 contains
 
@@ -202,6 +208,15 @@ contains
     rc = c_poi_solve(handle, A, p, maxError, maxit, num_iterations, HYPRESolverType)
     if (rc /= PPS_OK .and. rc /= PPS_ERR_NOT_CONVERGED) write(*,*) 'poi_solve: status ', rc
   end subroutine pps_poi_solve
+
+  ! Cell widths of a stretched grid, after pps_setup: the caller passes
+  ! module_grid's dx(is:ie), dy(js:je), dz(ks:ke) (paris.f90:3056-3102).
+  ! The fast methods then weight each row by its cell volume.
+  subroutine pps_cell_widths(dx, dy, dz, ierr)
+    real(8), intent(in) :: dx(*), dy(*), dz(*)
+    integer, intent(out) :: ierr
+    ierr = c_set_cell_widths(handle, dx, dy, dz)
+  end subroutine pps_cell_widths
 
   ! milliseconds of the last solve (whole, and the setup part)
   subroutine pps_timing(solve_ms, setup_ms)

@@ -1,0 +1,146 @@
+"""Stretched grids (xform, paris.f90:3056-3102).  SetupPoisson's system
+(modules.f90:2737-2746) is not symmetric there: a face's two couplings use
+the widths of the two cells, dx(i) A2(i) = dx(i+1) A1(i+1).  Given the cell
+widths, the fast methods weight each row by its cell volume.  The system is
+assembled on the host by a numpy restatement of SetupPoisson on the
+reference's stretched coordinates; the solution is pinned by the oracle's
+residual of the caller's (unweighted) system and by the two fast forms
+agreeing."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _host(t):
+    return np.ascontiguousarray(t.detach().cpu().numpy())
+
+
+def _axis(N, ng, L, form):
+    """x, dx, dxh of paris.f90:3056-3102 (Fortran index q = 1..N+2ng at
+    array index q-1)."""
+    nt = N + 2 * ng
+    s = (np.arange(1, nt + 1) - ng) / N
+    xh = L * (form * s * (0.5 - s) * (1.0 - s) + s)
+    x = np.empty(nt)
+    x[1:] = 0.5 * (xh[1:] + xh[:-1])
+    x[0] = 2.0 * xh[0] - x[1]
+    dxh = np.empty(nt)
+    dxh[:-1] = x[1:] - x[:-1]
+    dxh[-1] = dxh[-2]
+    dx = np.empty(nt)
+    dx[1:] = xh[1:] - xh[:-1]
+    dx[0] = dx[1]
+    return dx, dxh
+
+
+def _setup_poisson(g, form, rho, u, v, w, um, vm, wm, dt):
+    """modules.f90:2737-2746 on padded (z, y, x) arrays; walls only (codes
+    0, Poisson_BCs changes nothing)."""
+    ng, nx, ny, nz = g.ng, g.nx, g.ny, g.nz
+    dx, dxh = _axis(nx, ng, 1.0, form[0])
+    dy, dyh = _axis(ny, ng, 1.0, form[1])
+    dz, dzh = _axis(nz, ng, 1.0, form[2])
+    I, J, K = slice(ng, ng + nx), slice(ng, ng + ny), slice(ng, ng + nz)
+    Im, Jm, Km = slice(ng - 1, ng + nx - 1), slice(ng - 1, ng + ny - 1), slice(ng - 1, ng + nz - 1)
+    Ip, Jp, Kp = slice(ng + 1, ng + nx + 1), slice(ng + 1, ng + ny + 1), slice(ng + 1, ng + nz + 1)
+    r = rho
+    c = r[K, J, I]
+    A = np.empty((8, nz, ny, nx))
+    A[0] = 2 * dt * um[K, J, Im] / (dx[I] * dxh[Im] * (r[K, J, Im] + c))
+    A[1] = 2 * dt * um[K, J, I] / (dx[I] * dxh[I] * (r[K, J, Ip] + c))
+    A[2] = 2 * dt * vm[K, Jm, I] / (dy[J][:, None] * dyh[Jm][:, None] * (r[K, Jm, I] + c))
+    A[3] = 2 * dt * vm[K, J, I] / (dy[J][:, None] * dyh[J][:, None] * (r[K, Jp, I] + c))
+    A[4] = 2 * dt * wm[Km, J, I] / (dz[K][:, None, None] * dzh[Km][:, None, None] * (r[Km, J, I] + c))
+    A[5] = 2 * dt * wm[K, J, I] / (dz[K][:, None, None] * dzh[K][:, None, None] * (r[Kp, J, I] + c))
+    A[6] = ((((A[0] + A[1]) + A[2]) + A[3]) + A[4]) + A[5]
+    A[7] = -(((u[K, J, I] - u[K, J, Im]) / dx[I]
+              + (v[K, J, I] - v[K, Jm, I]) / dy[J][:, None])
+             + (w[K, J, I] - w[Km, J, I]) / dz[K][:, None, None])
+    return A, dx[I], dy[J], dz[K]
+
+
+def _problem(pps, n, form):
+    g = pps.Grid.make(*n)
+    drops = [(0.3, 0.5, 0.5, 0.2), (0.7, 0.4, 0.6, 0.15)]
+    rho, u, v, w = pps.synth_fields(g, drops, 1.0, 1000.0, 1.5, seed=6)
+    um, vm, wm = pps.pressure_masks(g)
+    A, dx, dy, dz = _setup_poisson(g, form, *[_host(t) for t in (rho, u, v, w, um, vm, wm)], 1.0)
+    # compatibility on the stretched grid: the volume-weighted mean of A8
+    vol = dz[:, None, None] * dy[None, :, None] * dx[None, None, :]
+    A[7] -= (vol * A[7]).sum() / vol.sum()
+    return g, np.ascontiguousarray(A), dx, dy, dz
+
+
+@pytest.mark.parametrize("n", [(32, 32, 32), (64, 32, 48)])
+def test_fast_solves_stretched_grid_with_cell_widths(pps, orc, n):
+    _say("stretched", n)
+    g, A, dx, dy, dz = _problem(pps, n, (0.8, 0.5, 0.6))
+    og = orc.OrcGrid.make(*n)
+    At = torch.from_numpy(A).cuda()
+    brms = np.sqrt((A[7] ** 2).mean())
+    sols = {}
+    for m in ("fp64", "mixed"):
+        with pps.Solver(g, method=m) as s:
+            p = torch.zeros(g.shape_p, dtype=torch.float64, device="cuda")
+            with pytest.raises(pps.PPSError) as e:
+                s.solve(At, p, 1e-10 * brms, 1.2, 100)
+            assert e.value.code == pps.ERR_NONSYMMETRIC
+            s.set_cell_widths(dx, dy, dz)
+            p.zero_()
+            it, t2 = s.solve(At, p, 1e-10 * brms, 1.2, 100, allow_maxit=False)
+            _say(f"   {m}: it={it} rel.res={t2 / brms:.2e}")
+            assert it <= 40
+            r = orc.residual_field(og, A, _host(p))
+            # the reported residual is the caller's unweighted one
+            assert np.sqrt((r ** 2).mean()) < 2e-10 * brms
+            assert abs(np.sqrt((r ** 2).mean()) - t2) <= 1e-3 * t2 + 1e-14 * brms
+            x = _host(g.interior(p))
+            sols[m] = x - x.mean()
+    d = np.abs(sols["mixed"] - sols["fp64"]).max() / np.abs(sols["fp64"]).max()
+    assert d < 1e-8
+
+
+def test_uniform_widths_change_nothing(pps):
+    """Equal widths normalise to weights of exactly 1: the solve is the
+    unweighted one bit for bit."""
+    _say("uniform widths")
+    g, A, f = pps.make_problem(n=32, drops=[(0.5, 0.5, 0.5, 0.25)], rho2=100.0)
+    brms = float(torch.sqrt((A[7] ** 2).mean()))
+    out = []
+    for widths in (None, np.full(32, 1.0 / 32)):
+        with pps.Solver(g, method="mixed") as s:
+            if widths is not None:
+                s.set_cell_widths(widths, widths, widths)
+            p = torch.zeros(g.shape_p, dtype=torch.float64, device="cuda")
+            it, t2 = s.solve(A, p, 1e-10 * brms, 1.2, 50)
+            out.append((it, t2, p.clone()))
+    assert out[0][0] == out[1][0] and out[0][1] == out[1][1]
+    assert torch.equal(out[0][2], out[1][2])
+
+
+def test_cell_widths_errors(pps):
+    _say("cell widths errors")
+    g, A, f = pps.make_problem(n=16, drops=[(0.5, 0.5, 0.5, 0.25)], rho2=10.0)
+    with pps.Solver(g, method="fp64") as s:
+        bad = np.full(16, 1.0 / 16)
+        bad[3] = 0.0
+        with pytest.raises(pps.PPSError) as e:
+            s.set_cell_widths(bad, bad, bad)
+        assert e.value.code == pps.ERR_BAD_ARG
+        s.set_cell_widths(np.full(16, 0.1), np.full(16, 0.1), np.full(16, 0.1))
+        lam = torch.zeros(g.shape_p, dtype=torch.float64, device="cuda")
+        with pytest.raises(pps.PPSError) as e:
+            s.adjoint(A, lam.clone(), lam)
+        assert e.value.code == pps.ERR_BAD_ARG
+        s.set_cell_widths()   # cleared: the unweighted solve again
+        brms = float(torch.sqrt((A[7] ** 2).mean()))
+        p = torch.zeros(g.shape_p, dtype=torch.float64, device="cuda")
+        it, t2 = s.solve(A, p, 1e-10 * brms, 1.2, 50, allow_maxit=False)
+        assert t2 < 1e-10 * brms
