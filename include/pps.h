@@ -238,6 +238,19 @@ int pps_setup_poisson_device(const pps_grid *grid, const double *u,
                              const double *umask, const double *vmask,
                              const double *wmask, const double *rho, double dt,
                              double volume_source, double *A, void *stream);
+/* The same on a stretched grid (xform, read_x: paris.f90:3056-3102): the
+ * caller's dx, dxh, dy, dyh, dz, dzh (host arrays with ghosts, n + 2 ng
+ * values each, the reference's dx(imin:imax) ...) replace the uniform
+ * spacings of the box lengths. */
+int pps_setup_poisson_coords_device(const pps_grid *grid, const double *u,
+                                    const double *v, const double *w,
+                                    const double *umask, const double *vmask,
+                                    const double *wmask, const double *rho,
+                                    double dt, double volume_source,
+                                    const double *dx, const double *dxh,
+                                    const double *dy, const double *dyh,
+                                    const double *dz, const double *dzh,
+                                    double *A, void *stream);
 /* check_poisson_setup (newsolver.f90:275-344); solid cells are those with
  * six zero couplings. */
 int pps_check_setup_device(const pps_grid *grid, const double *A, void *stream);

@@ -149,6 +149,8 @@ def _lib():
         "pps_face_gradient_device": (_I, [_GP, _P, _P, _P, _P, _P, _P, _P, _P]),
         "pps_pressure_masks_device": (_I, [_GP, _P, _P, _P, _P]),
         "pps_setup_poisson_device": (_I, [_GP, _P, _P, _P, _P, _P, _P, _P, _D, _D, _P, _P]),
+        "pps_setup_poisson_coords_device": (_I, [_GP, _P, _P, _P, _P, _P, _P, _P, _D, _D, _P, _P, _P,
+                                                 _P, _P, _P, _P, _P]),
         "pps_check_setup_device": (_I, [_GP, _P, _P]),
         "pps_remove_rhs_mean_device": (_I, [_GP, _P, _P]),
         "pps_calc_residuals_device": (_I, [_GP, _P, _P, _DP, _P]),
@@ -181,7 +183,8 @@ def exported_symbols():
         "pps_new_solver", "pps_new_solver_mg", "pps_solve_device", "pps_solve_project_device",
         "pps_last_history",
         "pps_last_timing", "pps_adjoint_device", "pps_face_gradient_device",
-        "pps_pressure_masks_device", "pps_setup_poisson_device", "pps_check_setup_device",
+        "pps_pressure_masks_device", "pps_setup_poisson_device", "pps_setup_poisson_coords_device",
+        "pps_check_setup_device",
         "pps_remove_rhs_mean_device", "pps_calc_residuals_device",
         "pps_project_velocity_device", "pps_synth_fields_device", "pps_synth_drops",
         "pps_coords", "pps_status_string", "pps_faces_to_A_device",
@@ -459,6 +462,21 @@ def setup_poisson(grid, u, v, w, um, vm, wm, rho, dt=1.0, volume_source=0.0):
                                            _ptr(um), _ptr(vm), _ptr(wm), _ptr(rho), dt,
                                            volume_source, _ptr(A), _stream()),
            "pps_setup_poisson_device")
+    return A
+
+
+def setup_poisson_coords(grid, u, v, w, um, vm, wm, rho, dx, dxh, dy, dyh, dz, dzh, dt=1.0,
+                         volume_source=0.0):
+    """SetupPoisson on a stretched grid: dx ... dzh are the reference's
+    spacing arrays with ghosts (numpy, n + 2 ng values each)."""
+    import torch
+    A = torch.empty(grid.shape_A, dtype=torch.float64, device=rho.device)
+    sp = [np.ascontiguousarray(a, dtype=np.float64) for a in (dx, dxh, dy, dyh, dz, dzh)]
+    _check(_lib().pps_setup_poisson_coords_device(ctypes.byref(grid), _ptr(u), _ptr(v), _ptr(w),
+                                                  _ptr(um), _ptr(vm), _ptr(wm), _ptr(rho), dt,
+                                                  volume_source, *[_ptr(a) for a in sp], _ptr(A),
+                                                  _stream()),
+           "pps_setup_poisson_coords_device")
     return A
 
 

@@ -565,22 +565,17 @@ extern "C" int pps_pressure_masks_device(const pps_grid *g, double *um, double *
     return PPS_OK;
 }
 
-extern "C" int pps_setup_poisson_device(const pps_grid *g, const double *u, const double *v,
-                                        const double *w, const double *um, const double *vm,
-                                        const double *wm, const double *rho, double dt,
-                                        double volume_source, double *A, void *stream) {
-    PPS_TRY(grid_ok(g));
-    for (int f = 0; f < 6; ++f)
-        if (g->bdry_cond[f] >= 5 || g->bdry_cond[f] < 0) return PPS_ERR_UNSUPPORTED;
+// ax / ay / az: {dx, dxh} of each axis on the device, padded (n + 2 ng each)
+static int setup_poisson_with(const pps_grid *g, const double *u, const double *v, const double *w,
+                              const double *um, const double *vm, const double *wm,
+                              const double *rho, double dt, double volume_source, const double *ax,
+                              const double *ay, const double *az, double *A, hipStream_t s) {
     const Dims d = dims_of(*g);
-    hipStream_t s = as_stream(stream);
-    Coords co;
-    PPS_TRY(co.make(*g, s));
     pps_handle *h = scratch_handle();
     PPS_TRY(ensure_red(h));
     PPS_HIP(hipMemsetAsync(red_flag(h), 0, sizeof(int), s));
-    k_setup_poisson<<<cgrid(d), kBlk, 0, s>>>(dk(d), u, v, w, um, vm, wm, rho, co.ax, co.ay, co.az,
-                                              dt, volume_source, g->bdry_cond[0], g->bdry_cond[1],
+    k_setup_poisson<<<cgrid(d), kBlk, 0, s>>>(dk(d), u, v, w, um, vm, wm, rho, ax, ay, az, dt,
+                                              volume_source, g->bdry_cond[0], g->bdry_cond[1],
                                               g->bdry_cond[2], g->bdry_cond[3], g->bdry_cond[4],
                                               g->bdry_cond[5], A, red_flag(h));
     PPS_HIP(hipGetLastError());
@@ -588,6 +583,56 @@ extern "C" int pps_setup_poisson_device(const pps_grid *g, const double *u, cons
     PPS_HIP(hipMemcpyAsync(&flag, red_flag(h), sizeof(int), hipMemcpyDeviceToHost, s));
     PPS_HIP(hipStreamSynchronize(s));
     return flag ? PPS_ERR_NAN_OR_TINY_DIAG : PPS_OK;
+}
+
+static int assembly_bc_ok(const pps_grid *g) {
+    for (int f = 0; f < 6; ++f)
+        if (g->bdry_cond[f] >= 5 || g->bdry_cond[f] < 0) return PPS_ERR_UNSUPPORTED;
+    return PPS_OK;
+}
+
+extern "C" int pps_setup_poisson_device(const pps_grid *g, const double *u, const double *v,
+                                        const double *w, const double *um, const double *vm,
+                                        const double *wm, const double *rho, double dt,
+                                        double volume_source, double *A, void *stream) {
+    PPS_TRY(grid_ok(g));
+    PPS_TRY(assembly_bc_ok(g));
+    hipStream_t s = as_stream(stream);
+    Coords co;
+    PPS_TRY(co.make(*g, s));
+    return setup_poisson_with(g, u, v, w, um, vm, wm, rho, dt, volume_source, co.ax, co.ay, co.az, A,
+                              s);
+}
+
+extern "C" int pps_setup_poisson_coords_device(const pps_grid *g, const double *u, const double *v,
+                                               const double *w, const double *um, const double *vm,
+                                               const double *wm, const double *rho, double dt,
+                                               double volume_source, const double *dx,
+                                               const double *dxh, const double *dy,
+                                               const double *dyh, const double *dz,
+                                               const double *dzh, double *A, void *stream) {
+    PPS_TRY(grid_ok(g));
+    PPS_TRY(assembly_bc_ok(g));
+    if (!dx || !dxh || !dy || !dyh || !dz || !dzh) return PPS_ERR_BAD_ARG;
+    hipStream_t s = as_stream(stream);
+    const int nt[3] = {g->nx + 2 * g->ng, g->ny + 2 * g->ng, g->nz + 2 * g->ng};
+    const double *src[3][2] = {{dx, dxh}, {dy, dyh}, {dz, dzh}};
+    std::vector<double> host(2 * ((size_t)nt[0] + nt[1] + nt[2]));
+    size_t off[3], o = 0;
+    for (int a = 0; a < 3; ++a) {
+        off[a] = o;
+        for (int h = 0; h < 2; ++h)
+            for (int q = 0; q < nt[a]; ++q) host[o + (size_t)h * nt[a] + q] = src[a][h][q];
+        o += 2 * (size_t)nt[a];
+    }
+    DevBuf buf;
+    PPS_TRY(buf.ensure(sizeof(double) * host.size()));
+    PPS_HIP(hipMemcpyAsync(buf.ptr, host.data(), sizeof(double) * host.size(), hipMemcpyHostToDevice, s));
+    const double *b = buf.as<double>();
+    const int rc = setup_poisson_with(g, u, v, w, um, vm, wm, rho, dt, volume_source, b + off[0],
+                                      b + off[1], b + off[2], A, s);
+    buf.release();
+    return rc;
 }
 
 extern "C" int pps_check_setup_device(const pps_grid *g, const double *A, void *stream) {

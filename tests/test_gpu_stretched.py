@@ -40,13 +40,15 @@ def _axis(N, ng, L, form):
     return dx, dxh
 
 
-def _setup_poisson(g, form, rho, u, v, w, um, vm, wm, dt):
+def _setup_poisson(g, form, rho, u, v, w, um, vm, wm, dt, spacings=None):
     """modules.f90:2737-2746 on padded (z, y, x) arrays; walls only (codes
     0, Poisson_BCs changes nothing)."""
     ng, nx, ny, nz = g.ng, g.nx, g.ny, g.nz
     dx, dxh = _axis(nx, ng, 1.0, form[0])
     dy, dyh = _axis(ny, ng, 1.0, form[1])
     dz, dzh = _axis(nz, ng, 1.0, form[2])
+    if spacings is not None:
+        spacings.extend([dx, dxh, dy, dyh, dz, dzh])
     I, J, K = slice(ng, ng + nx), slice(ng, ng + ny), slice(ng, ng + nz)
     Im, Jm, Km = slice(ng - 1, ng + nx - 1), slice(ng - 1, ng + ny - 1), slice(ng - 1, ng + nz - 1)
     Ip, Jp, Kp = slice(ng + 1, ng + nx + 1), slice(ng + 1, ng + ny + 1), slice(ng + 1, ng + nz + 1)
@@ -105,6 +107,21 @@ def test_fast_solves_stretched_grid_with_cell_widths(pps, orc, n):
             sols[m] = x - x.mean()
     d = np.abs(sols["mixed"] - sols["fp64"]).max() / np.abs(sols["fp64"]).max()
     assert d < 1e-8
+
+
+def test_stretched_assembly_on_device(pps):
+    """pps_setup_poisson_coords_device with the reference's stretched
+    spacings equals the numpy restatement of SetupPoisson bit for bit."""
+    _say("stretched assembly")
+    g = pps.Grid.make(24, 16, 20)
+    drops = [(0.3, 0.5, 0.5, 0.2)]
+    rho, u, v, w = pps.synth_fields(g, drops, 1.0, 1000.0, 1.5, seed=7)
+    um, vm, wm = pps.pressure_masks(g)
+    sp = []
+    A, *_ = _setup_poisson(g, (0.7, 0.4, 0.9), *[_host(t) for t in (rho, u, v, w, um, vm, wm)], 0.5,
+                           spacings=sp)
+    Ad = pps.setup_poisson_coords(g, u, v, w, um, vm, wm, rho, *sp, dt=0.5)
+    assert np.array_equal(_host(Ad), A)
 
 
 def test_uniform_widths_change_nothing(pps):
