@@ -100,6 +100,7 @@ extern "C" int pps_finalize(pps_handle *h) {
     h->hpgas.release();
     h->spacing.release();
     h->widths.release();
+    h->asplit.release();
     h->red.release();
     if (h->pinned) (void)hipHostFree(h->pinned);
     for (auto &e : h->prof.ev)

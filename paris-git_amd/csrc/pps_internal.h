@@ -125,6 +125,7 @@ struct pps_handle {
     pps::DevBuf hA, hp;
     pps::DevBuf hmask, hpgas;      // pcmask / P_gas staging for host FreeSolver calls
     pps::DevBuf spacing;           // dx, dxh, x per axis (handle_spacing)
+    pps::DevBuf asplit;            // colour-split A of the SOR sweeps (8n doubles)
     // row weights of the fast methods (pps_set_cell_widths): wx | wy | wz on
     // the device, normalised to a maximum of 1, and min wx * min wy * min wz
     pps::DevBuf widths;

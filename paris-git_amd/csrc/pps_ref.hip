@@ -64,6 +64,48 @@ k_rb_pass(LvlK L, const double *__restrict__ A, double *__restrict__ p,
     p[q] = (1.0 - beta) * p[q] + beta / A[6 * n + c] * s;
 }
 
+// Colour-split copy of the 8-slot A for the repeated sweeps of NewSolver_std
+// and FreeSolver (nx even): per colour (i+j+k)&1 and slot, the cells of that
+// colour in pair order h = (k ny + j) nx/2 + i/2.  A colour pass then
+// streams only its own half of A, where the interleaved layout fetches
+// every line of all 8 planes in both passes.  Values are copied, so the
+// updates stay bit for bit those of k_rb_pass.
+__global__ void __launch_bounds__(256)
+k_split_colours(LvlK L, const double *__restrict__ A, double *__restrict__ As) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= L.nx || j >= L.ny) return;
+    const long long n = lk_cells(L), hn = n / 2;
+    const long long c = icel(L, i, j, k);
+    const int colour = (i + j + k) & 1;
+    const long long h = ((long long)k * L.ny + j) * (L.nx / 2) + (i >> 1);
+    double *a = As + (long long)colour * 8 * hn;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) a[s * hn + h] = A[s * n + c];
+}
+
+// k_rb_pass on the colour-split A (the same expression)
+__global__ void __launch_bounds__(256)
+k_rb_pass_split(LvlK L, const double *__restrict__ As, double *__restrict__ p,
+                const int *__restrict__ pcmask, int flag, double beta, int colour) {
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = 2 * m + ((j + k + colour) & 1);
+    if (j >= L.ny || i >= L.nx) return;
+    const long long q = ipad(L, i, j, k);
+    if (pcmask && !(flag == 1 ? pcmask[q] == 0 : (pcmask[q] == 1 || pcmask[q] == 2))) return;
+    const long long hn = lk_cells(L) / 2;
+    const long long h = ((long long)k * L.ny + j) * (L.nx / 2) + m;
+    const double *a = As + (long long)colour * 8 * hn;
+    const long long sy = L.X, sz = (long long)L.X * L.Y;
+    const double s = a[h] * p[q - 1] + a[hn + h] * p[q + 1] + a[2 * hn + h] * p[q - sy] +
+                     a[3 * hn + h] * p[q + sy] + a[4 * hn + h] * p[q - sz] +
+                     a[5 * hn + h] * p[q + sz] + a[7 * hn + h];
+    p[q] = (1.0 - beta) * p[q] + beta / a[6 * hn + h] * s;
+}
+
 // One-layer periodic wrap of the face ghosts of a level.
 __global__ void __launch_bounds__(256)
 k_halo(LvlK L, double *__restrict__ p) {
@@ -263,6 +305,26 @@ static void launch_relax(const LvlK &L, const double *A, double *p, double beta,
     launch_halo(L, p, s);
     k_rb_pass<<<grid, kCellBlock, 0, s>>>(L, A, p, beta, 1);
 }
+// The caller's A split by colour into h->asplit when nx is even (else
+// nullptr: the interleaved kernels run).
+static const double *split_colours(pps_handle *h, const LvlK &L, const double *A, hipStream_t s,
+                                   int *rc) {
+    *rc = PPS_OK;
+    if (L.nx & 1) return nullptr;
+    const long long n = (long long)L.nx * L.ny * L.nz;
+    *rc = h->asplit.ensure(sizeof(double) * 8 * (size_t)n);
+    if (*rc != PPS_OK) return nullptr;
+    k_split_colours<<<cell_grid(L.nx, L.ny, L.nz), kCellBlock, 0, s>>>(L, A, h->asplit.as<double>());
+    return h->asplit.as<double>();
+}
+// one colour-split RB-SOR sweep: pass 0, halo, pass 1 (masked when pcmask)
+static void launch_relax_split(const LvlK &L, const double *As, double *p, const int *pcmask,
+                               int flag, double beta, hipStream_t s) {
+    dim3 grid((unsigned)((L.nx / 2 + 63) / 64), (unsigned)((L.ny + 3) / 4), (unsigned)L.nz);
+    k_rb_pass_split<<<grid, kCellBlock, 0, s>>>(L, As, p, pcmask, flag, beta, 0);
+    launch_halo(L, p, s);
+    k_rb_pass_split<<<grid, kCellBlock, 0, s>>>(L, As, p, pcmask, flag, beta, 1);
+}
 static void launch_residual(const LvlK &L, double *A, const double *p, hipStream_t s) {
     k_residual_inplace<<<cell_grid(L.nx, L.ny, L.nz), kCellBlock, 0, s>>>(L, A, p);
 }
@@ -361,9 +423,13 @@ int sor_solve(pps_handle *h, const double *dA, double *dp, double maxError,
     const double N = (double)g.nx * (double)g.ny * (double)g.nz;
     h->hist.clear();
     double t2 = 0.0, res = 0.0;
+    int rc = PPS_OK;
+    const double *As = split_colours(h, L, dA, s, &rc);
+    PPS_TRY(rc);
     int k;
     for (k = 1; k <= maxit; ++k) {
-        launch_relax(L, dA, dp, beta, s);
+        if (As) launch_relax_split(L, As, dp, nullptr, 0, beta, s);
+        else launch_relax(L, dA, dp, beta, s);
         launch_halo(L, dp, s);
         PPS_HIP(hipGetLastError());
         PPS_TRY(compute_residual_norm(h, D, dA, dp, norm, &res, s));
@@ -444,11 +510,18 @@ int free_solve(pps_handle *h, const double *dA, double *dp, const int *dmask, co
     launch_halo(L, dp, s);
     const dim3 grid((unsigned)(((L.nx + 1) / 2 + 63) / 64), (unsigned)((L.ny + 3) / 4), (unsigned)L.nz);
     double t2 = 0.0;
+    int rc = PPS_OK;
+    const double *As = split_colours(h, L, dA, s, &rc);
+    PPS_TRY(rc);
     int k;
     for (k = 1; k <= maxit; ++k) {
-        k_rb_pass_fs<<<grid, kCellBlock, 0, s>>>(L, dA, dp, dmask, solver_flag, beta, 0);
-        launch_halo(L, dp, s);
-        k_rb_pass_fs<<<grid, kCellBlock, 0, s>>>(L, dA, dp, dmask, solver_flag, beta, 1);
+        if (As) {
+            launch_relax_split(L, As, dp, dmask, solver_flag, beta, s);
+        } else {
+            k_rb_pass_fs<<<grid, kCellBlock, 0, s>>>(L, dA, dp, dmask, solver_flag, beta, 0);
+            launch_halo(L, dp, s);
+            k_rb_pass_fs<<<grid, kCellBlock, 0, s>>>(L, dA, dp, dmask, solver_flag, beta, 1);
+        }
         launch_halo(L, dp, s);
         PPS_HIP(hipGetLastError());
         double st[4];
