@@ -198,6 +198,15 @@ k_resid_stats_fs(DK d, const double *__restrict__ A, const double *__restrict__ 
     t = block_sum256(bad, sm); if (threadIdx.x == 0) part[3 * kRedBlocks + blockIdx.x] = t;
 }
 
+double *stats_partials(pps_handle *h) {
+    if (ensure_red(h) != PPS_OK) return nullptr;
+    return red_part(h);
+}
+int stats_finish(pps_handle *h, int nblocks, int ncomp, int maxmask, double *out, hipStream_t s) {
+    PPS_TRY(red_finish(h, nblocks, ncomp, maxmask, s));
+    for (int c = 0; c < ncomp; ++c) out[c] = h->pinned[c];
+    return PPS_OK;
+}
 int fs_residual_stats(pps_handle *h, const Dims &d, const double *A, const double *p,
                       const int *pcmask, int flag, double out[4], hipStream_t s) {
     PPS_TRY(ensure_red(h));

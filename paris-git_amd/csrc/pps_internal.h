@@ -180,6 +180,11 @@ int handle_spacing(pps_handle *h, const double *dh[3], hipStream_t s);
 // dst[a] = src[a] on the ghost layers of three padded arrays (interiors
 // untouched).
 int copy_ghosts(const Dims &d, const double *const src[3], double *const dst[3], hipStream_t s);
+// Block partials written by other kernels (comp c of block b at
+// part[c * kRedBlocks + b]), then their fixed-order totals (comps in
+// maxmask take the maximum) into out; synchronises.
+double *stats_partials(pps_handle *h);
+int stats_finish(pps_handle *h, int nblocks, int ncomp, int maxmask, double *out, hipStream_t s);
 // {sum |r|, sum r^2, max |r|} (calcResiduals before normalisation).
 int fs_residual_stats(pps_handle *h, const Dims &d, const double *A, const double *p,
                       const int *pcmask, int flag, double out[4], hipStream_t s);

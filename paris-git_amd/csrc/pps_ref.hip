@@ -13,6 +13,7 @@
 // ghosts are never read by the 7-point stencil.
 #pragma clang fp contract(off)
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -104,6 +105,83 @@ k_rb_pass_split(LvlK L, const double *__restrict__ As, double *__restrict__ p,
                      a[3 * hn + h] * p[q + sy] + a[4 * hn + h] * p[q - sz] +
                      a[5 * hn + h] * p[q + sz] + a[7 * hn + h];
     p[q] = (1.0 - beta) * p[q] + beta / a[6 * hn + h] * s;
+}
+
+// 256-thread block sum / max (t: linear thread id, wavefronts of 64
+// consecutive ids), result in thread 0 (sm: 4 doubles)
+__device__ __forceinline__ double blk_red(double v, double *sm, bool mx, int t) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double w = __shfl_down(v, o, 64);
+        v = mx ? fmax(v, w) : v + w;
+    }
+    if ((t & 63) == 0) sm[t >> 6] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (t == 0) {
+        r = sm[0];
+        for (int w = 1; w < 4; ++w) r = mx ? fmax(r, sm[w]) : r + sm[w];
+    }
+    __syncthreads();
+    return r;
+}
+
+// FreeSolver on the colour-split A, the residual statistics fused into the
+// sweep: the cells of one colour (64 x 4 pair tiles, the planes strided by
+// gridDim.z), with
+// UPDATE the RB-SOR update of k_rb_pass_split first.  Every neighbour of a
+// cell has the other colour, so after the second pass both the cells it
+// updated (UPDATE) and, in a pass of its own, the first colour's cells see
+// final values: their statistics equal k_resid_stats_fs's (same residual
+// expression, sums in another order).  Block partials {sum r^2, max |r|,
+// active cells, NaN cells} at part[c * kRedBlocks + boff + block].
+template <bool UPDATE>
+__global__ void __launch_bounds__(256)
+k_fs_pass_stats(LvlK L, const double *__restrict__ As, double *__restrict__ p,
+                const int *__restrict__ pcmask, int flag, double beta, int colour,
+                double *__restrict__ part, int boff) {
+    __shared__ double sm[4];
+    const long long hn = lk_cells(L) / 2;
+    const int hx = L.nx / 2;
+    const long long sy = L.X, sz = (long long)L.X * L.Y;
+    const double *a = As + (long long)colour * 8 * hn;
+    double s2 = 0.0, mx = 0.0, cnt = 0.0, bad = 0.0;
+    const int m = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+    for (int k = blockIdx.z; k < L.nz && m < hx && j < L.ny; k += gridDim.z) {
+        const long long h = ((long long)k * L.ny + j) * hx + m;
+        const int i = 2 * m + ((j + k + colour) & 1);
+        const long long q = ipad(L, i, j, k);
+        const int mk = pcmask[q];
+        if (!(flag == 1 ? mk == 0 : (mk == 1 || mk == 2))) continue;
+        double pc = p[q];
+        if (UPDATE) {
+            const double s = a[h] * p[q - 1] + a[hn + h] * p[q + 1] + a[2 * hn + h] * p[q - sy] +
+                             a[3 * hn + h] * p[q + sy] + a[4 * hn + h] * p[q - sz] +
+                             a[5 * hn + h] * p[q + sz] + a[7 * hn + h];
+            pc = (1.0 - beta) * pc + beta / a[6 * hn + h] * s;
+            p[q] = pc;
+        }
+        const double r = fabs(-pc * a[6 * hn + h] + a[h] * p[q - 1] + a[hn + h] * p[q + 1] +
+                              a[2 * hn + h] * p[q - sy] + a[3 * hn + h] * p[q + sy] +
+                              a[4 * hn + h] * p[q - sz] + a[5 * hn + h] * p[q + sz] + a[7 * hn + h]);
+        bool nan = r != r;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) nan |= a[s * hn + h] != a[s * hn + h];
+        s2 += r * r;
+        mx = fmax(mx, r);
+        cnt += 1.0;
+        if (nan) bad += 1.0;
+    }
+    const int t = threadIdx.x + 64 * threadIdx.y;
+    const long long b = boff + blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
+    double v = blk_red(s2, sm, false, t);
+    if (t == 0) part[b] = v;
+    v = blk_red(mx, sm, true, t);
+    if (t == 0) part[kRedBlocks + b] = v;
+    v = blk_red(cnt, sm, false, t);
+    if (t == 0) part[2 * kRedBlocks + b] = v;
+    v = blk_red(bad, sm, false, t);
+    if (t == 0) part[3 * kRedBlocks + b] = v;
 }
 
 // One-layer periodic wrap of the face ghosts of a level.
@@ -514,18 +592,36 @@ int free_solve(pps_handle *h, const double *dA, double *dp, const int *dmask, co
     const double *As = split_colours(h, L, dA, s, &rc);
     PPS_TRY(rc);
     int k;
+    // split path: pass 0, halo, pass 1 + its cells' statistics, halo, the
+    // statistics of pass 0's cells
+    // (the statistics kernels stride the planes so that each has at most
+    // ~2048 blocks, one partial slot each)
+    const dim3 sgrid((unsigned)((L.nx / 2 + 63) / 64), (unsigned)((L.ny + 3) / 4), (unsigned)L.nz);
+    const long long tiles = (long long)sgrid.x * sgrid.y;
+    const dim3 tgrid(sgrid.x, sgrid.y, (unsigned)std::max<long long>(1, std::min<long long>(L.nz, 2048 / tiles)));
+    const int nbk = (int)(tiles * tgrid.z);
+    if (As && 2LL * nbk > kRedBlocks) As = nullptr;   // too many row tiles: interleaved path
+    double *part = As ? stats_partials(h) : nullptr;
+    if (As && !part) return PPS_ERR_HIP;
     for (k = 1; k <= maxit; ++k) {
+        double st[4];
         if (As) {
-            launch_relax_split(L, As, dp, dmask, solver_flag, beta, s);
+            k_rb_pass_split<<<sgrid, kCellBlock, 0, s>>>(L, As, dp, dmask, solver_flag, beta, 0);
+            launch_halo(L, dp, s);
+            k_fs_pass_stats<true><<<tgrid, kCellBlock, 0, s>>>(L, As, dp, dmask, solver_flag, beta, 1, part, 0);
+            launch_halo(L, dp, s);
+            k_fs_pass_stats<false><<<tgrid, kCellBlock, 0, s>>>(L, As, dp, dmask, solver_flag, beta, 0, part,
+                                                                nbk);
+            PPS_HIP(hipGetLastError());
+            PPS_TRY(stats_finish(h, 2 * nbk, 4, 1 << 1, st, s));
         } else {
             k_rb_pass_fs<<<grid, kCellBlock, 0, s>>>(L, dA, dp, dmask, solver_flag, beta, 0);
             launch_halo(L, dp, s);
             k_rb_pass_fs<<<grid, kCellBlock, 0, s>>>(L, dA, dp, dmask, solver_flag, beta, 1);
+            launch_halo(L, dp, s);
+            PPS_HIP(hipGetLastError());
+            PPS_TRY(fs_residual_stats(h, D, dA, dp, dmask, solver_flag, st, s));
         }
-        launch_halo(L, dp, s);
-        PPS_HIP(hipGetLastError());
-        double st[4];
-        PPS_TRY(fs_residual_stats(h, D, dA, dp, dmask, solver_flag, st, s));
         if (st[3] > 0.0 || !(st[0] <= 1e18)) {
             *it = k;
             *tres2 = st[0];
