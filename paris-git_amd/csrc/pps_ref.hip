@@ -135,11 +135,13 @@ __device__ __forceinline__ double blk_red(double v, double *sm, bool mx, int t) 
 // final values: their statistics equal k_resid_stats_fs's (same residual
 // expression, sums in another order).  Block partials {sum r^2, max |r|,
 // active cells, NaN cells} at part[c * kRedBlocks + boff + block].
-template <bool UPDATE>
+// !FS (NewSolver_std): every cell, the residual of MGsolver.f90:97-100 as
+// compute_residual_norm forms it, partial 0 = sum |r|^norm.
+template <bool UPDATE, bool FS = true>
 __global__ void __launch_bounds__(256)
 k_fs_pass_stats(LvlK L, const double *__restrict__ As, double *__restrict__ p,
                 const int *__restrict__ pcmask, int flag, double beta, int colour,
-                double *__restrict__ part, int boff) {
+                double *__restrict__ part, int boff, int norm = 2) {
     __shared__ double sm[4];
     const long long hn = lk_cells(L) / 2;
     const int hx = L.nx / 2;
@@ -151,8 +153,10 @@ k_fs_pass_stats(LvlK L, const double *__restrict__ As, double *__restrict__ p,
         const long long h = ((long long)k * L.ny + j) * hx + m;
         const int i = 2 * m + ((j + k + colour) & 1);
         const long long q = ipad(L, i, j, k);
-        const int mk = pcmask[q];
-        if (!(flag == 1 ? mk == 0 : (mk == 1 || mk == 2))) continue;
+        if (FS) {
+            const int mk = pcmask[q];
+            if (!(flag == 1 ? mk == 0 : (mk == 1 || mk == 2))) continue;
+        }
         double pc = p[q];
         if (UPDATE) {
             const double s = a[h] * p[q - 1] + a[hn + h] * p[q + 1] + a[2 * hn + h] * p[q - sy] +
@@ -160,6 +164,14 @@ k_fs_pass_stats(LvlK L, const double *__restrict__ As, double *__restrict__ p,
                              a[5 * hn + h] * p[q + sz] + a[7 * hn + h];
             pc = (1.0 - beta) * pc + beta / a[6 * hn + h] * s;
             p[q] = pc;
+        }
+        if (!FS) {
+            const double r = fabs(a[7 * hn + h] - a[6 * hn + h] * pc + a[h] * p[q - 1] +
+                                  a[hn + h] * p[q + 1] + a[2 * hn + h] * p[q - sy] +
+                                  a[3 * hn + h] * p[q + sy] + a[4 * hn + h] * p[q - sz] +
+                                  a[5 * hn + h] * p[q + sz]);
+            s2 += norm == 1 ? r : (norm == 2 ? r * r : pow(r, (double)norm));
+            continue;
         }
         const double r = fabs(-pc * a[6 * hn + h] + a[h] * p[q - 1] + a[hn + h] * p[q + 1] +
                               a[2 * hn + h] * p[q - sy] + a[3 * hn + h] * p[q + sy] +
@@ -395,14 +407,6 @@ static const double *split_colours(pps_handle *h, const LvlK &L, const double *A
     k_split_colours<<<cell_grid(L.nx, L.ny, L.nz), kCellBlock, 0, s>>>(L, A, h->asplit.as<double>());
     return h->asplit.as<double>();
 }
-// one colour-split RB-SOR sweep: pass 0, halo, pass 1 (masked when pcmask)
-static void launch_relax_split(const LvlK &L, const double *As, double *p, const int *pcmask,
-                               int flag, double beta, hipStream_t s) {
-    dim3 grid((unsigned)((L.nx / 2 + 63) / 64), (unsigned)((L.ny + 3) / 4), (unsigned)L.nz);
-    k_rb_pass_split<<<grid, kCellBlock, 0, s>>>(L, As, p, pcmask, flag, beta, 0);
-    launch_halo(L, p, s);
-    k_rb_pass_split<<<grid, kCellBlock, 0, s>>>(L, As, p, pcmask, flag, beta, 1);
-}
 static void launch_residual(const LvlK &L, double *A, const double *p, hipStream_t s) {
     k_residual_inplace<<<cell_grid(L.nx, L.ny, L.nz), kCellBlock, 0, s>>>(L, A, p);
 }
@@ -504,13 +508,33 @@ int sor_solve(pps_handle *h, const double *dA, double *dp, double maxError,
     int rc = PPS_OK;
     const double *As = split_colours(h, L, dA, s, &rc);
     PPS_TRY(rc);
+    // split path: pass 0, halo, pass 1 + its cells' residuals, halo, the
+    // residuals of pass 0's cells (the planes strided: ~2048 blocks each)
+    const dim3 sgrid((unsigned)((L.nx / 2 + 63) / 64), (unsigned)((L.ny + 3) / 4), (unsigned)L.nz);
+    const long long tiles = (long long)sgrid.x * sgrid.y;
+    const dim3 tgrid(sgrid.x, sgrid.y, (unsigned)std::max<long long>(1, std::min<long long>(L.nz, 2048 / tiles)));
+    const int nbk = (int)(tiles * tgrid.z);
+    if (As && 2LL * nbk > kRedBlocks) As = nullptr;
+    double *part = As ? stats_partials(h) : nullptr;
+    if (As && !part) return PPS_ERR_HIP;
     int k;
     for (k = 1; k <= maxit; ++k) {
-        if (As) launch_relax_split(L, As, dp, nullptr, 0, beta, s);
-        else launch_relax(L, dA, dp, beta, s);
-        launch_halo(L, dp, s);
-        PPS_HIP(hipGetLastError());
-        PPS_TRY(compute_residual_norm(h, D, dA, dp, norm, &res, s));
+        if (As) {
+            k_rb_pass_split<<<sgrid, kCellBlock, 0, s>>>(L, As, dp, nullptr, 0, beta, 0);
+            launch_halo(L, dp, s);
+            k_fs_pass_stats<true, false><<<tgrid, kCellBlock, 0, s>>>(L, As, dp, nullptr, 0, beta, 1, part, 0,
+                                                                      norm);
+            launch_halo(L, dp, s);
+            k_fs_pass_stats<false, false><<<tgrid, kCellBlock, 0, s>>>(L, As, dp, nullptr, 0, beta, 0, part,
+                                                                       nbk, norm);
+            PPS_HIP(hipGetLastError());
+            PPS_TRY(stats_finish(h, 2 * nbk, 1, 0, &res, s));
+        } else {
+            launch_relax(L, dA, dp, beta, s);
+            launch_halo(L, dp, s);
+            PPS_HIP(hipGetLastError());
+            PPS_TRY(compute_residual_norm(h, D, dA, dp, norm, &res, s));
+        }
         double res2 = res / N;
         if (!(res2 <= 1e16)) {  // > 1e16 or NaN
             *it = k;
