@@ -1963,13 +1963,15 @@ static SweepShape sweep_shape1(int nx, int ny, int nz, int max_rows, int planes,
     return sh;
 }
 // The LOP form where its two coupling buffers (3 rb + 1 rows each) and the
-// X / R planes fit the 160 KiB of LDS of a workgroup, else the plain one.
+// X / R planes fit the device's LDS per workgroup (160 KiB on MI355X), else
+// the plain one.
 template <class T>
-static SweepShape sweep_shape(int nx, int ny, int nz, int max_rows, int planes, int cus, bool dgi) {
+static SweepShape sweep_shape(int nx, int ny, int nz, int max_rows, int planes, int cus, bool dgi,
+                              size_t lds_max) {
     if (sizeof(T) == 4) {
         const SweepShape sh = sweep_shape1<T>(nx, ny, nz, max_rows, planes, cus, dgi, true);
         if (sh.lop && sh.rb >= 3 && (long long)(3 * sh.rb + 1) * nx <= kLopCap &&
-            sh.lds + 2 * sizeof(float) * kLopCap <= 160 * 1024)
+            sh.lds + 2 * sizeof(float) * kLopCap <= lds_max)
             return sh;
     }
     return sweep_shape1<T>(nx, ny, nz, max_rows, planes, cus, dgi, false);
@@ -2098,7 +2100,7 @@ static void sweep_launch(const SweepShape &sh, const Level<T> &L, int l, const T
 // Whether level l's sweeps run as one fused kernel (else two colour passes).
 template <class T>
 static bool sweep_fused(const Level<T> &L, const Cycle &cy) {
-    const SweepShape sh = sweep_shape<T>(L.nx, L.ny, L.nz, cy.rows, cy.planes, cy.cus, cy.dgi);
+    const SweepShape sh = sweep_shape<T>(L.nx, L.ny, L.nz, cy.rows, cy.planes, cy.cus, cy.dgi, cy.lds_max);
     return cy.fused && !(sh.rb < 3 || sh.lds > 65536 || (L.nx & 1) || sh.tpr * sh.rb > 1024);
 }
 
@@ -2107,14 +2109,14 @@ static bool sweep_fused(const Level<T> &L, const Cycle &cy) {
 template <class T>
 static bool sweep_dot_ok(const Level<T> &L, const Cycle &cy) {
     if (!sweep_fused(L, cy)) return false;
-    const SweepShape sh = sweep_shape<T>(L.nx, L.ny, L.nz, cy.rows, cy.planes, cy.cus, cy.dgi);
+    const SweepShape sh = sweep_shape<T>(L.nx, L.ny, L.nz, cy.rows, cy.planes, cy.cus, cy.dgi, cy.lds_max);
     return (sh.tpr * sh.rb) % 64 == 0 && (long long)sh.grid.x <= kRedBlocks;
 }
 
 // Blocks of the fused sweep of level l (its partial sums under DOT).
 template <class T>
 static int sweep_blocks(const Level<T> &L, const Cycle &cy) {
-    return (int)sweep_shape<T>(L.nx, L.ny, L.nz, cy.rows, cy.planes, cy.cus, cy.dgi).grid.x;
+    return (int)sweep_shape<T>(L.nx, L.ny, L.nz, cy.rows, cy.planes, cy.cus, cy.dgi, cy.lds_max).grid.x;
 }
 
 // ec: the coarse correction to add to the iterate first (fused sweeps
@@ -2123,7 +2125,7 @@ template <class T>
 static void rb_sweep(Level<T> &L, int l, const T *b, int c1, bool first, const Cycle &cy, Prof *prof,
                      hipStream_t s, const T *ec = nullptr, double *dot = nullptr) {
     const bool dgi = cy.dgi;
-    const SweepShape sh = sweep_shape<T>(L.nx, L.ny, L.nz, cy.rows, cy.planes, cy.cus, cy.dgi);
+    const SweepShape sh = sweep_shape<T>(L.nx, L.ny, L.nz, cy.rows, cy.planes, cy.cus, cy.dgi, cy.lds_max);
     if (!sweep_fused(L, cy)) {
         // rows too long for the LDS planes: two colour passes instead
         rb_pass(L, l, b, c1, first, dgi, prof, s);
