@@ -181,8 +181,8 @@ int pps_free_solver_device(pps_handle *h, const double *dA, double *dp, const in
 
 /* poi_solve (modules.f90:2411-2593; called at paris.f90:490): the HYPRE
  * entry on the same 8-slot system (matrix A7 on the diagonal, -A1..-A6 off
- * it, right-hand side A8, modules.f90:2447-2469).  HYPRE is not part of
- * this build: every solver type (1 SMG, 2 PFMG, 3 GMRES, 4 BiCGSTAB) runs
+ * it, right-hand side A8, modules.f90:2447-2469).  HYPRE is not linked
+ * into this library: every solver type (1 SMG, 2 PFMG, 3 GMRES, 4 BiCGSTAB) runs
  * the handle's fast MG-PCG (PPS_MG_MIXED if the handle is mixed, else
  * PPS_MG_FP64) to ||A8 - M p||_2 / ||A8||_2 < maxError, the relative
  * tolerance HYPRE's SetTol gives.  p is the initial guess and the result;
