@@ -146,7 +146,9 @@ int pps_solve_device(pps_handle *h, const double *dA, double *dp,
  * projection, paris.f90:499 then paris.f90:898-908): on return p holds the
  * solve's result and u_out, v_out, w_out (padded like p, not aliasing u, v,
  * w) the corrected velocities, ghosts copied from u, v, w.  The fast methods
- * with norm 2 apply the correction inside their last residual pass. */
+ * with norm 2 apply the correction inside their last residual pass.  Uses
+ * the grid's uniform spacings: PPS_ERR_UNSUPPORTED once cell widths are set
+ * (stretched grids call pps_project_velocity_coords_device after the solve). */
 int pps_solve_project_device(pps_handle *h, const double *dA, double *dp,
                              double maxError, double beta, int maxit, int norm,
                              int *it, double *tres2, const double *u,
@@ -265,6 +267,17 @@ int pps_project_velocity_device(const pps_grid *grid, double *u, double *v,
                                 const double *vmask, const double *wmask,
                                 const double *rho, const double *p, double dt,
                                 void *stream);
+/* The same on a stretched grid: dx, dxh, dy, dyh, dz, dzh as for
+ * pps_setup_poisson_coords_device (the face corrections use dxh, dyh, dzh). */
+int pps_project_velocity_coords_device(const pps_grid *grid, double *u,
+                                       double *v, double *w,
+                                       const double *umask, const double *vmask,
+                                       const double *wmask, const double *rho,
+                                       const double *p, double dt,
+                                       const double *dx, const double *dxh,
+                                       const double *dy, const double *dyh,
+                                       const double *dz, const double *dzh,
+                                       void *stream);
 
 /* ---- synthetic inputs (benchmark / test fields, no reference code) ---- */
 /* Density from spherical droplets (tanh interface of width `width` cells,

@@ -155,6 +155,8 @@ def _lib():
         "pps_remove_rhs_mean_device": (_I, [_GP, _P, _P]),
         "pps_calc_residuals_device": (_I, [_GP, _P, _P, _DP, _P]),
         "pps_project_velocity_device": (_I, [_GP, _P, _P, _P, _P, _P, _P, _P, _P, _D, _P]),
+        "pps_project_velocity_coords_device": (_I, [_GP, _P, _P, _P, _P, _P, _P, _P, _P, _D, _P, _P, _P,
+                                                    _P, _P, _P, _P]),
         "pps_synth_fields_device": (_I, [_GP, _P, _I, _D, _D, _D, ctypes.c_ulonglong,
                                          _P, _P, _P, _P, _P]),
         "pps_synth_drops": (_I, [_GP, _I, _D, _D, ctypes.c_ulonglong, _P]),
@@ -186,7 +188,8 @@ def exported_symbols():
         "pps_pressure_masks_device", "pps_setup_poisson_device", "pps_setup_poisson_coords_device",
         "pps_check_setup_device",
         "pps_remove_rhs_mean_device", "pps_calc_residuals_device",
-        "pps_project_velocity_device", "pps_synth_fields_device", "pps_synth_drops",
+        "pps_project_velocity_device", "pps_project_velocity_coords_device",
+        "pps_synth_fields_device", "pps_synth_drops",
         "pps_coords", "pps_status_string", "pps_faces_to_A_device",
         "pps_profile_enable", "pps_profile_read", "pps_free_solver", "pps_free_solver_device",
         "pps_poi_solve", "pps_poi_solve_device",
@@ -503,6 +506,16 @@ def project_velocity(grid, u, v, w, um, vm, wm, rho, p, dt=1.0):
     _check(_lib().pps_project_velocity_device(ctypes.byref(grid), _ptr(u), _ptr(v), _ptr(w),
                                               _ptr(um), _ptr(vm), _ptr(wm), _ptr(rho), _ptr(p),
                                               dt, _stream()), "pps_project_velocity_device")
+
+
+def project_velocity_coords(grid, u, v, w, um, vm, wm, rho, p, dx, dxh, dy, dyh, dz, dzh, dt=1.0):
+    """project_velocity on a stretched grid (in place on u, v, w); dx ... dzh
+    are the reference's spacing arrays with ghosts (numpy)."""
+    sp = [np.ascontiguousarray(a, dtype=np.float64) for a in (dx, dxh, dy, dyh, dz, dzh)]
+    _check(_lib().pps_project_velocity_coords_device(ctypes.byref(grid), _ptr(u), _ptr(v), _ptr(w),
+                                                     _ptr(um), _ptr(vm), _ptr(wm), _ptr(rho), _ptr(p),
+                                                     dt, *[_ptr(a) for a in sp], _stream()),
+           "pps_project_velocity_coords_device")
 
 
 def face_weights(grid, dt=1.0, device="cuda"):

@@ -178,6 +178,9 @@ extern "C" int pps_solve_project_device(pps_handle *h, const double *dA, double 
     if (!h || !u || !v || !w || !umask || !vmask || !wmask || !rho || !u_out || !v_out || !w_out)
         return PPS_ERR_BAD_ARG;
     if (u_out == u || v_out == v || w_out == w) return PPS_ERR_BAD_ARG;
+    // the correction uses the grid's uniform spacings; a stretched grid
+    // (cell widths set) solves, then calls pps_project_velocity_coords_device
+    if (h->have_widths) return PPS_ERR_UNSUPPORTED;
     (void)hipSetDevice(h->device);
     hipStream_t s = static_cast<hipStream_t>(stream);
     const Dims D = dims_of(h->grid);

@@ -574,6 +574,28 @@ extern "C" int pps_pressure_masks_device(const pps_grid *g, double *um, double *
     return PPS_OK;
 }
 
+// {dx, dxh} per axis from the caller's padded host arrays into one device
+// buffer; ax / ay / az point into it
+static int upload_spacings(const pps_grid *g, const double *dx, const double *dxh,
+                           const double *dy, const double *dyh, const double *dz,
+                           const double *dzh, DevBuf &buf, const double *ax[3], hipStream_t s) {
+    const int nt[3] = {g->nx + 2 * g->ng, g->ny + 2 * g->ng, g->nz + 2 * g->ng};
+    const double *src[3][2] = {{dx, dxh}, {dy, dyh}, {dz, dzh}};
+    std::vector<double> host(2 * ((size_t)nt[0] + nt[1] + nt[2]));
+    size_t off[3], o = 0;
+    for (int a = 0; a < 3; ++a) {
+        off[a] = o;
+        for (int h = 0; h < 2; ++h)
+            for (int q = 0; q < nt[a]; ++q) host[o + (size_t)h * nt[a] + q] = src[a][h][q];
+        o += 2 * (size_t)nt[a];
+    }
+    PPS_TRY(buf.ensure(sizeof(double) * host.size()));
+    PPS_HIP(hipMemcpyAsync(buf.ptr, host.data(), sizeof(double) * host.size(), hipMemcpyHostToDevice, s));
+    PPS_HIP(hipStreamSynchronize(s));
+    for (int a = 0; a < 3; ++a) ax[a] = buf.as<double>() + off[a];
+    return PPS_OK;
+}
+
 // ax / ay / az: {dx, dxh} of each axis on the device, padded (n + 2 ng each)
 static int setup_poisson_with(const pps_grid *g, const double *u, const double *v, const double *w,
                               const double *um, const double *vm, const double *wm,
@@ -624,22 +646,12 @@ extern "C" int pps_setup_poisson_coords_device(const pps_grid *g, const double *
     PPS_TRY(assembly_bc_ok(g));
     if (!dx || !dxh || !dy || !dyh || !dz || !dzh) return PPS_ERR_BAD_ARG;
     hipStream_t s = as_stream(stream);
-    const int nt[3] = {g->nx + 2 * g->ng, g->ny + 2 * g->ng, g->nz + 2 * g->ng};
-    const double *src[3][2] = {{dx, dxh}, {dy, dyh}, {dz, dzh}};
-    std::vector<double> host(2 * ((size_t)nt[0] + nt[1] + nt[2]));
-    size_t off[3], o = 0;
-    for (int a = 0; a < 3; ++a) {
-        off[a] = o;
-        for (int h = 0; h < 2; ++h)
-            for (int q = 0; q < nt[a]; ++q) host[o + (size_t)h * nt[a] + q] = src[a][h][q];
-        o += 2 * (size_t)nt[a];
-    }
     DevBuf buf;
-    PPS_TRY(buf.ensure(sizeof(double) * host.size()));
-    PPS_HIP(hipMemcpyAsync(buf.ptr, host.data(), sizeof(double) * host.size(), hipMemcpyHostToDevice, s));
-    const double *b = buf.as<double>();
-    const int rc = setup_poisson_with(g, u, v, w, um, vm, wm, rho, dt, volume_source, b + off[0],
-                                      b + off[1], b + off[2], A, s);
+    const double *ax[3];
+    int rc = upload_spacings(g, dx, dxh, dy, dyh, dz, dzh, buf, ax, s);
+    if (rc == PPS_OK)
+        rc = setup_poisson_with(g, u, v, w, um, vm, wm, rho, dt, volume_source, ax[0], ax[1], ax[2], A,
+                                s);
     buf.release();
     return rc;
 }
@@ -725,17 +737,14 @@ int handle_spacing(pps_handle *h, const double *dh[3], hipStream_t s) {
 
 }  // namespace pps
 
-extern "C" int pps_project_velocity_device(const pps_grid *g, double *u, double *v, double *w,
-                                           const double *um, const double *vm, const double *wm,
-                                           const double *rho, const double *p, double dt,
-                                           void *stream) {
-    PPS_TRY(grid_ok(g));
+// ax / ay / az: {dx, dxh} of each axis on the device, padded
+static int project_with(const pps_grid *g, double *u, double *v, double *w, const double *um,
+                        const double *vm, const double *wm, const double *rho, const double *p,
+                        double dt, const double *ax, const double *ay, const double *az,
+                        hipStream_t s) {
     const Dims d = dims_of(*g);
-    hipStream_t s = as_stream(stream);
-    Coords co;
-    PPS_TRY(co.make(*g, s));
     const int nt[3] = {g->nx + 2 * g->ng, g->ny + 2 * g->ng, g->nz + 2 * g->ng};
-    const double *dh[3] = {co.ax + nt[0], co.ay + nt[1], co.az + nt[2]};
+    const double *dh[3] = {ax + nt[0], ay + nt[1], az + nt[2]};
     double *vel[3] = {u, v, w};
     const double *mk[3] = {um, vm, wm};
     for (int c = 0; c < 3; ++c) {
@@ -745,6 +754,36 @@ extern "C" int pps_project_velocity_device(const pps_grid *g, double *u, double 
     PPS_HIP(hipGetLastError());
     PPS_HIP(hipStreamSynchronize(s));
     return PPS_OK;
+}
+
+extern "C" int pps_project_velocity_device(const pps_grid *g, double *u, double *v, double *w,
+                                           const double *um, const double *vm, const double *wm,
+                                           const double *rho, const double *p, double dt,
+                                           void *stream) {
+    PPS_TRY(grid_ok(g));
+    hipStream_t s = as_stream(stream);
+    Coords co;
+    PPS_TRY(co.make(*g, s));
+    return project_with(g, u, v, w, um, vm, wm, rho, p, dt, co.ax, co.ay, co.az, s);
+}
+
+
+extern "C" int pps_project_velocity_coords_device(const pps_grid *g, double *u, double *v, double *w,
+                                                  const double *um, const double *vm,
+                                                  const double *wm, const double *rho,
+                                                  const double *p, double dt, const double *dx,
+                                                  const double *dxh, const double *dy,
+                                                  const double *dyh, const double *dz,
+                                                  const double *dzh, void *stream) {
+    PPS_TRY(grid_ok(g));
+    if (!dx || !dxh || !dy || !dyh || !dz || !dzh) return PPS_ERR_BAD_ARG;
+    hipStream_t s = as_stream(stream);
+    DevBuf buf;
+    const double *ax[3];
+    int rc = upload_spacings(g, dx, dxh, dy, dyh, dz, dzh, buf, ax, s);
+    if (rc == PPS_OK) rc = project_with(g, u, v, w, um, vm, wm, rho, p, dt, ax[0], ax[1], ax[2], s);
+    buf.release();
+    return rc;
 }
 
 extern "C" int pps_synth_drops(const pps_grid *g, int ndrop, double rmin, double rmax,

@@ -124,6 +124,46 @@ def test_stretched_assembly_on_device(pps):
     assert np.array_equal(_host(Ad), A)
 
 
+def test_stretched_projection(pps):
+    """project_velocity on stretched spacings (paris.f90:898-908 with the
+    reference's dxh, dyh, dzh) equals the numpy restatement bit for bit; the
+    fused solve + projection refuses a handle with cell widths (it uses the
+    uniform spacings)."""
+    _say("stretched projection")
+    g = pps.Grid.make(24, 16, 20)
+    ng = g.ng
+    rho, u, v, w = pps.synth_fields(g, [(0.4, 0.5, 0.5, 0.2)], 1.0, 1000.0, 1.5, seed=8)
+    um, vm, wm = pps.pressure_masks(g)
+    p = torch.randn(g.shape_p, dtype=torch.float64, device="cuda")
+    sp = []
+    _setup_poisson(g, (0.7, 0.4, 0.9), *[_host(t) for t in (rho, u, v, w, um, vm, wm)], 0.5,
+                   spacings=sp)
+    dx, dxh, dy, dyh, dz, dzh = sp
+    dt = 0.5
+    hu, hv, hw, hp, hr = (_host(t) for t in (u, v, w, p, rho))
+    hum, hvm, hwm = (_host(t) for t in (um, vm, wm))
+    K, J, I = slice(ng, ng + g.nz), slice(ng, ng + g.ny), slice(ng, ng + g.nx)
+    Iu, Jv, Kw = slice(ng, ng + g.nx - 1), slice(ng, ng + g.ny - 1), slice(ng, ng + g.nz - 1)
+    Iu1, Jv1, Kw1 = slice(ng + 1, ng + g.nx), slice(ng + 1, ng + g.ny), slice(ng + 1, ng + g.nz)
+    eu, ev, ew = hu.copy(), hv.copy(), hw.copy()
+    eu[K, J, Iu] = hu[K, J, Iu] - dt * (2.0 * hum[K, J, Iu] / dxh[Iu]) * (hp[K, J, Iu1] - hp[K, J, Iu]) / (
+        hr[K, J, Iu1] + hr[K, J, Iu])
+    ev[K, Jv, I] = hv[K, Jv, I] - dt * (2.0 * hvm[K, Jv, I] / dyh[Jv][:, None]) * (
+        hp[K, Jv1, I] - hp[K, Jv, I]) / (hr[K, Jv1, I] + hr[K, Jv, I])
+    ew[Kw, J, I] = hw[Kw, J, I] - dt * (2.0 * hwm[Kw, J, I] / dzh[Kw][:, None, None]) * (
+        hp[Kw1, J, I] - hp[Kw, J, I]) / (hr[Kw1, J, I] + hr[Kw, J, I])
+    pps.project_velocity_coords(g, u, v, w, um, vm, wm, rho, p, *sp, dt=dt)
+    for a, b in ((u, eu), (v, ev), (w, ew)):
+        assert np.array_equal(_host(a), b)
+    with pps.Solver(g, method="mixed") as s:
+        s.set_cell_widths(dx[I], dy[J], dz[K])
+        A = torch.zeros((8,) + (g.nz, g.ny, g.nx), dtype=torch.float64, device="cuda")
+        q = torch.zeros(g.shape_p, dtype=torch.float64, device="cuda")
+        with pytest.raises(pps.PPSError) as e:
+            s.solve_project(A, q, u, v, w, um, vm, wm, rho, dt, 1e-8, 1.2, 5)
+        assert e.value.code == pps.ERR_UNSUPPORTED
+
+
 def test_uniform_widths_change_nothing(pps):
     """Equal widths normalise to weights of exactly 1: the solve is the
     unweighted one bit for bit."""
