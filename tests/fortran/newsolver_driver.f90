@@ -8,6 +8,9 @@
 ! gas cells (pcmask 1) holding P_gas = 0 and 1, the source is zero, and the
 ! max-norm residual of the liquid cells is checked.  Method 8 calls
 ! poi_solve (modules.f90:2411), the HYPRE entry, with a PFMG solver type.
+! Method 7 assembles the same field on a stretched grid (xform 0.6, the
+! spacings of paris.f90:3056-3102) and solves it with the mixed method
+! after passing the cell widths (pps_cell_widths).
 program newsolver_driver
   use pps_paris
   implicit none
@@ -62,6 +65,10 @@ program newsolver_driver
   bc = 0
   if (method == 9) then
      call free_case()
+     stop
+  end if
+  if (method == 7) then
+     call stretched_case()
      stop
   end if
   call pps_setup(N, N, N, Ng, bc, 1d0, 1d0, 1d0, .true., 3, merge(PPS_MG_FP64, method, method == 8), &
@@ -119,6 +126,64 @@ contains
        stop 1
     end if
   end subroutine free_case
+
+  subroutine stretched_case()
+    real(8) :: xh(imin:imax), xc(imin:imax), dx(imin:imax), dxh(imin:imax), s, vol, sv
+    integer :: q
+    do q = imin, imax
+       s = dble(q - Ng) / dble(N)
+       xh(q) = 0.6d0 * s * (0.5d0 - s) * (1d0 - s) + s
+    end do
+    do q = imin + 1, imax; xc(q) = 0.5d0 * (xh(q) + xh(q - 1)); end do
+    xc(imin) = 2d0 * xh(imin) - xc(imin + 1)
+    do q = imin, imax - 1; dxh(q) = xc(q + 1) - xc(q); end do
+    dxh(imax) = dxh(imax - 1)
+    do q = imin + 1, imax; dx(q) = xh(q) - xh(q - 1); end do
+    dx(imin) = dx(imin + 1)
+    ! SetupPoisson (modules.f90:2737-2746) with the same spacings on all axes
+    do k = is, ie; do j = is, ie; do i = is, ie
+       A(i,j,k,1) = merge(0d0, 2d0 / (dx(i) * dxh(i-1) * (rho(i-1,j,k) + rho(i,j,k))), i == is)
+       A(i,j,k,2) = merge(0d0, 2d0 / (dx(i) * dxh(i) * (rho(i+1,j,k) + rho(i,j,k))), i == ie)
+       A(i,j,k,3) = merge(0d0, 2d0 / (dx(j) * dxh(j-1) * (rho(i,j-1,k) + rho(i,j,k))), j == is)
+       A(i,j,k,4) = merge(0d0, 2d0 / (dx(j) * dxh(j) * (rho(i,j+1,k) + rho(i,j,k))), j == ie)
+       A(i,j,k,5) = merge(0d0, 2d0 / (dx(k) * dxh(k-1) * (rho(i,j,k-1) + rho(i,j,k))), k == is)
+       A(i,j,k,6) = merge(0d0, 2d0 / (dx(k) * dxh(k) * (rho(i,j,k+1) + rho(i,j,k))), k == ie)
+       A(i,j,k,7) = sum(A(i,j,k,1:6))
+       A(i,j,k,8) = -((u(i,j,k) - u(i-1,j,k)) / dx(i) + (v(i,j,k) - v(i,j-1,k)) / dx(j) + &
+            (w(i,j,k) - w(i,j,k-1)) / dx(k))
+    end do; end do; end do
+    ! compatibility on the stretched grid: the volume-weighted mean of A8
+    sv = 0d0; vol = 0d0
+    do k = is, ie; do j = is, ie; do i = is, ie
+       sv = sv + dx(i) * dx(j) * dx(k) * A(i,j,k,8)
+       vol = vol + dx(i) * dx(j) * dx(k)
+    end do; end do; end do
+    A(:,:,:,8) = A(:,:,:,8) - sv / vol
+    rms_b = sqrt(sum(A(:,:,:,8)**2) / dble(N)**3)
+    maxError = 1d-10 * rms_b
+    p = 0d0
+    call pps_setup(N, N, N, Ng, bc, 1d0, 1d0, 1d0, .true., 3, PPS_MG_MIXED, ierr)
+    call pps_cell_widths(dx(is:ie), dx(is:ie), dx(is:ie), ierr)
+    if (ierr /= 0) then
+       print *, 'FAIL cell widths', ierr
+       stop 1
+    end if
+    call pps_newsolver(A, p, maxError, 1.2d0, 100, it, ierr, 2)
+    call pps_teardown()
+    res = 0d0
+    do k = is, ie; do j = is, ie; do i = is, ie
+       res = res + (A(i,j,k,8) - A(i,j,k,7) * p(i,j,k) + A(i,j,k,1) * p(i-1,j,k) + &
+            A(i,j,k,2) * p(i+1,j,k) + A(i,j,k,3) * p(i,j-1,k) + A(i,j,k,4) * p(i,j+1,k) + &
+            A(i,j,k,5) * p(i,j,k-1) + A(i,j,k,6) * p(i,j,k+1))**2
+    end do; end do; end do
+    tres2 = sqrt(res / dble(N)**3)
+    if (ierr == 0 .and. it <= 100 .and. tres2 < maxError) then
+       print '(a,i5,a,es10.3)', 'PASS stretched it=', it, ' rel=', tres2 / rms_b
+    else
+       print '(a,i5,a,i3,a,es10.3)', 'FAIL stretched it=', it, ' ierr=', ierr, ' rel=', tres2 / rms_b
+       stop 1
+    end if
+  end subroutine stretched_case
 
   real(8) function frand()
     ! deterministic sequence in (-1, 1)

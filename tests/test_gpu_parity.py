@@ -339,7 +339,7 @@ def test_fortran_caller(pps):
     exe = os.path.join(os.path.dirname(pps.LIB_PATH), "_build", "fortran", "newsolver_driver")
     if not os.path.exists(exe):
         pytest.skip("Fortran caller not built")
-    for method in (1, 2, 0, 9, 8):
+    for method in (1, 2, 0, 9, 8, 7):
         _say("fortran method", method)
         r = subprocess.run([exe, str(method)], capture_output=True, text=True, timeout=100)
         _say("  ", r.stdout.strip())
