@@ -171,6 +171,43 @@ def test_fast_mg_converged_solution_matches_oracle(pps, orc, method, case):
 
 
 @pytest.mark.parametrize("method", ["fp64", "mixed"])
+@pytest.mark.parametrize("nu", [1, 3])
+def test_fast_mg_smoothing_counts(pps, orc, method, nu):
+    """One and three sweeps per side instead of two (one post-sweep both adds
+    the coarse correction and forms the CG sums): the same converged solution
+    as the oracle at 32^3, and as the default cycle at 128^3."""
+    _say("smoothing", method, nu)
+    bc, drops = (1, 0, 0, 1, 0, 0), [(0.1, 0.5, 0.5, 0.2), (0.6, 0.3, 0.7, 0.12)]
+    g, og = _grids(pps, orc, 32, bc)
+    _, At, _ = pps.make_problem(n=32, bc=bc, drops=drops, rho2=1000.0, seed=9)
+    A = _host(At)
+    pref = np.zeros(og.shape_p)
+    orc.newsolver_mg(og, A, pref, 1e-300, 1.2, 3000)
+    ref = _mf(og.interior(pref))
+    brms = np.sqrt((A[7] ** 2).mean())
+    with pps.Solver(g, method=method, nu_pre=nu, nu_post=nu) as s:
+        p = torch.zeros(g.shape_p, dtype=torch.float64, device="cuda")
+        it, t2 = s.solve(At, p, 1e-14 * brms, 1.2, 200)
+    x = _mf(og.interior(_host(p)))
+    err = np.abs(x - ref).max() / np.abs(ref).max()
+    _say(f"   32^3 it={it} err={err:.2e}")
+    assert it <= 200 and err <= 1e-10
+    g2, At2, _ = pps.make_problem("sphere128")
+    og2 = orc.OrcGrid.make(128)
+    b2 = float(torch.sqrt((At2[7] ** 2).mean()))
+    sols = []
+    for k in (nu, 2):
+        with pps.Solver(g2, method=method, nu_pre=k, nu_post=k) as s:
+            p = torch.zeros(g2.shape_p, dtype=torch.float64, device="cuda")
+            it, t2 = s.solve(At2, p, 1e-13 * b2, 1.2, 200)
+            assert it <= 200
+            sols.append(_mf(og2.interior(_host(p))))
+    err = np.abs(sols[0] - sols[1]).max() / np.abs(sols[1]).max()
+    _say(f"   128^3 err={err:.2e}")
+    assert err <= 1e-10
+
+
+@pytest.mark.parametrize("method", ["fp64", "mixed"])
 def test_fast_mg_relative_1e10_iterations(pps, orc, method):
     """Config-1-like problem to the metric's tolerance: a handful of outer
     iterations, and the residual the solver reports is the reference's."""
