@@ -200,9 +200,11 @@ int pps_poi_solve_device(pps_handle *h, const double *dA, double *dp, double max
 int pps_last_history(pps_handle *h, double *hist, int cap);
 
 /* Adjoint of the last-solved system (no reference counterpart).  With
- * M = diag(A7) - offdiag(A1..A6) symmetric, solves M lam = g (g projected
- * mean-free when M is singular) on the same hierarchy, to relative residual
- * rtol.  dg, dlam: padded like p. */
+ * M = diag(A7) - offdiag(A1..A6), solves M^T lam = g (g projected mean-free
+ * when M is singular) on the same hierarchy, to relative residual rtol.
+ * M^T = M when M is symmetric; with cell widths set (pps_set_cell_widths)
+ * the volume-weighted rows W M are symmetric and lam = W (W M)^-1 g.
+ * dg, dlam: padded like p. */
 int pps_adjoint_device(pps_handle *h, const double *dA, const double *dg,
                        double *dlam, double rtol, int maxit, int *it,
                        void *stream);

@@ -389,7 +389,8 @@ class Solver:
         return it.value, t2.value, uo, vo, wo
 
     def adjoint(self, A, g, lam, rtol=1e-12, maxit=100, stream=None):
-        """Solve M lam = g (g, lam padded device tensors); returns iterations."""
+        """Solve M^T lam = g (g, lam padded device tensors; M^T = M unless cell
+        widths are set); returns iterations."""
         _need_gpu(A, g, lam)
         it = ctypes.c_int()
         rc = _lib().pps_adjoint_device(self._h, _ptr(A), _ptr(g), _ptr(lam), rtol, maxit,

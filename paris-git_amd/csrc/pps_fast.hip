@@ -1737,6 +1737,19 @@ k_interior(PackK d, const double *__restrict__ p, double *__restrict__ x) {
     x[c] = p[(i + d.ng) + (long long)d.X * (j + d.ng) + (long long)d.X * d.Y * (k + d.ng)];
 }
 
+// x *= wx[i] wy[j] wz[k] on the compact interior (the adjoint of a
+// row-weighted system: lam = W mu)
+__global__ void __launch_bounds__(256)
+k_weight(PackK d, double *__restrict__ x, const double *__restrict__ wx, const double *__restrict__ wy,
+         const double *__restrict__ wz) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= d.nx || j >= d.ny) return;
+    const long long c = i + (long long)d.nx * (j + (long long)d.ny * k);
+    x[c] *= wx[i] * wy[j] * wz[k];
+}
+
 __global__ void __launch_bounds__(256)
 k_fill(double *__restrict__ a, long long n, double v) {
     for (long long q = blockIdx.x * 256LL + threadIdx.x; q < n; q += (long long)gridDim.x * 256) a[q] = v;
@@ -2530,10 +2543,13 @@ int fast_solve(pps_handle *h, const double *dA, double *dp, double maxError, int
     return PPS_OK;
 }
 
+// M^T lam = g.  With cell widths the stored operator is S = W M (rows
+// weighted, symmetric), so M^T = S W^-1: the hierarchy solves S mu = g and
+// lam = W mu; the residual of S mu = g is that of M^T lam = g.  The
+// singular case projects g mean-free either way (the null space of S, and
+// of M, is the constants).
 int fast_adjoint(pps_handle *h, const double *dA, const double *dg, double *dlam, double rtol,
                  int maxit, int *it, hipStream_t s) {
-    // the adjoint of a row-weighted system is not the system itself
-    if (h->have_widths) return PPS_ERR_BAD_ARG;
     PPS_TRY(fast_ensure(h));
     FastState *f = h->fast;
     const bool mixed = h->opt.method == PPS_MG_MIXED;
@@ -2573,6 +2589,10 @@ int fast_adjoint(pps_handle *h, const double *dA, const double *dg, double *dlam
     if (mixed) rc = outer_iterate<float>(h, f->h32, true, tol, maxit, &k, &last, true, s);
     else rc = outer_iterate<double>(h, f->h64, false, tol, maxit, &k, &last, true, s);
     if (rc != PPS_OK) return rc;
+    if (h->have_widths) {
+        const double *w = h->widths.as<double>();
+        k_weight<<<cgrid3(d.nx, d.ny, d.nz), kB3, 0, s>>>(packk(d), V.x, w, w + d.nx, w + d.nx + d.ny);
+    }
     // padded output, ghosts: wrapped (periodic) or zero (walls)
     PPS_HIP(hipMemsetAsync(dlam, 0, sizeof(double) * d.padded(), s));
     k_unpack<<<cgrid3(d.nx, d.ny, d.nz), kB3, 0, s>>>(packk(d), V.x, dlam);
