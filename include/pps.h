@@ -93,7 +93,9 @@ typedef struct pps_options {
     int outer;             /* enum pps_outer (PPS_MG_FP64 / PPS_MG_MIXED) */
     int nrelax;            /* reference nrelax (default 3) */
     int sym_mg;            /* reference sym_MG (default 0) */
-    int nu_pre, nu_post;   /* fast V-cycle smoothing sweeps (default 2, 2) */
+    int nu_pre, nu_post;   /* fast V-cycle smoothing sweeps (default 2, 2); equal
+                              counts keep the cycle symmetric, as the CG outer
+                              iteration assumes */
     int check_setup;       /* run the check_poisson_setup tests first (default 0) */
     int multigrid;         /* reference MultiGrid flag for pps_new_solver:
                               1 -> method, 0 -> PPS_SOR (default 1) */
