@@ -233,6 +233,17 @@ int pps_faces_to_A_device(const pps_grid *grid, const double *cface,
                           void *stream);
 
 /* ---- producer / consumer of the system (modules.f90, paris.f90) ------ */
+/* linfunc's MeanFlag values (vofmodules.f90:109-110) */
+enum pps_mean { PPS_ARITH_MEAN = 101, PPS_HARM_MEAN = 102 };
+/* linfunc (vofmodules.f90:171-218) then do_all_ghost (modules.f90:2062-2072)
+ * on one rank: field = cvof * (a2 - a1) + a1 (PPS_ARITH_MEAN) or
+ * 1 / (cvof * (1/a2 - 1/a1) + 1/a1) (PPS_HARM_MEAN, each inverse of a + 1e-50),
+ * with cvof 27-point filtered off the outermost ghost layer when nfilter is 1;
+ * then the ghost layers of periodic axes are wrapped, axis by axis over the
+ * full extent of the others.  cvof, field: padded like p; field must not be
+ * cvof when nfilter is 1.  nfilter other than 0, 1: PPS_ERR_BAD_ARG. */
+int pps_linfunc_device(const pps_grid *grid, const double *cvof, double a1, double a2,
+                       int mean_flag, int nfilter, double *field, void *stream);
 /* SetPressureBC on all-ones masks (modules.f90:916-946), padded arrays. */
 int pps_pressure_masks_device(const pps_grid *grid, double *umask,
                               double *vmask, double *wmask, void *stream);

@@ -85,6 +85,7 @@ def lib():
         "orc_get_mgtest_err": (_D, [G, _P]),
         "orc_project_velocity": (None, [G, _P, _P, _P, _P, _P, _P, _P, _P, _D]),
         "orc_pcg_solve": (_I, [G, _P, _P, _D, _I, DP]),
+        "orc_linfunc": (_I, [G, _P, _D, _D, _I, _I, _P]),
         "orc_set_threads": (None, [_I]),
         "orc_max_threads": (_I, []),
     }
@@ -212,6 +213,15 @@ def get_mgtest_err(g, p):
 def project_velocity(g, u, v, w, um, vm, wm, rho, p, dt=1.0):
     lib().orc_project_velocity(ctypes.byref(g), _p(u), _p(v), _p(w), _p(um), _p(vm), _p(wm),
                                _p(rho), _p(p), dt)
+
+
+def linfunc(g, cvof, a1, a2, mean_flag=101, nfilter=0):
+    """vofmodules.f90:171-218 then do_all_ghost; returns the padded field."""
+    out = np.empty(g.shape_p)
+    rc = lib().orc_linfunc(ctypes.byref(g), _p(cvof), a1, a2, mean_flag, nfilter, _p(out))
+    if rc != 0:
+        raise ValueError("linfunc: mean_flag must be 101 or 102 and nfilter 0 or 1")
+    return out
 
 
 def set_threads(n):

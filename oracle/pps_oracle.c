@@ -922,3 +922,54 @@ int orc_pcg_solve(const orc_grid *g, const double *A, double *p, double rtol,
     free(r); free(z); free(q); free(d); free(pc);
     return it;
 }
+
+/* ------------------------------------------------------------------ */
+/* linfunc: a material property from the volume fraction               */
+/* (vofmodules.f90:133-147 weights, 171-218 linfunc), then do_all_ghost */
+/* (modules.f90:2062-2072): ghost_x, ghost_y, ghost_z over the full     */
+/* extent of the other two axes, i.e. a periodic wrap of the ng ghost   */
+/* layers axis by axis, and nothing at walls (MPI_PROC_NULL).           */
+/* ------------------------------------------------------------------ */
+int orc_linfunc(const orc_grid *g, const double *cvof, double a1, double a2, int mean_flag,
+                int nfilter, double *field) {
+    if (mean_flag != 101 && mean_flag != 102) return -1;
+    if (nfilter != 0 && nfilter != 1) return -1;
+    lvl_t L = grid_lvl(g);
+    const int X = L.X, Y = L.Y, Z = L.Z;
+    const long sy = X, sz = (long)X * Y;
+    const double b1 = 8.0 / 64, b2 = 4.0 / 64, b3 = 2.0 / 64, b4 = 1.0 / 64;
+    const double inva1 = 1.0 / (a1 + 1e-50), inva2 = 1.0 / (a2 + 1e-50);
+    for (int k = 0; k < Z; ++k)
+        for (int j = 0; j < Y; ++j)
+            for (int i = 0; i < X; ++i) {
+                const long q = i + sy * j + sz * k;
+                const double *c = cvof;
+                double cf = c[q];
+                if (nfilter == 1 && i > 0 && i < X - 1 && j > 0 && j < Y - 1 && k > 0 && k < Z - 1) {
+                    const double s2 = c[q - 1] + c[q - sy] + c[q - sz] + c[q + 1] + c[q + sy] + c[q + sz];
+                    const double s3 = c[q + 1 + sy] + c[q + 1 - sy] + c[q - 1 + sy] + c[q - 1 - sy] +
+                                      c[q + 1 + sz] + c[q + 1 - sz] + c[q - 1 + sz] + c[q - 1 - sz] +
+                                      c[q + sy + sz] + c[q + sy - sz] + c[q - sy + sz] + c[q - sy - sz];
+                    const double s4 = c[q + 1 + sy + sz] + c[q + 1 + sy - sz] + c[q + 1 - sy + sz] +
+                                      c[q + 1 - sy - sz] + c[q - 1 + sy + sz] + c[q - 1 + sy - sz] +
+                                      c[q - 1 - sy + sz] + c[q - 1 - sy - sz];
+                    cf = b1 * c[q] + b2 * s2 + b3 * s3 + b4 * s4;
+                }
+                field[q] = mean_flag == 101 ? cf * (a2 - a1) + a1 : 1.0 / (cf * (inva2 - inva1) + inva1);
+            }
+    const int ext[3] = {X, Y, Z}, n[3] = {L.n[0], L.n[1], L.n[2]};
+    for (int d = 0; d < 3; ++d) {
+        if (!L.per[d]) continue;
+        for (int k = 0; k < Z; ++k)
+            for (int j = 0; j < Y; ++j)
+                for (int i = 0; i < X; ++i) {
+                    int idx[3] = {i, j, k};
+                    const int a = idx[d];
+                    if (a >= L.ng && a < L.ng + n[d]) continue;
+                    idx[d] = a < L.ng ? a + n[d] : a - n[d];
+                    field[i + sy * j + sz * k] = field[idx[0] + sy * idx[1] + sz * idx[2]];
+                }
+        (void)ext;
+    }
+    return 0;
+}

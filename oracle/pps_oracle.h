@@ -122,6 +122,12 @@ void orc_setup_testmg(const orc_grid *g, const double *rho, double *A);
 double orc_get_mgtest_err(const orc_grid *g, const double *p);
 
 /* project_velocity, plain staggered branch (paris.f90:898-908). */
+/* linfunc (vofmodules.f90:171-218) + do_all_ghost (modules.f90:2062-2072) on
+ * padded arrays; mean_flag 101 ArithMean / 102 HarmMean, nfilter 0 or 1.
+ * Returns 0, or -1 for other flags. */
+int orc_linfunc(const orc_grid *g, const double *cvof, double a1, double a2, int mean_flag,
+                int nfilter, double *field);
+
 void orc_project_velocity(const orc_grid *g, double *u, double *v, double *w,
                           const double *umask, const double *vmask,
                           const double *wmask, const double *rho,

@@ -148,6 +148,7 @@ def _lib():
         "pps_adjoint_device": (_I, [_P, _P, _P, _P, _D, _I, _IP, _P]),
         "pps_face_gradient_device": (_I, [_GP, _P, _P, _P, _P, _P, _P, _P, _P]),
         "pps_pressure_masks_device": (_I, [_GP, _P, _P, _P, _P]),
+        "pps_linfunc_device": (_I, [_GP, _P, _D, _D, _I, _I, _P, _P]),
         "pps_setup_poisson_device": (_I, [_GP, _P, _P, _P, _P, _P, _P, _P, _D, _D, _P, _P]),
         "pps_setup_poisson_coords_device": (_I, [_GP, _P, _P, _P, _P, _P, _P, _P, _D, _D, _P, _P, _P,
                                                  _P, _P, _P, _P, _P]),
@@ -185,7 +186,7 @@ def exported_symbols():
         "pps_new_solver", "pps_new_solver_mg", "pps_solve_device", "pps_solve_project_device",
         "pps_last_history",
         "pps_last_timing", "pps_adjoint_device", "pps_face_gradient_device",
-        "pps_pressure_masks_device", "pps_setup_poisson_device", "pps_setup_poisson_coords_device",
+        "pps_pressure_masks_device", "pps_linfunc_device", "pps_setup_poisson_device", "pps_setup_poisson_coords_device",
         "pps_check_setup_device",
         "pps_remove_rhs_mean_device", "pps_calc_residuals_device",
         "pps_project_velocity_device", "pps_project_velocity_coords_device",
@@ -448,6 +449,22 @@ def synth_fields(grid, drops, rho1, rho2, width=1.5, seed=1, device="cuda"):
                                           seed, _ptr(rho), _ptr(u), _ptr(v), _ptr(w), _stream()),
            "pps_synth_fields_device")
     return rho, u, v, w
+
+
+ARITH_MEAN, HARM_MEAN = 101, 102
+
+
+def linfunc(grid, cvof, a1, a2, mean_flag=ARITH_MEAN, nfilter=0):
+    """linfunc + do_all_ghost (vofmodules.f90:171-218, modules.f90:2062-2072):
+    the padded property field (a1 where cvof = 0, a2 where 1) from the
+    padded volume fraction; a new device tensor."""
+    import torch
+    _need_gpu(cvof)
+    out = torch.empty(grid.shape_p, dtype=torch.float64, device=cvof.device)
+    _check(_lib().pps_linfunc_device(ctypes.byref(grid), _ptr(cvof), float(a1), float(a2),
+                                     int(mean_flag), int(nfilter), _ptr(out), _stream()),
+           "pps_linfunc_device")
+    return out
 
 
 def pressure_masks(grid, device="cuda"):

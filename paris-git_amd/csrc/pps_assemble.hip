@@ -471,6 +471,62 @@ k_fill_ghosts(DK d, int dir, int periodic, int mode, double *__restrict__ f) {
     }
 }
 
+// linfunc (vofmodules.f90:171-218) on every padded cell: the volume
+// fraction, 27-point filtered (weights 8, 4, 2, 1 / 64 of
+// vofmodules.f90:133-147) off the outermost ghost layer when nfilter is 1,
+// mapped to a1..a2 arithmetically (harm 0) or harmonically (harm 1).  The
+// operand order is the reference's, without contraction.
+__global__ void __launch_bounds__(256)
+k_linfunc(DK d, const double *__restrict__ c, double a1, double a2, int harm, int nfilter,
+          double *__restrict__ field) {
+#pragma clang fp contract(off)
+    const int Z = d.nz + 2 * d.ng;
+    const long long sy = d.X, sz = (long long)d.X * d.Y, np = sz * Z;
+    const double b1 = 8.0 / 64, b2 = 4.0 / 64, b3 = 2.0 / 64, b4 = 1.0 / 64;
+    const double inva1 = 1.0 / (a1 + 1e-50), inva2 = 1.0 / (a2 + 1e-50);
+    for (long long q = blockIdx.x * 256LL + threadIdx.x; q < np; q += (long long)gridDim.x * 256) {
+        const int i = (int)(q % d.X);
+        const long long t = q / d.X;
+        const int j = (int)(t % d.Y), k = (int)(t / d.Y);
+        double cf = c[q];
+        if (nfilter == 1 && i > 0 && i < d.X - 1 && j > 0 && j < d.Y - 1 && k > 0 && k < Z - 1) {
+            const double s2 = c[q - 1] + c[q - sy] + c[q - sz] + c[q + 1] + c[q + sy] + c[q + sz];
+            const double s3 = c[q + 1 + sy] + c[q + 1 - sy] + c[q - 1 + sy] + c[q - 1 - sy] +
+                              c[q + 1 + sz] + c[q + 1 - sz] + c[q - 1 + sz] + c[q - 1 - sz] +
+                              c[q + sy + sz] + c[q + sy - sz] + c[q - sy + sz] + c[q - sy - sz];
+            const double s4 = c[q + 1 + sy + sz] + c[q + 1 + sy - sz] + c[q + 1 - sy + sz] +
+                              c[q + 1 - sy - sz] + c[q - 1 + sy + sz] + c[q - 1 + sy - sz] +
+                              c[q - 1 - sy + sz] + c[q - 1 - sy - sz];
+            cf = b1 * c[q] + b2 * s2 + b3 * s3 + b4 * s4;
+        }
+        field[q] = harm ? 1.0 / (cf * (inva2 - inva1) + inva1) : cf * (a2 - a1) + a1;
+    }
+}
+
+// One axis of do_all_ghost (modules.f90:2062-2072) on one rank: the ng
+// ghost layers of a periodic axis take the wrapped interior values, over
+// the full padded extent of the other two axes (ghost_x/y/z send whole
+// planes), so the three launches compose edges and corners as the
+// reference's sequence does.  Sources are interior along the axis and
+// destinations ghosts, so one launch never reads what it writes.
+__global__ void __launch_bounds__(256)
+k_wrap_axis(DK d, int dir, double *__restrict__ f) {
+    const int Z = d.nz + 2 * d.ng;
+    const long long np = (long long)d.X * d.Y * Z;
+    const int n[3] = {d.nx, d.ny, d.nz};
+    for (long long q = blockIdx.x * 256LL + threadIdx.x; q < np; q += (long long)gridDim.x * 256) {
+        int idx[3];
+        idx[0] = (int)(q % d.X);
+        const long long t = q / d.X;
+        idx[1] = (int)(t % d.Y);
+        idx[2] = (int)(t / d.Y);
+        const int a = idx[dir];
+        if (a >= d.ng && a < d.ng + n[dir]) continue;
+        idx[dir] = a < d.ng ? a + n[dir] : a - n[dir];
+        f[q] = f[idx[0] + (long long)d.X * (idx[1] + (long long)d.Y * idx[2])];
+    }
+}
+
 static unsigned lin_blocks(long long n) {
     long long b = (n + 255) / 256;
     if (b > 16384) b = 16384;
@@ -569,6 +625,25 @@ extern "C" int pps_pressure_masks_device(const pps_grid *g, double *um, double *
     k_masks<<<lin_blocks(d.padded()), 256, 0, s>>>(dk(d), g->bdry_cond[0], g->bdry_cond[1],
                                                   g->bdry_cond[2], g->bdry_cond[3],
                                                   g->bdry_cond[4], g->bdry_cond[5], um, vm, wm);
+    PPS_HIP(hipGetLastError());
+    PPS_HIP(hipStreamSynchronize(s));
+    return PPS_OK;
+}
+
+// linfunc + do_all_ghost: field (padded) from cvof (padded)
+extern "C" int pps_linfunc_device(const pps_grid *g, const double *cvof, double a1, double a2,
+                                  int mean_flag, int nfilter, double *field, void *stream) {
+    PPS_TRY(grid_ok(g));
+    if (!cvof || !field) return PPS_ERR_BAD_ARG;
+    if (mean_flag != PPS_ARITH_MEAN && mean_flag != PPS_HARM_MEAN) return PPS_ERR_BAD_ARG;
+    if (nfilter != 0 && nfilter != 1) return PPS_ERR_BAD_ARG;
+    if (nfilter == 1 && field == cvof) return PPS_ERR_BAD_ARG;   // the filter reads neighbours
+    const Dims d = dims_of(*g);
+    hipStream_t s = as_stream(stream);
+    k_linfunc<<<lin_blocks(d.padded()), 256, 0, s>>>(dk(d), cvof, a1, a2, mean_flag == PPS_HARM_MEAN,
+                                                     nfilter, field);
+    for (int a = 0; a < 3; ++a)
+        if (g->bdry_cond[a] == 1) k_wrap_axis<<<lin_blocks(d.padded()), 256, 0, s>>>(dk(d), a, field);
     PPS_HIP(hipGetLastError());
     PPS_HIP(hipStreamSynchronize(s));
     return PPS_OK;
