@@ -69,6 +69,7 @@ def lib():
         "orc_coords": (None, [_I, _I, _D, _P, _P, _P, _P]),
         "orc_pressure_masks": (None, [G, _P, _P, _P]),
         "orc_setup_poisson": (_I, [G, _P, _P, _P, _P, _P, _P, _P, _D, _D, _P]),
+        "orc_setup_poisson_bp": (_I, [G, _P, _P, _P, _P, _P, _P, _P, _D, _D, _P, _P]),
         "orc_check_poisson_setup": (_I, [G, _P, _P, _P, _P, _P]),
         "orc_residual_field": (None, [G, _P, _P, _P]),
         "orc_compute_residual": (_D, [G, _P, _P, _I]),
@@ -114,10 +115,14 @@ def pressure_masks(g):
     return um, vm, wm
 
 
-def setup_poisson(g, u, v, w, um, vm, wm, rho, dt=1.0, volume_source=0.0):
+def setup_poisson(g, u, v, w, um, vm, wm, rho, dt=1.0, volume_source=0.0, bp=None):
+    """SetupPoisson + Poisson_BCs + check_and_debug_Poisson; bp is the
+    reference's BoundaryPressure(1:6) (None: zeros).  Returns (rc, A)."""
     A = np.empty(g.shape_A)
-    rc = lib().orc_setup_poisson(ctypes.byref(g), _p(u), _p(v), _p(w), _p(um), _p(vm), _p(wm),
-                                 _p(rho), dt, volume_source, _p(A))
+    bpa = np.zeros(6) if bp is None else np.ascontiguousarray(bp, dtype=np.float64)
+    assert bpa.shape == (6,)
+    rc = lib().orc_setup_poisson_bp(ctypes.byref(g), _p(u), _p(v), _p(w), _p(um), _p(vm), _p(wm),
+                                    _p(rho), dt, volume_source, _p(bpa), _p(A))
     return rc, A
 
 

@@ -142,8 +142,17 @@ int orc_setup_poisson(const orc_grid *g, const double *u, const double *v,
                       const double *vmask, const double *wmask,
                       const double *rho, double dt, double volume_source,
                       double *A) {
+    const double bp[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    return orc_setup_poisson_bp(g, u, v, w, umask, vmask, wmask, rho, dt, volume_source, bp, A);
+}
+
+int orc_setup_poisson_bp(const orc_grid *g, const double *u, const double *v,
+                         const double *w, const double *umask,
+                         const double *vmask, const double *wmask,
+                         const double *rho, double dt, double volume_source,
+                         const double bp[6], double *A) {
     for (int f = 0; f < 6; ++f)
-        if (g->bc[f] >= 5) return ORC_ERR_UNSUPPORTED;
+        if (g->bc[f] < 0 || g->bc[f] > 6) return ORC_ERR_UNSUPPORTED;
     lvl_t L = grid_lvl(g);
     long n = ncell(&L);
     axis_t ax, ay, az;
@@ -180,25 +189,137 @@ int orc_setup_poisson(const orc_grid *g, const double *u, const double *v,
             }
     axis_free(&ax); axis_free(&ay); axis_free(&az);
 
-    /* Poisson_BCs: flux conditions on a single rank (codes 3 and 4) */
-    for (int k = 0; k < g->nz; ++k)
-        for (int j = 0; j < g->ny; ++j) {
-            long c0 = ic(&L, 0, j, k), c1 = ic(&L, g->nx - 1, j, k);
-            if (g->bc[0] == 3) { A[6 * n + c0] = A[6 * n + c0] - A[0 * n + c0]; A[0 * n + c0] = 0.0; }
-            if (g->bc[3] == 4 || g->bc[3] == 3) { A[6 * n + c1] = A[6 * n + c1] - A[1 * n + c1]; A[1 * n + c1] = 0.0; }
+    /* Poisson_BCs on a single rank (every face is a domain face), block by
+     * block in the reference's statement order; later blocks overwrite the
+     * corner entries earlier ones set.  Codes 3 / 4: drop the outward
+     * coupling (flux condition).  Code 5: the face row becomes
+     * P_0 - P_inner / 3 = 2/3 P_b (linear extrapolation to the face value
+     * BoundaryPressure), corners adding the conditions of both faces
+     * (P_bc flags).  Code 6 (x+ only): p_b and du/dn at the face.  The z+
+     * block of code 5 writes its y-coupling corner entries into the ks plane
+     * (modules.f90:2884-2887), as the reference does. */
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    const int ie = nx - 1, je = ny - 1, ke = nz - 1;
+    double P_bc[4] = {0.0, 0.0, 0.0, 0.0};
+    const double c23 = 2.0 / 3.0, c13 = 1.0 / 3.0;
+#define AA(s, i, j, k) A[(long)(s) * n + ic(&L, (i), (j), (k))]
+    /* x- (2768-2781) */
+    if (g->bc[0] == 3) {
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j) { AA(6, 0, j, k) = AA(6, 0, j, k) - AA(0, 0, j, k); AA(0, 0, j, k) = 0.0; }
+    } else if (g->bc[0] == 5) {
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j) {
+                AA(7, 0, j, k) = c23 * bp[0];
+                AA(6, 0, j, k) = 1.0;
+                for (int s = 0; s < 6; ++s) AA(s, 0, j, k) = 0.0;
+                AA(1, 0, j, k) = c13;
+            }
+        P_bc[0] = 1.0;
+    }
+    /* x+ (2784-2801) */
+    if (g->bc[3] == 4 || g->bc[3] == 3) {
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j) { AA(6, ie, j, k) = AA(6, ie, j, k) - AA(1, ie, j, k); AA(1, ie, j, k) = 0.0; }
+    } else if (g->bc[3] == 5) {
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j) {
+                AA(7, ie, j, k) = c23 * bp[1];
+                AA(6, ie, j, k) = 1.0;
+                for (int s = 1; s < 6; ++s) AA(s, ie, j, k) = 0.0;
+                AA(0, ie, j, k) = c13;
+            }
+        P_bc[1] = 1.0;
+    } else if (g->bc[3] == 6) {
+        axis_t bx;
+        if (axis_alloc(&bx, g->nx, g->ng, g->xlength)) return ORC_ERR_NOMEM;
+        const double dxe = bx.dx[ie + G];
+        axis_free(&bx);
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j) {
+                AA(6, ie, j, k) = AA(6, ie, j, k) - AA(0, ie, j, k) * 2.0 / 3.0 + AA(1, ie, j, k) * 5.0 / 3.0;
+                AA(7, ie, j, k) = AA(7, ie, j, k) + 8.0 / 3.0 * AA(1, ie, j, k) * bp[1] +
+                                  (u[ip(&L, ie, j, k)] - u[ip(&L, ie - 1, j, k)]) / dxe;
+                AA(0, ie, j, k) = AA(0, ie, j, k) * 1.0 / 3.0;
+                AA(1, ie, j, k) = 0.0;
+            }
+    }
+    /* y- (2804-2822) */
+    if (g->bc[1] == 3) {
+        for (int k = 0; k < nz; ++k)
+            for (int i = 0; i < nx; ++i) { AA(6, i, 0, k) = AA(6, i, 0, k) - AA(2, i, 0, k); AA(2, i, 0, k) = 0.0; }
+    } else if (g->bc[1] == 5) {
+        for (int k = 0; k < nz; ++k) {
+            for (int i = 0; i < nx; ++i) AA(7, i, 0, k) = c23 * bp[2];
+            AA(7, 0, 0, k) = c23 * (bp[2] + bp[0] * P_bc[0]);
+            AA(7, ie, 0, k) = c23 * (bp[2] + bp[1] * P_bc[1]);
+            for (int i = 0; i < nx; ++i) AA(6, i, 0, k) = 1.0;
+            AA(6, 0, 0, k) = 1.0 + P_bc[0];
+            AA(6, ie, 0, k) = 1.0 + P_bc[1];
+            for (int i = 0; i < nx; ++i)
+                for (int s = 0; s < 6; ++s) AA(s, i, 0, k) = 0.0;
+            AA(1, 0, 0, k) = c13 * P_bc[0];
+            AA(0, ie, 0, k) = c13 * P_bc[1];
+            for (int i = 0; i < nx; ++i) AA(3, i, 0, k) = c13;
         }
-    for (int k = 0; k < g->nz; ++k)
-        for (int i = 0; i < g->nx; ++i) {
-            long c0 = ic(&L, i, 0, k), c1 = ic(&L, i, g->ny - 1, k);
-            if (g->bc[1] == 3) { A[6 * n + c0] = A[6 * n + c0] - A[2 * n + c0]; A[2 * n + c0] = 0.0; }
-            if (g->bc[4] == 3) { A[6 * n + c1] = A[6 * n + c1] - A[3 * n + c1]; A[3 * n + c1] = 0.0; }
+        P_bc[2] = 1.0;
+    }
+    /* y+ (2824-2843) */
+    if (g->bc[4] == 3) {
+        for (int k = 0; k < nz; ++k)
+            for (int i = 0; i < nx; ++i) { AA(6, i, je, k) = AA(6, i, je, k) - AA(3, i, je, k); AA(3, i, je, k) = 0.0; }
+    } else if (g->bc[4] == 5) {
+        for (int k = 0; k < nz; ++k) {
+            for (int i = 0; i < nx; ++i) AA(7, i, je, k) = c23 * bp[3];
+            AA(7, 0, je, k) = c23 * (bp[3] + bp[0] * P_bc[0]);
+            AA(7, ie, je, k) = c23 * (bp[3] + bp[1] * P_bc[1]);
+            for (int i = 0; i < nx; ++i) AA(6, i, je, k) = 1.0;
+            AA(6, 0, je, k) = 1.0 + P_bc[0];
+            AA(6, ie, je, k) = 1.0 + P_bc[1];
+            for (int i = 0; i < nx; ++i)
+                for (int s = 0; s < 6; ++s) AA(s, i, je, k) = 0.0;
+            AA(1, 0, je, k) = c13 * P_bc[0];
+            AA(0, ie, je, k) = c13 * P_bc[1];
+            for (int i = 0; i < nx; ++i) AA(2, i, je, k) = c13;
         }
-    for (int j = 0; j < g->ny; ++j)
-        for (int i = 0; i < g->nx; ++i) {
-            long c0 = ic(&L, i, j, 0), c1 = ic(&L, i, j, g->nz - 1);
-            if (g->bc[2] == 3) { A[6 * n + c0] = A[6 * n + c0] - A[4 * n + c0]; A[4 * n + c0] = 0.0; }
-            if (g->bc[5] == 3) { A[6 * n + c1] = A[6 * n + c1] - A[5 * n + c1]; A[5 * n + c1] = 0.0; }
+        P_bc[3] = 1.0;
+    }
+    /* z- (2845-2867) and z+ (2869-2891) */
+    for (int side = 0; side < 2; ++side) {
+        const int code = g->bc[side ? 5 : 2];
+        const int kk = side ? ke : 0;
+        const double P = bp[side ? 5 : 4];
+        if (code == 3) {
+            const int s0 = side ? 5 : 4;
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) { AA(6, i, j, kk) = AA(6, i, j, kk) - AA(s0, i, j, kk); AA(s0, i, j, kk) = 0.0; }
+        } else if (code == 5) {
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) AA(7, i, j, kk) = c23 * P;
+            AA(7, 0, 0, kk) = c23 * ((P + bp[0] * P_bc[0]) + bp[2] * P_bc[2]);
+            AA(7, ie, 0, kk) = c23 * ((P + bp[1] * P_bc[1]) + bp[2] * P_bc[2]);
+            AA(7, 0, je, kk) = c23 * ((P + bp[0] * P_bc[0]) + bp[3] * P_bc[3]);
+            AA(7, ie, je, kk) = c23 * ((P + bp[1] * P_bc[1]) + bp[3] * P_bc[3]);
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) AA(6, i, j, kk) = 1.0;
+            AA(6, 0, 0, kk) = (1.0 + P_bc[0]) + P_bc[2];
+            AA(6, ie, 0, kk) = (1.0 + P_bc[1]) + P_bc[2];
+            AA(6, 0, je, kk) = (1.0 + P_bc[0]) + P_bc[3];
+            AA(6, ie, je, kk) = (1.0 + P_bc[1]) + P_bc[3];
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i)
+                    for (int s = 0; s < 6; ++s) AA(s, i, j, kk) = 0.0;
+            /* the y-coupling corner entries: the ks plane in both blocks */
+            AA(1, 0, 0, kk) = c13 * P_bc[0]; AA(3, 0, 0, 0) = c13 * P_bc[2];
+            AA(0, ie, 0, kk) = c13 * P_bc[1]; AA(3, ie, 0, 0) = c13 * P_bc[2];
+            AA(1, 0, je, kk) = c13 * P_bc[0]; AA(2, 0, je, 0) = c13 * P_bc[3];
+            AA(0, ie, je, kk) = c13 * P_bc[1]; AA(2, ie, je, 0) = c13 * P_bc[3];
+            const int s1 = side ? 4 : 5;
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) AA(s1, i, j, kk) = c13;
         }
+    }
+#undef AA
 
     /* check_and_debug_Poisson: a tiny diagonal is only legal in a solid
      * (all six face masks closed), where A7 becomes 1 */
@@ -841,20 +962,54 @@ void orc_project_velocity(const orc_grid *g, double *u, double *v, double *w,
 /* (their contribution moves to the right-hand side) and periodic ghosts  */
 /* wrapped.  Not a reference algorithm: it pins converged solutions.      */
 /* ------------------------------------------------------------------ */
-static void op_apply(const lvl_t *L, const double *A, double *x, double *y) {
-    /* y = M x on the interior; x is a padded vector whose wall ghosts are 0 */
-    apply_bc_lvl(L, x);
-    long n = ncell(L), sx = 1, sy = L->X, sz = (long)L->X * L->Y;
-    for (int k = 0; k < L->n[2]; ++k)
-        for (int j = 0; j < L->n[1]; ++j)
-            for (int i = 0; i < L->n[0]; ++i) {
-                long c = ic(L, i, j, k), q = ip(L, i, j, k);
-                y[c] = A[6 * n + c] * x[q] - (A[0 * n + c] * x[q - sx] + A[1 * n + c] * x[q + sx] +
-                                              A[2 * n + c] * x[q - sy] + A[3 * n + c] * x[q + sy] +
-                                              A[4 * n + c] * x[q - sz] + A[5 * n + c] * x[q + sz]);
-            }
+/* Threads of the Jacobi-PCG below (a checker of converged solutions, not
+ * a restatement of a reference loop): all host threads by default.  The
+ * thread count changes only the summation order of its dot products. */
+static int orc_pcg_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }
 
+/* y = M x on the interior and the dot product x.y; x is a padded vector
+ * whose wall ghosts are 0 */
+static double op_apply_dot(const lvl_t *L, const double *A, double *x, double *y) {
+    apply_bc_lvl(L, x);
+    const long n = ncell(L), sx = 1, sy = L->X, sz = (long)L->X * L->Y;
+    const int nx = L->n[0], ny = L->n[1], nz = L->n[2];
+    double dot = 0.0;
+#pragma omp parallel for collapse(2) num_threads(orc_pcg_threads()) reduction(+ : dot) schedule(static)
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                long c = ic(L, i, j, k), q = ip(L, i, j, k);
+                double v = A[6 * n + c] * x[q] - (A[0 * n + c] * x[q - sx] + A[1 * n + c] * x[q + sx] +
+                                                  A[2 * n + c] * x[q - sy] + A[3 * n + c] * x[q + sy] +
+                                                  A[4 * n + c] * x[q - sz] + A[5 * n + c] * x[q + sz]);
+                y[c] = v;
+                dot += x[q] * v;
+            }
+    return dot;
+}
+
+static double true_residual(const orc_grid *g, const lvl_t *L, const double *A, const double *p,
+                            double *pc, double *r) {
+    const long n = ncell(L), np = npad(L);
+    memcpy(pc, p, sizeof(double) * np);
+    apply_bc_lvl(L, pc);
+    orc_residual_field(g, A, pc, r);
+    double rn = 0.0;
+#pragma omp parallel for num_threads(orc_pcg_threads()) reduction(+ : rn) schedule(static)
+    for (long c = 0; c < n; ++c) rn += r[c] * r[c];
+    return rn;
+}
+
+/* Jacobi-preconditioned CG on the 8-slot system (M = diag(A7) -
+ * offdiag(A1..A6), the residual of MGsolver.f90:97-100), to ||r||/||A8|| <=
+ * rtol.  Independent of every multigrid component: used only to pin
+ * converged solutions. */
 int orc_pcg_solve(const orc_grid *g, const double *A, double *p, double rtol,
                   int maxit, double *relres) {
     lvl_t L = grid_lvl(g);
@@ -863,61 +1018,52 @@ int orc_pcg_solve(const orc_grid *g, const double *A, double *p, double rtol,
     double *q = malloc(sizeof(double) * n), *d = calloc(np, sizeof(double));
     double *pc = malloc(sizeof(double) * np);
     if (!r || !z || !q || !d || !pc) { free(r); free(z); free(q); free(d); free(pc); return -1; }
+    const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
     /* r = b - M p using the residual of the reference (ghosts included) */
-    memcpy(pc, p, sizeof(double) * np);
-    apply_bc_lvl(&L, pc);
-    orc_residual_field(g, A, pc, r);
+    double rn = true_residual(g, &L, A, p, pc, r);
     double bn = 0.0;
     for (long c = 0; c < n; ++c) bn += A[7 * n + c] * A[7 * n + c];
     bn = sqrt(bn);
     if (bn == 0.0) bn = 1.0;
-    double rz = 0.0, rn = 0.0;
-    for (long c = 0; c < n; ++c) { z[c] = r[c] / A[6 * n + c]; rz += r[c] * z[c]; rn += r[c] * r[c]; }
+    double rz = 0.0;
+#pragma omp parallel for num_threads(orc_pcg_threads()) reduction(+ : rz) schedule(static)
+    for (long c = 0; c < n; ++c) { z[c] = r[c] / A[6 * n + c]; rz += r[c] * z[c]; }
     int it = 0;
-    for (int k = 0; k < L.n[2]; ++k)
-        for (int j = 0; j < L.n[1]; ++j)
-            for (int i = 0; i < L.n[0]; ++i) d[ip(&L, i, j, k)] = z[ic(&L, i, j, k)];
+#pragma omp parallel for collapse(2) num_threads(orc_pcg_threads()) schedule(static)
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) d[ip(&L, i, j, k)] = z[ic(&L, i, j, k)];
     while (sqrt(rn) / bn > rtol && it < maxit) {
-        op_apply(&L, A, d, q);
-        double dq = 0.0;
-        for (int k = 0; k < L.n[2]; ++k)
-            for (int j = 0; j < L.n[1]; ++j)
-                for (int i = 0; i < L.n[0]; ++i) dq += d[ip(&L, i, j, k)] * q[ic(&L, i, j, k)];
-        double alpha = rz / dq;
-        rn = 0.0;
-        for (int k = 0; k < L.n[2]; ++k)
-            for (int j = 0; j < L.n[1]; ++j)
-                for (int i = 0; i < L.n[0]; ++i) {
+        const double dq = op_apply_dot(&L, A, d, q);
+        const double alpha = rz / dq;
+        double rn2 = 0.0, rz2 = 0.0;
+#pragma omp parallel for collapse(2) num_threads(orc_pcg_threads()) reduction(+ : rn2, rz2) schedule(static)
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
                     long c = ic(&L, i, j, k), qq = ip(&L, i, j, k);
                     p[qq] += alpha * d[qq];
                     r[c] -= alpha * q[c];
-                    rn += r[c] * r[c];
+                    rn2 += r[c] * r[c];
+                    z[c] = r[c] / A[6 * n + c];
+                    rz2 += r[c] * z[c];
                 }
-        double rz2 = 0.0;
-        for (long c = 0; c < n; ++c) { z[c] = r[c] / A[6 * n + c]; rz2 += r[c] * z[c]; }
-        double beta = rz2 / rz;
+        rn = rn2;
+        const double beta = rz2 / rz;
         rz = rz2;
-        for (int k = 0; k < L.n[2]; ++k)
-            for (int j = 0; j < L.n[1]; ++j)
-                for (int i = 0; i < L.n[0]; ++i) {
+#pragma omp parallel for collapse(2) num_threads(orc_pcg_threads()) schedule(static)
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
                     long qq = ip(&L, i, j, k);
                     d[qq] = z[ic(&L, i, j, k)] + beta * d[qq];
                 }
         ++it;
         /* recompute the true residual now and then to avoid drift */
-        if (it % 50 == 0) {
-            memcpy(pc, p, sizeof(double) * np);
-            apply_bc_lvl(&L, pc);
-            orc_residual_field(g, A, pc, r);
-            rn = 0.0;
-            for (long c = 0; c < n; ++c) rn += r[c] * r[c];
-        }
+        if (it % 50 == 0) rn = true_residual(g, &L, A, p, pc, r);
     }
     apply_bc_lvl(&L, p);
-    memcpy(pc, p, sizeof(double) * np);
-    orc_residual_field(g, A, pc, r);
-    rn = 0.0;
-    for (long c = 0; c < n; ++c) rn += r[c] * r[c];
+    rn = true_residual(g, &L, A, p, pc, r);
     if (relres) *relres = sqrt(rn) / bn;
     free(r); free(z); free(q); free(d); free(pc);
     return it;

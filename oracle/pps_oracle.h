@@ -51,13 +51,21 @@ void orc_coords(int N, int ng, double length, double *x, double *xh,
 void orc_pressure_masks(const orc_grid *g, double *umask, double *vmask,
                         double *wmask);
 
-/* SetupPoisson + Poisson_BCs(codes 0..4) + check_and_debug_Poisson
- * (modules.f90:2722-2953).  Fields carry ng ghosts.  Returns ORC_* code. */
+/* SetupPoisson + Poisson_BCs (codes 0..6) + check_and_debug_Poisson
+ * (modules.f90:2722-2953) with BoundaryPressure = 0.  Fields carry ng
+ * ghosts.  Returns ORC_* code. */
 int orc_setup_poisson(const orc_grid *g, const double *u, const double *v,
                       const double *w, const double *umask,
                       const double *vmask, const double *wmask,
                       const double *rho, double dt, double volume_source,
                       double *A);
+/* The same with the module's BoundaryPressure(1:6) (-x, +x, -y, +y, -z, +z,
+ * modules.f90:878-883), which the pressure codes 5 / 6 read. */
+int orc_setup_poisson_bp(const orc_grid *g, const double *u, const double *v,
+                         const double *w, const double *umask,
+                         const double *vmask, const double *wmask,
+                         const double *rho, double dt, double volume_source,
+                         const double bp[6], double *A);
 
 /* check_poisson_setup (newsolver.f90:275-344); rowsum (optional, n cells)
  * receives sum(A1..6) - A7 with A7 := 0 in solid cells. */
