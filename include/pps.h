@@ -59,9 +59,14 @@ enum pps_method {
      * (MGsolver.f90:203-320).  FP64.  Same iteration history as the
      * reference up to summation order. */
     PPS_MG_REFERENCE = 0,
-    /* FP64 multigrid-preconditioned CG: red-black Gauss-Seidel, fused
-     * residual + full-weighting restriction, face-coefficient coarsening,
-     * trilinear prolongation, dense coarsest solve in one workgroup. */
+    /* FP64 multigrid-preconditioned CG: red-black Gauss-Seidel (nu_pre,
+     * nu_post sweeps), residual fused with the 8-child average
+     * restriction, face-coefficient coarsening (a coarse face = 1/16 of the
+     * sum of its 4 fine faces), piecewise-constant prolongation (the
+     * transpose of the restriction, which keeps the V-cycle symmetric for
+     * CG), dense coarsest solve in one workgroup.  pps_options.transfer
+     * selects trilinear prolongation with its transpose (27-point full
+     * weighting) instead. */
     PPS_MG_FP64 = 1,
     /* Same V-cycle in FP32 (FP32 coefficients and accumulation) inside the
      * FP64 outer iteration: residual, update and reductions stay FP64. */
@@ -79,6 +84,19 @@ enum pps_smoother {
 enum pps_outer {
     PPS_OUTER_CG = 0,      /* preconditioned conjugate gradients */
     PPS_OUTER_REFINE = 1   /* plain iterative refinement p += V(b - A p) */
+};
+
+/* Grid transfers of the fast V-cycle.  Both pairs keep R = P^T / 8, so the
+ * V-cycle stays a symmetric operator under CG. */
+enum pps_transfer {
+    /* 8-child average restriction, piecewise-constant prolongation
+     * (default; the prolongation rides in the first post-smoothing sweep) */
+    PPS_TRANSFER_CONSTANT = 0,
+    /* trilinear prolongation (weights 3/4, 1/4 per axis towards the nearer
+     * coarse neighbour; constant extrapolation at walls, wrap on periodic
+     * axes) with its transpose as the restriction: full weighting over the
+     * 4 x 4 x 4 fine cells around each coarse cell */
+    PPS_TRANSFER_TRILINEAR = 1
 };
 
 typedef struct pps_grid {
@@ -105,6 +123,7 @@ typedef struct pps_options {
     int tail_cells;        /* fast V-cycle: levels below the finest with at most
                               this many cells run in one workgroup;
                               0 = automatic (4096), < 0 = never */
+    int transfer;          /* enum pps_transfer (fast methods, default constant) */
 } pps_options;
 
 typedef struct pps_handle pps_handle;
@@ -212,14 +231,17 @@ int pps_adjoint_device(pps_handle *h, const double *dA, const double *dg,
                        void *stream);
 /* Fused face kernel: grad_f = -w_f (lam_L - lam_R)(p_L - p_R) for the
  * +x, +y, +z face of every cell (3 planes of nx*ny*nz), w_f = the
- * geometric weight dt/(dx*dxh) of the face; a face with a closed mask
- * (coefficient 0) gets 0.  wx/wy/wz: per-plane weights (length
- * nx/ny/nz).  This is dJ/dc_f for c_f = 2/(rho_L+rho_R) = "1/rho at the
- * face" (modules.f90:2737-2742). */
-int pps_face_gradient_device(const pps_grid *grid, const double *dA,
-                             const double *dp, const double *dlam,
-                             const double *wx, const double *wy,
-                             const double *wz, double *dgrad, void *stream);
+ * geometric weight dt/(dx*dxh) of the face; the last face of a wall axis
+ * (which couples to no cell) gets 0.  wx/wy/wz: per-plane weights (length
+ * nx/ny/nz); dp, dlam padded like p (periodic neighbours are wrapped here,
+ * ghosts are not read).  This is dJ/dc_f for the map of
+ * pps_faces_to_A_device (A = w c), with c_f = 2/(rho_L+rho_R) = "1/rho at
+ * the face" (modules.f90:2737-2742); a caller whose couplings also carry
+ * SetupPoisson's masks multiplies the result by them. */
+int pps_face_gradient_device(const pps_grid *grid, const double *dp,
+                             const double *dlam, const double *wx,
+                             const double *wy, const double *wz,
+                             double *dgrad, void *stream);
 /* The linear map from face values to the 8-slot system that the face
  * gradient differentiates: with c = 3 planes (+x, +y, +z face of every
  * cell) and per-plane weights w, A2(i) = A1(i+1) = w_x(i) c_x(i) (wrapping
@@ -247,14 +269,28 @@ int pps_linfunc_device(const pps_grid *grid, const double *cvof, double a1, doub
 /* SetPressureBC on all-ones masks (modules.f90:916-946), padded arrays. */
 int pps_pressure_masks_device(const pps_grid *grid, double *umask,
                               double *vmask, double *wmask, void *stream);
-/* SetupPoisson + Poisson_BCs codes 0..4 + the solid-cell rule of
+/* SetupPoisson + Poisson_BCs (codes 0..6) + the solid-cell rule of
  * check_and_debug_Poisson (modules.f90:2722-2953), padded fields in,
- * 8-slot A out. */
+ * 8-slot A out; BoundaryPressure = 0. */
 int pps_setup_poisson_device(const pps_grid *grid, const double *u,
                              const double *v, const double *w,
                              const double *umask, const double *vmask,
                              const double *wmask, const double *rho, double dt,
                              double volume_source, double *A, void *stream);
+/* The same with the reference's BoundaryPressure(1:6) (module_BC,
+ * modules.f90:878-883; -x, +x, -y, +y, -z, +z), a host array of 6 doubles,
+ * which the pressure codes read: code 5 makes the face row
+ * P_0 - P_inner / 3 = 2/3 P_b, code 6 (x+) sets p and du/dn at the face
+ * (modules.f90:2768-2891).  Rows of codes 5 / 6 are not symmetric: only
+ * PPS_MG_REFERENCE and PPS_SOR solve them (the reference switches
+ * check_setup off for them, paris.f90:2992). */
+int pps_setup_poisson_bp_device(const pps_grid *grid, const double *u,
+                                const double *v, const double *w,
+                                const double *umask, const double *vmask,
+                                const double *wmask, const double *rho,
+                                double dt, double volume_source,
+                                const double *boundary_pressure, double *A,
+                                void *stream);
 /* The same on a stretched grid (xform, read_x: paris.f90:3056-3102): the
  * caller's dx, dxh, dy, dyh, dz, dzh (host arrays with ghosts, n + 2 ng
  * values each, the reference's dx(imin:imax) ...) replace the uniform
@@ -268,8 +304,17 @@ int pps_setup_poisson_coords_device(const pps_grid *grid, const double *u,
                                     const double *dy, const double *dyh,
                                     const double *dz, const double *dzh,
                                     double *A, void *stream);
-/* check_poisson_setup (newsolver.f90:275-344); solid cells are those with
- * six zero couplings. */
+/* check_poisson_setup (newsolver.f90:275-344) with the face masks, as the
+ * reference reads them: solid cells (A7 := 0 in the row-sum test) are those
+ * whose six face masks are closed (< 1/2).  Returns PPS_OK or the first
+ * failed test: PPS_ERR_NAN_OR_TINY_DIAG, PPS_ERR_VOLUME_SOURCE,
+ * PPS_ERR_NOT_IN_KERNEL. */
+int pps_check_poisson_setup_device(const pps_grid *grid, const double *A,
+                                   const double *umask, const double *vmask,
+                                   const double *wmask, void *stream);
+/* The same without masks (what the solvers' check_setup option runs on A
+ * alone): solid cells are those with six zero couplings, the same cells
+ * when A comes from SetupPoisson. */
 int pps_check_setup_device(const pps_grid *grid, const double *A, void *stream);
 /* Subtract the mean of A8 (compatibility projection of the right-hand side). */
 int pps_remove_rhs_mean_device(const pps_grid *grid, double *A, void *stream);

@@ -50,6 +50,9 @@ OUTER_CG = 0
 OUTER_REFINE = 1
 SMOOTH_FUSED = 0
 SMOOTH_COLOUR_PASSES = 1
+TRANSFER_CONSTANT = 0
+TRANSFER_TRILINEAR = 1
+TRANSFERS = {"constant": TRANSFER_CONSTANT, "trilinear": TRANSFER_TRILINEAR}
 
 METHODS = {"reference": MG_REFERENCE, "fp64": MG_FP64, "mixed": MG_MIXED, "sor": SOR}
 
@@ -111,7 +114,7 @@ class Options(ctypes.Structure):
                 ("check_setup", ctypes.c_int), ("multigrid", ctypes.c_int),
                 ("smoother", ctypes.c_int), ("sweep_rows", ctypes.c_int),
                 ("sweep_planes", ctypes.c_int),
-                ("tail_cells", ctypes.c_int)]
+                ("tail_cells", ctypes.c_int), ("transfer", ctypes.c_int)]
 
 
 _LIBH = None
@@ -146,13 +149,15 @@ def _lib():
         "pps_last_history": (_I, [_P, _DP, _I]),
         "pps_last_timing": (_I, [_P, _DP, _DP]),
         "pps_adjoint_device": (_I, [_P, _P, _P, _P, _D, _I, _IP, _P]),
-        "pps_face_gradient_device": (_I, [_GP, _P, _P, _P, _P, _P, _P, _P, _P]),
+        "pps_face_gradient_device": (_I, [_GP, _P, _P, _P, _P, _P, _P, _P]),
         "pps_pressure_masks_device": (_I, [_GP, _P, _P, _P, _P]),
         "pps_linfunc_device": (_I, [_GP, _P, _D, _D, _I, _I, _P, _P]),
         "pps_setup_poisson_device": (_I, [_GP, _P, _P, _P, _P, _P, _P, _P, _D, _D, _P, _P]),
+        "pps_setup_poisson_bp_device": (_I, [_GP, _P, _P, _P, _P, _P, _P, _P, _D, _D, _P, _P, _P]),
         "pps_setup_poisson_coords_device": (_I, [_GP, _P, _P, _P, _P, _P, _P, _P, _D, _D, _P, _P, _P,
                                                  _P, _P, _P, _P, _P]),
         "pps_check_setup_device": (_I, [_GP, _P, _P]),
+        "pps_check_poisson_setup_device": (_I, [_GP, _P, _P, _P, _P, _P]),
         "pps_remove_rhs_mean_device": (_I, [_GP, _P, _P]),
         "pps_calc_residuals_device": (_I, [_GP, _P, _P, _DP, _P]),
         "pps_project_velocity_device": (_I, [_GP, _P, _P, _P, _P, _P, _P, _P, _P, _D, _P]),
@@ -186,8 +191,9 @@ def exported_symbols():
         "pps_new_solver", "pps_new_solver_mg", "pps_solve_device", "pps_solve_project_device",
         "pps_last_history",
         "pps_last_timing", "pps_adjoint_device", "pps_face_gradient_device",
-        "pps_pressure_masks_device", "pps_linfunc_device", "pps_setup_poisson_device", "pps_setup_poisson_coords_device",
-        "pps_check_setup_device",
+        "pps_pressure_masks_device", "pps_linfunc_device", "pps_setup_poisson_device",
+        "pps_setup_poisson_bp_device", "pps_setup_poisson_coords_device",
+        "pps_check_setup_device", "pps_check_poisson_setup_device",
         "pps_remove_rhs_mean_device", "pps_calc_residuals_device",
         "pps_project_velocity_device", "pps_project_velocity_coords_device",
         "pps_synth_fields_device", "pps_synth_drops",
@@ -203,29 +209,44 @@ def _check(rc, what, allow=(OK,)):
     return rc
 
 
-def _ptr(t):
-    """Device (or host) address of a contiguous torch tensor / numpy array."""
+def _ptr(t, n=None, what="array"):
+    """Device (or host) address of a contiguous float64 torch tensor / numpy
+    array; with n given, the array must hold exactly n elements (the
+    kernels index the extents the grid implies, so a wrong size would read
+    or write past the allocation)."""
     if isinstance(t, np.ndarray):
         if not (t.flags["C_CONTIGUOUS"] or t.flags["F_CONTIGUOUS"]):
-            raise ValueError("array must be contiguous")
+            raise ValueError(f"{what} must be contiguous")
         if t.dtype != np.float64:
-            raise TypeError("array must be float64")
-        return ctypes.c_void_p(t.ctypes.data)
-    if not t.is_contiguous():
-        raise ValueError("tensor must be contiguous")
-    return ctypes.c_void_p(t.data_ptr())
-
-
-def _iptr(t):
-    """Address of a contiguous int32 numpy array / torch tensor (masks)."""
-    if isinstance(t, np.ndarray):
-        if t.dtype != np.int32 or not t.flags["C_CONTIGUOUS"]:
-            raise TypeError("mask must be a C-contiguous int32 array")
+            raise TypeError(f"{what} must be float64, got {t.dtype}")
+        if n is not None and t.size != n:
+            raise ValueError(f"{what} must hold {n} elements, got {t.size}")
         return ctypes.c_void_p(t.ctypes.data)
     import torch
-    if t.dtype != torch.int32 or not t.is_contiguous():
-        raise TypeError("mask must be a contiguous int32 tensor")
+    if t.dtype != torch.float64:
+        raise TypeError(f"{what} must be a float64 tensor, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{what} must be contiguous")
+    if n is not None and t.numel() != n:
+        raise ValueError(f"{what} must hold {n} elements, got {t.numel()}")
     return ctypes.c_void_p(t.data_ptr())
+
+
+def _iptr(t, n=None, what="mask"):
+    """Address of a contiguous int32 numpy array / torch tensor (masks) of
+    n elements when n is given."""
+    if isinstance(t, np.ndarray):
+        if t.dtype != np.int32 or not t.flags["C_CONTIGUOUS"]:
+            raise TypeError(f"{what} must be a C-contiguous int32 array")
+        size = t.size
+    else:
+        import torch
+        if t.dtype != torch.int32 or not t.is_contiguous():
+            raise TypeError(f"{what} must be a contiguous int32 tensor")
+        size = t.numel()
+    if n is not None and size != n:
+        raise ValueError(f"{what} must hold {n} elements, got {size}")
+    return ctypes.c_void_p(t.data_ptr() if not isinstance(t, np.ndarray) else t.ctypes.data)
 
 
 def _stream(stream=None):
@@ -235,9 +256,37 @@ def _stream(stream=None):
 
 
 def _need_gpu(*ts):
+    import torch
     for t in ts:
-        if isinstance(t, np.ndarray) or not t.is_cuda or t.dtype.is_floating_point is False:
+        if isinstance(t, np.ndarray) or not t.is_cuda or t.dtype != torch.float64:
             raise TypeError("expected a CUDA (HIP) float64 tensor")
+
+
+def _nA(grid):
+    return 8 * grid.cells
+
+
+def _np(grid):
+    z, y, x = grid.shape_p
+    return z * y * x
+
+
+def _padded_ptrs(grid, **arrays):
+    """Pointers of padded float64 arrays (shape of p), in keyword order."""
+    n = _np(grid)
+    return [_ptr(a, n, name) for name, a in arrays.items()]
+
+
+def _spacing_arrays(grid, dx, dxh, dy, dyh, dz, dzh):
+    """The reference's spacing arrays with ghosts (n + 2 ng values per
+    axis) as contiguous float64 numpy arrays, sizes checked."""
+    out = []
+    for a, m, name in ((dx, grid.nx, "dx"), (dxh, grid.nx, "dxh"), (dy, grid.ny, "dy"),
+                       (dyh, grid.ny, "dyh"), (dz, grid.nz, "dz"), (dzh, grid.nz, "dzh")):
+        arr = np.ascontiguousarray(a, dtype=np.float64)
+        _ptr(arr, m + 2 * grid.ng, name)
+        out.append(arr)
+    return out
 
 
 def default_options():
@@ -251,7 +300,7 @@ class Solver:
 
     def __init__(self, grid, method="fp64", outer=OUTER_CG, nrelax=3, sym_mg=0, nu_pre=2,
                  nu_post=2, check_setup=0, multigrid=1, smoother=SMOOTH_FUSED, sweep_rows=0,
-                 sweep_planes=0, tail_cells=0):
+                 sweep_planes=0, tail_cells=0, transfer=TRANSFER_CONSTANT):
         self.grid = grid
         o = default_options()
         o.method = METHODS[method] if isinstance(method, str) else int(method)
@@ -261,6 +310,7 @@ class Solver:
         o.smoother = int(smoother)
         o.sweep_rows, o.sweep_planes = int(sweep_rows), int(sweep_planes)
         o.tail_cells = int(tail_cells)
+        o.transfer = TRANSFERS[transfer] if isinstance(transfer, str) else int(transfer)
         self.options = o
         h = ctypes.c_void_p()
         _check(_lib().pps_init(ctypes.byref(grid), ctypes.byref(o), ctypes.byref(h)), "pps_init")
@@ -295,14 +345,18 @@ class Solver:
             _check(_lib().pps_set_cell_widths(self._h, None, None, None), "pps_set_cell_widths")
             return
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (dx, dy, dz)]
-        _check(_lib().pps_set_cell_widths(self._h, *[_ptr(a) for a in arrs]), "pps_set_cell_widths")
+        g = self.grid
+        ptrs = [_ptr(a, m, nm) for a, m, nm in zip(arrs, (g.nx, g.ny, g.nz), ("dx", "dy", "dz"))]
+        _check(_lib().pps_set_cell_widths(self._h, *ptrs), "pps_set_cell_widths")
 
     # -- reference signatures on host arrays ---------------------------
     def new_solver(self, A, p, maxError, beta=1.2, maxit=50, norm=2):
         """NewSolver: p (host, float64, padded) is updated in place.
         Returns (it, ierr); it == maxit+1 when not converged."""
         it, ierr = ctypes.c_int(), ctypes.c_int()
-        rc = _lib().pps_new_solver(self._h, _ptr(A), _ptr(p), maxError, beta, maxit,
+        g = self.grid
+        rc = _lib().pps_new_solver(self._h, _ptr(A, _nA(g), "A"), _ptr(p, _np(g), "p"), maxError,
+                                   beta, maxit,
                                    ctypes.byref(it), ctypes.byref(ierr), norm)
         _check(rc, "NewSolver", allow=(OK, ERR_NOT_CONVERGED))
         return it.value, ierr.value
@@ -310,7 +364,9 @@ class Solver:
     def new_solver_mg(self, A, p, maxError, beta=1.2, maxit=50, norm=2):
         """NewSolverMG: returns (it, ierr, tres2)."""
         it, ierr, t2 = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
-        rc = _lib().pps_new_solver_mg(self._h, _ptr(A), _ptr(p), maxError, beta, maxit,
+        g = self.grid
+        rc = _lib().pps_new_solver_mg(self._h, _ptr(A, _nA(g), "A"), _ptr(p, _np(g), "p"), maxError,
+                                      beta, maxit,
                                       ctypes.byref(it), ctypes.byref(ierr), norm,
                                       ctypes.byref(t2))
         _check(rc, "NewSolverMG", allow=(OK, ERR_NOT_CONVERGED))
@@ -321,8 +377,11 @@ class Solver:
         is updated in place; pcmask (int32) and P_gas (float64, or None for
         solver_flag 2) are padded like p.  Returns (it, ierr, tres2)."""
         it, ierr, t2 = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
-        rc = _lib().pps_free_solver(self._h, _ptr(A), _ptr(p), _iptr(pcmask),
-                                    _ptr(P_gas) if P_gas is not None else None, solver_flag,
+        g = self.grid
+        rc = _lib().pps_free_solver(self._h, _ptr(A, _nA(g), "A"), _ptr(p, _np(g), "p"),
+                                    _iptr(pcmask, _np(g), "pcmask"),
+                                    _ptr(P_gas, _np(g), "P_gas") if P_gas is not None else None,
+                                    solver_flag,
                                     maxError, beta, maxit, ctypes.byref(it), ctypes.byref(ierr),
                                     ctypes.byref(t2))
         _check(rc, "FreeSolver", allow=(OK, ERR_NOT_CONVERGED))
@@ -336,8 +395,10 @@ class Solver:
         if not pcmask.is_cuda:
             raise TypeError("pcmask must be a CUDA (HIP) int32 tensor")
         it, t2 = ctypes.c_int(), ctypes.c_double()
-        rc = _lib().pps_free_solver_device(self._h, _ptr(A), _ptr(p), _iptr(pcmask),
-                                           _ptr(P_gas) if P_gas is not None else None,
+        g = self.grid
+        rc = _lib().pps_free_solver_device(self._h, _ptr(A, _nA(g), "A"), _ptr(p, _np(g), "p"),
+                                           _iptr(pcmask, _np(g), "pcmask"),
+                                           _ptr(P_gas, _np(g), "P_gas") if P_gas is not None else None,
                                            solver_flag, maxError, beta, maxit, ctypes.byref(it),
                                            ctypes.byref(t2), _stream(stream))
         _check(rc, "pps_free_solver_device", allow=(OK, ERR_NOT_CONVERGED))
@@ -347,7 +408,9 @@ class Solver:
         """poi_solve (modules.f90:2411) on host arrays: relative tolerance
         ||r||/||A8|| < maxError; returns num_iterations."""
         n = ctypes.c_int()
-        rc = _lib().pps_poi_solve(self._h, _ptr(A), _ptr(p), maxError, maxit, ctypes.byref(n),
+        g = self.grid
+        rc = _lib().pps_poi_solve(self._h, _ptr(A, _nA(g), "A"), _ptr(p, _np(g), "p"), maxError,
+                                  maxit, ctypes.byref(n),
                                   solver_type)
         _check(rc, "poi_solve", allow=(OK, ERR_NOT_CONVERGED))
         return n.value
@@ -356,7 +419,9 @@ class Solver:
         """poi_solve on device tensors; returns num_iterations."""
         _need_gpu(A, p)
         n = ctypes.c_int()
-        rc = _lib().pps_poi_solve_device(self._h, _ptr(A), _ptr(p), maxError, maxit,
+        g = self.grid
+        rc = _lib().pps_poi_solve_device(self._h, _ptr(A, _nA(g), "A"), _ptr(p, _np(g), "p"),
+                                         maxError, maxit,
                                          ctypes.byref(n), solver_type, _stream(stream))
         _check(rc, "pps_poi_solve_device", allow=(OK, ERR_NOT_CONVERGED))
         return n.value
@@ -366,7 +431,9 @@ class Solver:
         """Solve in place on device tensors; returns (it, tres2)."""
         _need_gpu(A, p)
         it, t2 = ctypes.c_int(), ctypes.c_double()
-        rc = _lib().pps_solve_device(self._h, _ptr(A), _ptr(p), maxError, beta, maxit, norm,
+        g = self.grid
+        rc = _lib().pps_solve_device(self._h, _ptr(A, _nA(g), "A"), _ptr(p, _np(g), "p"), maxError,
+                                     beta, maxit, norm,
                                      ctypes.byref(it), ctypes.byref(t2), _stream(stream))
         _check(rc, "pps_solve_device",
                allow=(OK, ERR_NOT_CONVERGED) if allow_maxit else (OK,))
@@ -381,10 +448,14 @@ class Solver:
         _need_gpu(A, p, u, v, w, umask, vmask, wmask, rho)
         uo, vo, wo = torch.empty_like(u), torch.empty_like(v), torch.empty_like(w)
         it, t2 = ctypes.c_int(), ctypes.c_double()
+        g = self.grid
+        npad = _np(g)
         rc = _lib().pps_solve_project_device(
-            self._h, _ptr(A), _ptr(p), maxError, beta, maxit, norm, ctypes.byref(it),
-            ctypes.byref(t2), _ptr(u), _ptr(v), _ptr(w), _ptr(umask), _ptr(vmask), _ptr(wmask),
-            _ptr(rho), dt, _ptr(uo), _ptr(vo), _ptr(wo), _stream(stream))
+            self._h, _ptr(A, _nA(g), "A"), _ptr(p, npad, "p"), maxError, beta, maxit, norm,
+            ctypes.byref(it), ctypes.byref(t2), *[_ptr(a, npad, nm) for a, nm in (
+                (u, "u"), (v, "v"), (w, "w"), (umask, "umask"), (vmask, "vmask"), (wmask, "wmask"),
+                (rho, "rho"))],
+            dt, _ptr(uo), _ptr(vo), _ptr(wo), _stream(stream))
         _check(rc, "pps_solve_project_device",
                allow=(OK, ERR_NOT_CONVERGED) if allow_maxit else (OK,))
         return it.value, t2.value, uo, vo, wo
@@ -394,7 +465,9 @@ class Solver:
         widths are set); returns iterations."""
         _need_gpu(A, g, lam)
         it = ctypes.c_int()
-        rc = _lib().pps_adjoint_device(self._h, _ptr(A), _ptr(g), _ptr(lam), rtol, maxit,
+        gr = self.grid
+        rc = _lib().pps_adjoint_device(self._h, _ptr(A, _nA(gr), "A"), _ptr(g, _np(gr), "g"),
+                                       _ptr(lam, _np(gr), "lam"), rtol, maxit,
                                        ctypes.byref(it), _stream(stream))
         _check(rc, "pps_adjoint_device")
         return it.value
@@ -461,7 +534,8 @@ def linfunc(grid, cvof, a1, a2, mean_flag=ARITH_MEAN, nfilter=0):
     import torch
     _need_gpu(cvof)
     out = torch.empty(grid.shape_p, dtype=torch.float64, device=cvof.device)
-    _check(_lib().pps_linfunc_device(ctypes.byref(grid), _ptr(cvof), float(a1), float(a2),
+    _check(_lib().pps_linfunc_device(ctypes.byref(grid), _ptr(cvof, _np(grid), "cvof"), float(a1),
+                                     float(a2),
                                      int(mean_flag), int(nfilter), _ptr(out), _stream()),
            "pps_linfunc_device")
     return out
@@ -475,13 +549,20 @@ def pressure_masks(grid, device="cuda"):
     return um, vm, wm
 
 
-def setup_poisson(grid, u, v, w, um, vm, wm, rho, dt=1.0, volume_source=0.0):
-    """SetupPoisson + Poisson_BCs: the 8-slot system (device tensor)."""
+def setup_poisson(grid, u, v, w, um, vm, wm, rho, dt=1.0, volume_source=0.0,
+                  boundary_pressure=None):
+    """SetupPoisson + Poisson_BCs: the 8-slot system (device tensor).
+    boundary_pressure: the reference's BoundaryPressure(1:6) (-x, +x, -y,
+    +y, -z, +z) read by the pressure codes 5 / 6; None means zeros."""
     import torch
+    _need_gpu(u, v, w, um, vm, wm, rho)
     A = torch.empty(grid.shape_A, dtype=torch.float64, device=rho.device)
-    _check(_lib().pps_setup_poisson_device(ctypes.byref(grid), _ptr(u), _ptr(v), _ptr(w),
-                                           _ptr(um), _ptr(vm), _ptr(wm), _ptr(rho), dt,
-                                           volume_source, _ptr(A), _stream()),
+    fields = _padded_ptrs(grid, u=u, v=v, w=w, umask=um, vmask=vm, wmask=wm, rho=rho)
+    bp = np.zeros(6) if boundary_pressure is None else np.ascontiguousarray(
+        boundary_pressure, dtype=np.float64)
+    _check(_lib().pps_setup_poisson_bp_device(ctypes.byref(grid), *fields, dt,
+                                              volume_source, _ptr(bp, 6, "boundary_pressure"),
+                                              _ptr(A), _stream()),
            "pps_setup_poisson_device")
     return A
 
@@ -491,48 +572,62 @@ def setup_poisson_coords(grid, u, v, w, um, vm, wm, rho, dx, dxh, dy, dyh, dz, d
     """SetupPoisson on a stretched grid: dx ... dzh are the reference's
     spacing arrays with ghosts (numpy, n + 2 ng values each)."""
     import torch
+    _need_gpu(u, v, w, um, vm, wm, rho)
     A = torch.empty(grid.shape_A, dtype=torch.float64, device=rho.device)
-    sp = [np.ascontiguousarray(a, dtype=np.float64) for a in (dx, dxh, dy, dyh, dz, dzh)]
-    _check(_lib().pps_setup_poisson_coords_device(ctypes.byref(grid), _ptr(u), _ptr(v), _ptr(w),
-                                                  _ptr(um), _ptr(vm), _ptr(wm), _ptr(rho), dt,
-                                                  volume_source, *[_ptr(a) for a in sp], _ptr(A),
-                                                  _stream()),
+    fields = _padded_ptrs(grid, u=u, v=v, w=w, umask=um, vmask=vm, wmask=wm, rho=rho)
+    sp = _spacing_arrays(grid, dx, dxh, dy, dyh, dz, dzh)
+    _check(_lib().pps_setup_poisson_coords_device(ctypes.byref(grid), *fields, dt,
+                                                  volume_source, *[_ptr(a) for a in sp],
+                                                  _ptr(A), _stream()),
            "pps_setup_poisson_coords_device")
     return A
 
 
 def remove_rhs_mean(grid, A):
-    _check(_lib().pps_remove_rhs_mean_device(ctypes.byref(grid), _ptr(A), _stream()),
+    _need_gpu(A)
+    _check(_lib().pps_remove_rhs_mean_device(ctypes.byref(grid), _ptr(A, _nA(grid), "A"), _stream()),
            "pps_remove_rhs_mean_device")
     return A
 
 
-def check_setup(grid, A):
-    """check_poisson_setup: returns the status code (0 = well posed)."""
-    return _lib().pps_check_setup_device(ctypes.byref(grid), _ptr(A), _stream())
+def check_setup(grid, A, umask=None, vmask=None, wmask=None):
+    """check_poisson_setup: returns the status code (0 = well posed).  With
+    the face masks, solid cells are read from them as the reference does;
+    without, they are the cells with six zero couplings."""
+    _need_gpu(A)
+    if umask is None and vmask is None and wmask is None:
+        return _lib().pps_check_setup_device(ctypes.byref(grid), _ptr(A, _nA(grid), "A"), _stream())
+    _need_gpu(umask, vmask, wmask)
+    return _lib().pps_check_poisson_setup_device(
+        ctypes.byref(grid), _ptr(A, _nA(grid), "A"),
+        *_padded_ptrs(grid, umask=umask, vmask=vmask, wmask=wmask), _stream())
 
 
 def calc_residuals(grid, A, p):
     """calcResiduals: (L1, L2, Linf)."""
+    _need_gpu(A, p)
     out = (ctypes.c_double * 3)()
-    _check(_lib().pps_calc_residuals_device(ctypes.byref(grid), _ptr(A), _ptr(p), out, _stream()),
+    _check(_lib().pps_calc_residuals_device(ctypes.byref(grid), _ptr(A, _nA(grid), "A"),
+                                            _ptr(p, _np(grid), "p"), out, _stream()),
            "pps_calc_residuals_device")
     return tuple(out)
 
 
 def project_velocity(grid, u, v, w, um, vm, wm, rho, p, dt=1.0):
-    _check(_lib().pps_project_velocity_device(ctypes.byref(grid), _ptr(u), _ptr(v), _ptr(w),
-                                              _ptr(um), _ptr(vm), _ptr(wm), _ptr(rho), _ptr(p),
-                                              dt, _stream()), "pps_project_velocity_device")
+    _need_gpu(u, v, w, um, vm, wm, rho, p)
+    fields = _padded_ptrs(grid, u=u, v=v, w=w, umask=um, vmask=vm, wmask=wm, rho=rho, p=p)
+    _check(_lib().pps_project_velocity_device(ctypes.byref(grid), *fields, dt, _stream()),
+           "pps_project_velocity_device")
 
 
 def project_velocity_coords(grid, u, v, w, um, vm, wm, rho, p, dx, dxh, dy, dyh, dz, dzh, dt=1.0):
     """project_velocity on a stretched grid (in place on u, v, w); dx ... dzh
     are the reference's spacing arrays with ghosts (numpy)."""
-    sp = [np.ascontiguousarray(a, dtype=np.float64) for a in (dx, dxh, dy, dyh, dz, dzh)]
-    _check(_lib().pps_project_velocity_coords_device(ctypes.byref(grid), _ptr(u), _ptr(v), _ptr(w),
-                                                     _ptr(um), _ptr(vm), _ptr(wm), _ptr(rho), _ptr(p),
-                                                     dt, *[_ptr(a) for a in sp], _stream()),
+    _need_gpu(u, v, w, um, vm, wm, rho, p)
+    fields = _padded_ptrs(grid, u=u, v=v, w=w, umask=um, vmask=vm, wmask=wm, rho=rho, p=p)
+    sp = _spacing_arrays(grid, dx, dxh, dy, dyh, dz, dzh)
+    _check(_lib().pps_project_velocity_coords_device(ctypes.byref(grid), *fields, dt,
+                                                     *[_ptr(a) for a in sp], _stream()),
            "pps_project_velocity_coords_device")
 
 
@@ -547,13 +642,21 @@ def face_weights(grid, dt=1.0, device="cuda"):
     return tuple(out)
 
 
-def face_gradient(grid, A, p, lam, weights):
-    """dJ/dc_f of the +x, +y, +z face of every cell: (3, nz, ny, nx)."""
-    import torch
-    g = torch.empty((3, grid.nz, grid.ny, grid.nx), dtype=torch.float64, device=A.device)
+def _face_weight_ptrs(grid, weights):
     wx, wy, wz = weights
-    _check(_lib().pps_face_gradient_device(ctypes.byref(grid), _ptr(A), _ptr(p), _ptr(lam),
-                                           _ptr(wx), _ptr(wy), _ptr(wz), _ptr(g), _stream()),
+    _need_gpu(wx, wy, wz)
+    return [_ptr(wx, grid.nx, "wx"), _ptr(wy, grid.ny, "wy"), _ptr(wz, grid.nz, "wz")]
+
+
+def face_gradient(grid, p, lam, weights):
+    """dJ/dc_f of the +x, +y, +z face of every cell for A = w c (the map of
+    faces_to_A): (3, nz, ny, nx); wall faces get 0."""
+    import torch
+    _need_gpu(p, lam)
+    g = torch.empty((3, grid.nz, grid.ny, grid.nx), dtype=torch.float64, device=p.device)
+    _check(_lib().pps_face_gradient_device(ctypes.byref(grid), _ptr(p, _np(grid), "p"),
+                                           _ptr(lam, _np(grid), "lam"),
+                                           *_face_weight_ptrs(grid, weights), _ptr(g), _stream()),
            "pps_face_gradient_device")
     return g
 
@@ -561,10 +664,11 @@ def face_gradient(grid, A, p, lam, weights):
 def faces_to_A(grid, cface, weights, b):
     """8-slot system from face values c (3, nz, ny, nx) and an interior RHS."""
     import torch
+    _need_gpu(cface, b)
     A = torch.empty(grid.shape_A, dtype=torch.float64, device=cface.device)
-    wx, wy, wz = weights
-    _check(_lib().pps_faces_to_A_device(ctypes.byref(grid), _ptr(cface), _ptr(wx), _ptr(wy),
-                                        _ptr(wz), _ptr(b), _ptr(A), _stream()),
+    _check(_lib().pps_faces_to_A_device(ctypes.byref(grid), _ptr(cface, 3 * grid.cells, "cface"),
+                                        *_face_weight_ptrs(grid, weights),
+                                        _ptr(b, grid.cells, "b"), _ptr(A), _stream()),
            "pps_faces_to_A_device")
     return A
 
@@ -646,7 +750,7 @@ def _autograd():
             grid.interior(g).copy_(grad_p)
             lam = torch.empty_like(g)
             solver.adjoint(A, g, lam, rtol=ctx.tol, maxit=ctx.maxit)
-            gc = face_gradient(grid, A, p, lam, ctx.weights)
+            gc = face_gradient(grid, p, lam, ctx.weights)
             return gc, grid.interior(lam).clone(), None, None, None, None
 
     return PressureSolve

@@ -6,8 +6,10 @@
 // M_LL += w_f c_f, M_RR += w_f c_f, M_LR = M_RL -= w_f c_f.  For any
 // objective J(p) with adjoint M lam = dJ/dp, dJ/dc_f = -lam^T (dM/dc_f) p
 //   = -w_f (lam_L - lam_R) (p_L - p_R).
-// One thread per cell writes the gradient of its +x, +y, +z faces; faces
-// with a closed mask (zero coupling) and wall faces get 0.
+// The gradient is that of the map k_faces_to_A (A = w c, no masks): a
+// face with c_f = 0 still has a gradient; only wall faces (the last face of
+// a non-periodic axis, which couples to no cell) get 0.  A caller whose
+// system also carries masks (SetupPoisson's umask, ...) multiplies by them.
 #include "pps_internal.h"
 
 namespace pps {
@@ -17,37 +19,48 @@ struct FaceK {
     int px, py, pz;
 };
 
+// Threads own (i, j) columns of 64 x 4 cells and march through zc planes;
+// p and lam of the +z plane are loaded once and carried to the next step,
+// so each plane of p and lam is fetched from HBM once (the +x / +y
+// neighbours are the same wavefront's or the next row's lines, served by
+// the caches).  Algorithmic bytes: p, lam read (16 B), three gradient
+// planes written (24 B) per cell.
 __global__ void __launch_bounds__(256)
-k_face_grad(FaceK d, const double *__restrict__ A, const double *__restrict__ p,
-            const double *__restrict__ lam, const double *__restrict__ wx,
-            const double *__restrict__ wy, const double *__restrict__ wz,
-            double *__restrict__ g) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
+k_face_grad(FaceK d, int zc, const double *__restrict__ p, const double *__restrict__ lam,
+            const double *__restrict__ wx, const double *__restrict__ wy,
+            const double *__restrict__ wz, double *__restrict__ g) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int k0 = blockIdx.z * zc;
+    const int k1 = min(k0 + zc, d.nz);
     if (i >= d.nx || j >= d.ny) return;
     const long long n = (long long)d.nx * d.ny * d.nz;
-    const long long c = i + (long long)d.nx * (j + (long long)d.ny * k);
     const long long Py = d.X, Pz = (long long)d.X * d.Y;
-    const long long q = (i + d.ng) + Py * (j + d.ng) + Pz * (k + d.ng);
-    const double pc = p[q], lc = lam[q];
-    // +x face
-    double gx = 0.0, gy = 0.0, gz = 0.0;
-    if (A[n + c] != 0.0 && (i < d.nx - 1 || d.px)) {
-        const long long e = i < d.nx - 1 ? q + 1 : q - (d.nx - 1);
-        gx = -wx[i] * (lc - lam[e]) * (pc - p[e]);
+    const long long col = (i + d.ng) + Py * (j + d.ng);
+    // +x / +y neighbour offsets (wrapped on the last cell of a periodic
+    // axis; unused on a wall face)
+    const bool ox = i < d.nx - 1 || d.px, oy = j < d.ny - 1 || d.py;
+    const long long ex = i < d.nx - 1 ? 1 : -(long long)(d.nx - 1);
+    const long long ey = j < d.ny - 1 ? Py : -(long long)(d.ny - 1) * Py;
+    const double gxw = ox ? -wx[i] : 0.0, gyw = oy ? -wy[j] : 0.0;
+    const long long cc = i + (long long)d.nx * j, sz = (long long)d.nx * d.ny;
+    long long q = col + Pz * (k0 + d.ng);
+    double pc = p[q], lc = lam[q];
+    for (int k = k0; k < k1; ++k) {
+        // +z neighbour: the next plane, or the first one (periodic z)
+        const bool oz = k < d.nz - 1 || d.pz;
+        const long long qz = k < d.nz - 1 ? q + Pz : q - (long long)(d.nz - 1) * Pz;
+        const double pz = p[qz], lz = lam[qz];
+        const double pxe = p[q + ex], lxe = lam[q + ex];
+        const double pye = p[q + ey], lye = lam[q + ey];
+        const long long c = cc + sz * k;
+        g[c] = gxw * (lc - lxe) * (pc - pxe);
+        g[n + c] = gyw * (lc - lye) * (pc - pye);
+        g[2 * n + c] = oz ? -wz[k] * (lc - lz) * (pc - pz) : 0.0;
+        pc = pz;
+        lc = lz;
+        q += Pz;
     }
-    if (A[3 * n + c] != 0.0 && (j < d.ny - 1 || d.py)) {
-        const long long e = j < d.ny - 1 ? q + Py : q - (long long)(d.ny - 1) * Py;
-        gy = -wy[j] * (lc - lam[e]) * (pc - p[e]);
-    }
-    if (A[5 * n + c] != 0.0 && (k < d.nz - 1 || d.pz)) {
-        const long long e = k < d.nz - 1 ? q + Pz : q - (long long)(d.nz - 1) * Pz;
-        gz = -wz[k] * (lc - lam[e]) * (pc - p[e]);
-    }
-    g[c] = gx;
-    g[n + c] = gy;
-    g[2 * n + c] = gz;
 }
 
 // face values -> 8-slot system (the map k_face_grad differentiates)
@@ -99,17 +112,25 @@ extern "C" int pps_faces_to_A_device(const pps_grid *grid, const double *cface, 
     return PPS_OK;
 }
 
-extern "C" int pps_face_gradient_device(const pps_grid *grid, const double *dA, const double *dp,
-                                        const double *dlam, const double *wx, const double *wy,
-                                        const double *wz, double *dgrad, void *stream) {
-    if (!grid || !dA || !dp || !dlam || !wx || !wy || !wz || !dgrad) return PPS_ERR_BAD_ARG;
+extern "C" int pps_face_gradient_device(const pps_grid *grid, const double *dp, const double *dlam,
+                                        const double *wx, const double *wy, const double *wz,
+                                        double *dgrad, void *stream) {
+    if (!grid || !dp || !dlam || !wx || !wy || !wz || !dgrad) return PPS_ERR_BAD_ARG;
     const Dims D = dims_of(*grid);
+    if (D.nx < 1 || D.ny < 1 || D.nz < 1 || D.ng < 1) return PPS_ERR_BAD_ARG;
     hipStream_t s = static_cast<hipStream_t>(stream);
     FaceK d;
     d.nx = D.nx; d.ny = D.ny; d.nz = D.nz; d.ng = D.ng; d.X = D.X(); d.Y = D.Y();
     d.px = D.per[0]; d.py = D.per[1]; d.pz = D.per[2];
-    const dim3 grid3((unsigned)((d.nx + 63) / 64), (unsigned)((d.ny + 3) / 4), (unsigned)d.nz);
-    k_face_grad<<<grid3, dim3(64, 4, 1), 0, s>>>(d, dA, dp, dlam, wx, wy, wz, dgrad);
+    // about 16 blocks of 256 threads per CU over (x, y) columns x z chunks
+    const int gx = (d.nx + 63) / 64, gy = (d.ny + 3) / 4;
+    long long chunks = 4096 / ((long long)gx * gy);
+    if (chunks < 1) chunks = 1;
+    if (chunks > d.nz) chunks = d.nz;
+    const int zc = (int)((d.nz + chunks - 1) / chunks);
+    chunks = (d.nz + zc - 1) / zc;
+    k_face_grad<<<dim3((unsigned)gx, (unsigned)gy, (unsigned)chunks), dim3(64, 4, 1), 0, s>>>(
+        d, zc, dp, dlam, wx, wy, wz, dgrad);
     PPS_HIP(hipGetLastError());
     PPS_HIP(hipStreamSynchronize(s));
     return PPS_OK;

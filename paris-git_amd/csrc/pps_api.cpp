@@ -43,6 +43,7 @@ extern "C" int pps_default_options(pps_options *o) {
     o->sweep_rows = 0;
     o->sweep_planes = 0;
     o->tail_cells = 0;
+    o->transfer = PPS_TRANSFER_CONSTANT;
     return PPS_OK;
 }
 
@@ -62,6 +63,7 @@ static int valid_opt(const pps_options *o) {
     if (o->nrelax < 0 || o->nu_pre < 0 || o->nu_post < 0) return PPS_ERR_BAD_ARG;
     if (o->smoother != PPS_SMOOTH_FUSED && o->smoother != PPS_SMOOTH_COLOUR_PASSES) return PPS_ERR_BAD_ARG;
     if (o->sweep_rows < 0 || o->sweep_rows > 1024 || o->sweep_planes < 0) return PPS_ERR_BAD_ARG;
+    if (o->transfer != PPS_TRANSFER_CONSTANT && o->transfer != PPS_TRANSFER_TRILINEAR) return PPS_ERR_BAD_ARG;
     return PPS_OK;
 }
 
@@ -116,6 +118,9 @@ extern "C" int pps_set_options(pps_handle *h, const pps_options *opt) {
     if (!h) return PPS_ERR_BAD_ARG;
     PPS_TRY(valid_opt(opt));
     h->opt = *opt;
+    // the next solve or adjoint builds its operator and hierarchy again
+    // (method, transfers or smoothing may have changed)
+    fast_invalidate(h->fast);
     return PPS_OK;
 }
 

@@ -230,15 +230,20 @@ int residual_stats(pps_handle *h, const Dims &d, const double *A,
 }
 
 // ---------------------------------------------------------------------
-// check_poisson_setup (newsolver.f90:275-344).  Solid cells (all six face
-// masks closed) are exactly the cells whose six couplings are zero, since
-// an open mask and a finite density give a positive coupling.
+// check_poisson_setup (newsolver.f90:275-344).  Solid cells (A7 := 0 in the
+// row-sum test): with the face masks, the cells whose six masks are all
+// below 1/2, as the reference reads them; without masks (the solvers'
+// check_setup option, which has only A), the cells whose six couplings are
+// zero, the same cells when A comes from SetupPoisson (an open mask and a
+// finite density give a positive coupling).
 // comp 0: count of NaN / A7 < 1e-55 cells, 1: sum A8, 2: max |rowsum|
 // ---------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
-k_check(DK d, const double *__restrict__ A, double *__restrict__ part) {
+k_check(DK d, const double *__restrict__ A, const double *__restrict__ um,
+        const double *__restrict__ vm, const double *__restrict__ wm, double *__restrict__ part) {
     __shared__ double sm[4];
     const long long n = (long long)d.nx * d.ny * d.nz;
+    const long long sy = d.X, sz = (long long)d.X * d.Y;
     double bad = 0.0, src = 0.0, row = 0.0;
     for (long long c = blockIdx.x * 256LL + threadIdx.x; c < n; c += (long long)gridDim.x * 256) {
         double a[8];
@@ -249,8 +254,18 @@ k_check(DK d, const double *__restrict__ A, double *__restrict__ part) {
         for (int s = 0; s < 8; ++s) nan |= (a[s] != a[s]);
         if (nan || a[6] < 1e-55) bad += 1.0;
         src += a[7];
-        const bool solid = a[0] == 0.0 && a[1] == 0.0 && a[2] == 0.0 && a[3] == 0.0 &&
-                           a[4] == 0.0 && a[5] == 0.0;
+        bool solid;
+        if (um) {
+            const int i = (int)(c % d.nx);
+            const long long t = c / d.nx;
+            const int j = (int)(t % d.ny), k = (int)(t / d.ny);
+            const long long q = padi(d, i, j, k);
+            solid = um[q - 1] < 0.5 && um[q] < 0.5 && vm[q - sy] < 0.5 && vm[q] < 0.5 &&
+                    wm[q - sz] < 0.5 && wm[q] < 0.5;
+        } else {
+            solid = a[0] == 0.0 && a[1] == 0.0 && a[2] == 0.0 && a[3] == 0.0 && a[4] == 0.0 &&
+                    a[5] == 0.0;
+        }
         const double a7 = solid ? 0.0 : a[6];
         const double t = (((((a[0] + a[1]) + a[2]) + a[3]) + a[4]) + a[5]) - a7;
         row = fmax(row, fabs(t));
@@ -261,15 +276,20 @@ k_check(DK d, const double *__restrict__ A, double *__restrict__ part) {
     t = block_max256(row, sm); if (threadIdx.x == 0) part[2 * kRedBlocks + blockIdx.x] = t;
 }
 
-int check_setup(pps_handle *h, const Dims &d, const double *A, hipStream_t s) {
+static int check_with(pps_handle *h, const Dims &d, const double *A, const double *um,
+                      const double *vm, const double *wm, hipStream_t s) {
     PPS_TRY(ensure_red(h));
     const int nb = red_blocks(d.cells());
-    k_check<<<nb, 256, 0, s>>>(dk(d), A, red_part(h));
+    k_check<<<nb, 256, 0, s>>>(dk(d), A, um, vm, wm, red_part(h));
     PPS_TRY(red_finish(h, nb, 3, 1 << 2, s));
     if (h->pinned[0] != 0.0) return PPS_ERR_NAN_OR_TINY_DIAG;
     if (h->pinned[1] / (double)d.cells() > 1e-10) return PPS_ERR_VOLUME_SOURCE;
     if (h->pinned[2] > 1e-10) return PPS_ERR_NOT_IN_KERNEL;
     return PPS_OK;
+}
+
+int check_setup(pps_handle *h, const Dims &d, const double *A, hipStream_t s) {
+    return check_with(h, d, A, nullptr, nullptr, nullptr, s);
 }
 
 // ---------------------------------------------------------------------
@@ -315,17 +335,116 @@ k_masks(DK d, int b0, int b1, int b2, int b3, int b4, int b5, double *__restrict
 }
 
 // ---------------------------------------------------------------------
-// SetupPoisson + Poisson_BCs (codes 3/4) + solid rule
+// SetupPoisson + Poisson_BCs (codes 3..6) + solid rule
 // coordinate arrays are indexed by padded index (Fortran index - 1)
 // ---------------------------------------------------------------------
+// Boundary codes of the six faces (x-, y-, z-, x+, y+, z+: bdry_cond) and
+// BoundaryPressure (-x, +x, -y, +y, -z, +z: modules.f90:878-883).
+struct BcK {
+    int b[6];
+    double bp[6];
+};
+
+// Poisson_BCs (modules.f90:2755-2892) applied to the row of cell (i, j, k),
+// block by block in the reference's statement order, so that where one cell
+// lies on several faces the later block overwrites what an earlier one set,
+// as the whole-plane assignments of the reference do.  Codes 3 / 4: the
+// outward coupling is dropped (flux condition).  Code 5: P_0 - P_inner / 3 =
+// 2/3 P_b, corners adding the conditions of both faces (pbc: which of x-, x+,
+// y-, y+ carry code 5).  Code 6 (x+ only): P_b and du/dn at the face (du: the
+// cell's (u(i) - u(i-1)) / dx(i)).  The z+ block of code 5 writes its
+// y-coupling corner entries into the ks plane (modules.f90:2884-2887): the
+// four corner cells of the first plane take them last.
+__device__ __forceinline__ void poisson_bcs(const BcK &bc, int nx, int ny, int nz, int i, int j, int k,
+                                            double du, double a[8]) {
+    const int ie = nx - 1, je = ny - 1, ke = nz - 1;
+    const double c23 = 2.0 / 3.0, c13 = 1.0 / 3.0;
+    const double pbc0 = bc.b[0] == 5 ? 1.0 : 0.0, pbc1 = bc.b[3] == 5 ? 1.0 : 0.0;
+    const double pbc2 = bc.b[1] == 5 ? 1.0 : 0.0, pbc3 = bc.b[4] == 5 ? 1.0 : 0.0;
+    // x- (2768-2781)
+    if (i == 0) {
+        if (bc.b[0] == 3) { a[6] = a[6] - a[0]; a[0] = 0.0; }
+        else if (bc.b[0] == 5) {
+            a[7] = c23 * bc.bp[0];
+            a[6] = 1.0;
+            for (int s = 0; s < 6; ++s) a[s] = 0.0;
+            a[1] = c13;
+        }
+    }
+    // x+ (2784-2801)
+    if (i == ie) {
+        if (bc.b[3] == 4 || bc.b[3] == 3) { a[6] = a[6] - a[1]; a[1] = 0.0; }
+        else if (bc.b[3] == 5) {
+            a[7] = c23 * bc.bp[1];
+            a[6] = 1.0;
+            for (int s = 1; s < 6; ++s) a[s] = 0.0;
+            a[0] = c13;
+        } else if (bc.b[3] == 6) {
+            a[6] = a[6] - a[0] * 2.0 / 3.0 + a[1] * 5.0 / 3.0;
+            a[7] = a[7] + 8.0 / 3.0 * a[1] * bc.bp[1] + du;
+            a[0] = a[0] * 1.0 / 3.0;
+            a[1] = 0.0;
+        }
+    }
+    // y- (2804-2822), y+ (2824-2843)
+    for (int side = 0; side < 2; ++side) {
+        if (j != (side ? je : 0)) continue;
+        const int code = bc.b[side ? 4 : 1];
+        const int out = side ? 3 : 2;
+        if (code == 3) { a[6] = a[6] - a[out]; a[out] = 0.0; }
+        else if (code == 5) {
+            const double P = bc.bp[side ? 3 : 2];
+            a[7] = c23 * P;
+            if (i == 0) a[7] = c23 * (P + bc.bp[0] * pbc0);
+            if (i == ie) a[7] = c23 * (P + bc.bp[1] * pbc1);
+            a[6] = 1.0;
+            if (i == 0) a[6] = 1.0 + pbc0;
+            if (i == ie) a[6] = 1.0 + pbc1;
+            for (int s = 0; s < 6; ++s) a[s] = 0.0;
+            if (i == 0) a[1] = c13 * pbc0;
+            if (i == ie) a[0] = c13 * pbc1;
+            a[side ? 2 : 3] = c13;
+        }
+    }
+    // z- (2845-2867), z+ (2869-2891)
+    for (int side = 0; side < 2; ++side) {
+        const int code = bc.b[side ? 5 : 2];
+        if (k == (side ? ke : 0)) {
+            const int out = side ? 5 : 4;
+            if (code == 3) { a[6] = a[6] - a[out]; a[out] = 0.0; }
+            else if (code == 5) {
+                const double P = bc.bp[side ? 5 : 4];
+                a[7] = c23 * P;
+                if (i == 0 && j == 0) a[7] = c23 * ((P + bc.bp[0] * pbc0) + bc.bp[2] * pbc2);
+                if (i == ie && j == 0) a[7] = c23 * ((P + bc.bp[1] * pbc1) + bc.bp[2] * pbc2);
+                if (i == 0 && j == je) a[7] = c23 * ((P + bc.bp[0] * pbc0) + bc.bp[3] * pbc3);
+                if (i == ie && j == je) a[7] = c23 * ((P + bc.bp[1] * pbc1) + bc.bp[3] * pbc3);
+                a[6] = 1.0;
+                if (i == 0 && j == 0) a[6] = (1.0 + pbc0) + pbc2;
+                if (i == ie && j == 0) a[6] = (1.0 + pbc1) + pbc2;
+                if (i == 0 && j == je) a[6] = (1.0 + pbc0) + pbc3;
+                if (i == ie && j == je) a[6] = (1.0 + pbc1) + pbc3;
+                for (int s = 0; s < 6; ++s) a[s] = 0.0;
+                if (i == 0 && (j == 0 || j == je)) a[1] = c13 * pbc0;
+                if (i == ie && (j == 0 || j == je)) a[0] = c13 * pbc1;
+            }
+        }
+        // the y-coupling corner entries of both blocks land in the ks plane
+        if (code == 5 && k == 0) {
+            if ((i == 0 || i == ie) && j == 0) a[3] = c13 * pbc2;
+            if ((i == 0 || i == ie) && j == je) a[2] = c13 * pbc3;
+        }
+        if (code == 5 && k == (side ? ke : 0)) a[side ? 4 : 5] = c13;
+    }
+}
+
 __global__ void __launch_bounds__(256)
 k_setup_poisson(DK d, const double *__restrict__ u, const double *__restrict__ v,
                 const double *__restrict__ w, const double *__restrict__ um,
                 const double *__restrict__ vm, const double *__restrict__ wm,
                 const double *__restrict__ rho, const double *__restrict__ ax,
                 const double *__restrict__ ay, const double *__restrict__ az, double dt,
-                double vs, int b0, int b1, int b2, int b3, int b4, int b5,
-                double *__restrict__ A, int *__restrict__ flag) {
+                double vs, BcK bc, double *__restrict__ A, int *__restrict__ flag) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int k = blockIdx.z;
@@ -339,33 +458,28 @@ k_setup_poisson(DK d, const double *__restrict__ u, const double *__restrict__ v
     const double *dx = ax, *dxh = ax + NX, *dy = ay, *dyh = ay + NY, *dz = az, *dzh = az + NZ;
     const int I = i + d.ng, J = j + d.ng, K = k + d.ng;
     const double r0 = rho[q];
-    double a1 = 2.0 * dt * um[q - 1] / (dx[I] * dxh[I - 1] * (rho[q - 1] + r0));
-    double a2 = 2.0 * dt * um[q] / (dx[I] * dxh[I] * (rho[q + 1] + r0));
-    double a3 = 2.0 * dt * vm[q - sy] / (dy[J] * dyh[J - 1] * (rho[q - sy] + r0));
-    double a4 = 2.0 * dt * vm[q] / (dy[J] * dyh[J] * (rho[q + sy] + r0));
-    double a5 = 2.0 * dt * wm[q - sz] / (dz[K] * dzh[K - 1] * (rho[q - sz] + r0));
-    double a6 = 2.0 * dt * wm[q] / (dz[K] * dzh[K] * (rho[q + sz] + r0));
-    double a7 = ((((a1 + a2) + a3) + a4) + a5) + a6;
-    const double a8 = -(((vs + (u[q] - u[q - 1]) / dx[I]) + (v[q] - v[q - sy]) / dy[J]) +
-                        (w[q] - w[q - sz]) / dz[K]);
-    // Poisson_BCs, flux conditions: drop the boundary coupling
-    if (i == 0 && b0 == 3) { a7 = a7 - a1; a1 = 0.0; }
-    if (i == d.nx - 1 && (b3 == 4 || b3 == 3)) { a7 = a7 - a2; a2 = 0.0; }
-    if (j == 0 && b1 == 3) { a7 = a7 - a3; a3 = 0.0; }
-    if (j == d.ny - 1 && b4 == 3) { a7 = a7 - a4; a4 = 0.0; }
-    if (k == 0 && b2 == 3) { a7 = a7 - a5; a5 = 0.0; }
-    if (k == d.nz - 1 && b5 == 3) { a7 = a7 - a6; a6 = 0.0; }
+    double a[8];
+    a[0] = 2.0 * dt * um[q - 1] / (dx[I] * dxh[I - 1] * (rho[q - 1] + r0));
+    a[1] = 2.0 * dt * um[q] / (dx[I] * dxh[I] * (rho[q + 1] + r0));
+    a[2] = 2.0 * dt * vm[q - sy] / (dy[J] * dyh[J - 1] * (rho[q - sy] + r0));
+    a[3] = 2.0 * dt * vm[q] / (dy[J] * dyh[J] * (rho[q + sy] + r0));
+    a[4] = 2.0 * dt * wm[q - sz] / (dz[K] * dzh[K - 1] * (rho[q - sz] + r0));
+    a[5] = 2.0 * dt * wm[q] / (dz[K] * dzh[K] * (rho[q + sz] + r0));
+    a[6] = ((((a[0] + a[1]) + a[2]) + a[3]) + a[4]) + a[5];
+    const double dudx = (u[q] - u[q - 1]) / dx[I];
+    a[7] = -(((vs + dudx) + (v[q] - v[q - sy]) / dy[J]) + (w[q] - w[q - sz]) / dz[K]);
+    poisson_bcs(bc, d.nx, d.ny, d.nz, i, j, k, dudx, a);
     // check_and_debug_Poisson (modules.f90:2905-2945)
-    if (a7 < 1e-50) {
+    if (a[6] < 1e-50) {
         const bool solid = um[q - 1] < 0.5 && um[q] < 0.5 && vm[q - sy] < 0.5 && vm[q] < 0.5 &&
                            wm[q - sz] < 0.5 && wm[q] < 0.5;
-        const double mx = fmax(fmax(fmax(a1, a2), fmax(a3, a4)), fmax(a5, a6));
-        const double mn = fmin(fmin(fmin(a1, a2), fmin(a3, a4)), fmin(a5, a6));
-        if (!solid || a8 > 1e-50 || mx > 1e-50 || mn < 0.0) atomicOr(flag, 1);
-        a7 = 1.0;
+        const double mx = fmax(fmax(fmax(a[0], a[1]), fmax(a[2], a[3])), fmax(a[4], a[5]));
+        const double mn = fmin(fmin(fmin(a[0], a[1]), fmin(a[2], a[3])), fmin(a[4], a[5]));
+        if (!solid || a[7] > 1e-50 || mx > 1e-50 || mn < 0.0) atomicOr(flag, 1);
+        a[6] = 1.0;
     }
-    A[c] = a1; A[n + c] = a2; A[2 * n + c] = a3; A[3 * n + c] = a4;
-    A[4 * n + c] = a5; A[5 * n + c] = a6; A[6 * n + c] = a7; A[7 * n + c] = a8;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) A[s * n + c] = a[s];
 }
 
 // ---------------------------------------------------------------------
@@ -675,15 +789,20 @@ static int upload_spacings(const pps_grid *g, const double *dx, const double *dx
 static int setup_poisson_with(const pps_grid *g, const double *u, const double *v, const double *w,
                               const double *um, const double *vm, const double *wm,
                               const double *rho, double dt, double volume_source, const double *ax,
-                              const double *ay, const double *az, double *A, hipStream_t s) {
+                              const double *ay, const double *az, const double *bp, double *A,
+                              hipStream_t s) {
+    if (!u || !v || !w || !um || !vm || !wm || !rho || !A) return PPS_ERR_BAD_ARG;
     const Dims d = dims_of(*g);
     pps_handle *h = scratch_handle();
     PPS_TRY(ensure_red(h));
+    BcK bc;
+    for (int f = 0; f < 6; ++f) {
+        bc.b[f] = g->bdry_cond[f];
+        bc.bp[f] = bp ? bp[f] : 0.0;
+    }
     PPS_HIP(hipMemsetAsync(red_flag(h), 0, sizeof(int), s));
     k_setup_poisson<<<cgrid(d), kBlk, 0, s>>>(dk(d), u, v, w, um, vm, wm, rho, ax, ay, az, dt,
-                                              volume_source, g->bdry_cond[0], g->bdry_cond[1],
-                                              g->bdry_cond[2], g->bdry_cond[3], g->bdry_cond[4],
-                                              g->bdry_cond[5], A, red_flag(h));
+                                              volume_source, bc, A, red_flag(h));
     PPS_HIP(hipGetLastError());
     int flag = 0;
     PPS_HIP(hipMemcpyAsync(&flag, red_flag(h), sizeof(int), hipMemcpyDeviceToHost, s));
@@ -691,23 +810,34 @@ static int setup_poisson_with(const pps_grid *g, const double *u, const double *
     return flag ? PPS_ERR_NAN_OR_TINY_DIAG : PPS_OK;
 }
 
+// codes 0..6 (modules.f90:877): wall, periodic, shear, vel-in, vel-out,
+// pressure, pres-out
 static int assembly_bc_ok(const pps_grid *g) {
     for (int f = 0; f < 6; ++f)
-        if (g->bdry_cond[f] >= 5 || g->bdry_cond[f] < 0) return PPS_ERR_UNSUPPORTED;
+        if (g->bdry_cond[f] > 6 || g->bdry_cond[f] < 0) return PPS_ERR_UNSUPPORTED;
     return PPS_OK;
+}
+
+extern "C" int pps_setup_poisson_bp_device(const pps_grid *g, const double *u, const double *v,
+                                           const double *w, const double *um, const double *vm,
+                                           const double *wm, const double *rho, double dt,
+                                           double volume_source, const double *boundary_pressure,
+                                           double *A, void *stream) {
+    PPS_TRY(grid_ok(g));
+    PPS_TRY(assembly_bc_ok(g));
+    hipStream_t s = as_stream(stream);
+    Coords co;
+    PPS_TRY(co.make(*g, s));
+    return setup_poisson_with(g, u, v, w, um, vm, wm, rho, dt, volume_source, co.ax, co.ay, co.az,
+                              boundary_pressure, A, s);
 }
 
 extern "C" int pps_setup_poisson_device(const pps_grid *g, const double *u, const double *v,
                                         const double *w, const double *um, const double *vm,
                                         const double *wm, const double *rho, double dt,
                                         double volume_source, double *A, void *stream) {
-    PPS_TRY(grid_ok(g));
-    PPS_TRY(assembly_bc_ok(g));
-    hipStream_t s = as_stream(stream);
-    Coords co;
-    PPS_TRY(co.make(*g, s));
-    return setup_poisson_with(g, u, v, w, um, vm, wm, rho, dt, volume_source, co.ax, co.ay, co.az, A,
-                              s);
+    return pps_setup_poisson_bp_device(g, u, v, w, um, vm, wm, rho, dt, volume_source, nullptr, A,
+                                       stream);
 }
 
 extern "C" int pps_setup_poisson_coords_device(const pps_grid *g, const double *u, const double *v,
@@ -725,15 +855,24 @@ extern "C" int pps_setup_poisson_coords_device(const pps_grid *g, const double *
     const double *ax[3];
     int rc = upload_spacings(g, dx, dxh, dy, dyh, dz, dzh, buf, ax, s);
     if (rc == PPS_OK)
-        rc = setup_poisson_with(g, u, v, w, um, vm, wm, rho, dt, volume_source, ax[0], ax[1], ax[2], A,
-                                s);
+        rc = setup_poisson_with(g, u, v, w, um, vm, wm, rho, dt, volume_source, ax[0], ax[1], ax[2],
+                                nullptr, A, s);
     buf.release();
     return rc;
 }
 
 extern "C" int pps_check_setup_device(const pps_grid *g, const double *A, void *stream) {
     PPS_TRY(grid_ok(g));
+    if (!A) return PPS_ERR_BAD_ARG;
     return check_setup(scratch_handle(), dims_of(*g), A, as_stream(stream));
+}
+
+extern "C" int pps_check_poisson_setup_device(const pps_grid *g, const double *A,
+                                              const double *umask, const double *vmask,
+                                              const double *wmask, void *stream) {
+    PPS_TRY(grid_ok(g));
+    if (!A || !umask || !vmask || !wmask) return PPS_ERR_BAD_ARG;
+    return check_with(scratch_handle(), dims_of(*g), A, umask, vmask, wmask, as_stream(stream));
 }
 
 extern "C" int pps_remove_rhs_mean_device(const pps_grid *g, double *A, void *stream) {

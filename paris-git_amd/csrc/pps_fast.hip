@@ -676,6 +676,112 @@ k_prolong_add(int fx, int fy, int fz, T *__restrict__ xf, const T *__restrict__ 
 }
 
 // ---------------------------------------------------------------------
+// Trilinear transfers (PPS_TRANSFER_TRILINEAR).  Along one axis, fine cell
+// i lies a quarter of a coarse cell from the centre of its parent I = i/2,
+// towards the coarse neighbour O = I - 1 (i even) or I + 1 (i odd):
+// interpolation weighs the parent 3/4 and O 1/4.  O wraps on a periodic
+// axis; beyond a wall it is the parent itself (constant extrapolation, the
+// zero-gradient condition), which gives the wall cell the weight 1.  The 3-D
+// prolongation P is the tensor product; the restriction is R = P^T / 8, so
+// coarse cell I gathers fine cells 2I-1, 2I, 2I+1, 2I+2 with 1/4, 3/4, 3/4,
+// 1/4 per axis (a tap beyond a wall falls on the fine cell next to it).
+// ---------------------------------------------------------------------
+struct TriK {
+    int fx, fy, fz;     // fine level
+    int px, py, pz;     // periodic axes
+};
+// coarse neighbour index of fine cell i (nc coarse cells on the axis)
+__device__ __forceinline__ int tri_other(int i, int nc, int per) {
+    const int I = i >> 1;
+    int o = (i & 1) ? I + 1 : I - 1;
+    if (o < 0) o = per ? nc - 1 : 0;
+    else if (o >= nc) o = per ? 0 : nc - 1;
+    return o;
+}
+// fine tap t (0..3) of coarse cell I: index 2I - 1 + t, wrapped or moved
+// back inside at a wall
+__device__ __forceinline__ int tri_tap(int I, int t, int nf, int per) {
+    int i = 2 * I - 1 + t;
+    if (i < 0) i = per ? nf - 1 : 0;
+    else if (i >= nf) i = per ? 0 : nf - 1;
+    return i;
+}
+
+// r = b + sum couplings x_nb - d x on one level (the residual the
+// full-weighting restriction reads)
+template <class T, class B, bool DGI>
+__global__ void __launch_bounds__(256)
+k_resid_field(LevelK<T> L, const B *__restrict__ b, const T *__restrict__ x, T *__restrict__ r) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= L.nx || j >= L.ny) return;
+    const long long c = i + (long long)L.nx * (j + (long long)L.ny * k);
+    r[c] = ((T)b[c] + nb_sum(L, x, i, j, k, c)) - diag_of<T, DGI>(L, i, j, k, c) * x[c];
+}
+
+// bc = R r: one thread per coarse cell, 64 taps
+template <class T>
+__global__ void __launch_bounds__(256)
+k_restrict_tri(TriK t, const T *__restrict__ r, T *__restrict__ bc) {
+    const int cx = t.fx >> 1, cy = t.fy >> 1, cz = t.fz >> 1;
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int J = blockIdx.y * blockDim.y + threadIdx.y;
+    const int K = blockIdx.z;
+    if (I >= cx || J >= cy) return;
+    const T w[4] = {(T)0.25, (T)0.75, (T)0.75, (T)0.25};
+    int ii[4], jj[4], kk[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        ii[a] = tri_tap(I, a, t.fx, t.px);
+        jj[a] = tri_tap(J, a, t.fy, t.py);
+        kk[a] = tri_tap(K, a, t.fz, t.pz);
+    }
+    T acc = 0;
+#pragma unroll
+    for (int c3 = 0; c3 < 4; ++c3) {
+        T accy = 0;
+#pragma unroll
+        for (int b3 = 0; b3 < 4; ++b3) {
+            const long long row = (long long)t.fx * (jj[b3] + (long long)t.fy * kk[c3]);
+            T accx = 0;
+#pragma unroll
+            for (int a = 0; a < 4; ++a) accx += w[a] * r[row + ii[a]];
+            accy += w[b3] * accx;
+        }
+        acc += w[c3] * accy;
+    }
+    bc[I + (long long)cx * (J + (long long)cy * K)] = acc * (T)0.125;
+}
+
+// xf += P xc: one thread per fine cell
+template <class T>
+__global__ void __launch_bounds__(256)
+k_prolong_tri_add(TriK t, T *__restrict__ xf, const T *__restrict__ xc) {
+    const int cx = t.fx >> 1, cy = t.fy >> 1, cz = t.fz >> 1;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= t.fx || j >= t.fy) return;
+    const int ia[2] = {i >> 1, tri_other(i, cx, t.px)};
+    const int ja[2] = {j >> 1, tri_other(j, cy, t.py)};
+    const int ka[2] = {k >> 1, tri_other(k, cz, t.pz)};
+    const T w[2] = {(T)0.75, (T)0.25};
+    T acc = 0;
+#pragma unroll
+    for (int c3 = 0; c3 < 2; ++c3) {
+        T accy = 0;
+#pragma unroll
+        for (int b3 = 0; b3 < 2; ++b3) {
+            const long long row = (long long)cx * (ja[b3] + (long long)cy * ka[c3]);
+            accy += w[b3] * (w[0] * xc[row + ia[0]] + w[1] * xc[row + ia[1]]);
+        }
+        acc += w[c3] * accy;
+    }
+    xf[i + (long long)t.fx * (j + (long long)t.fy * k)] += acc;
+}
+
+// ---------------------------------------------------------------------
 // operator packing: 8-slot FP64 A + padded p -> couplings, diagonal,
 // folded right-hand side, interior iterate.  flags: bit0 NaN/tiny A7,
 // bit1 asymmetric, bit2 nonzero excess (system not singular).
@@ -995,9 +1101,13 @@ template <class T> struct TailK {
     const T *minv;
     int off[kTailMax][6];
     int off_minv;
+    int off_r;               // residual of a level (trilinear transfers)
+    int px, py, pz;          // periodic axes (trilinear transfers)
 };
 
-template <class T, bool DGI>
+// TRI: trilinear transfers (k_resid_field + k_restrict_tri, k_prolong_tri_add
+// on LDS) instead of the 8-child average and piecewise-constant prolongation
+template <class T, bool DGI, bool TRI>
 __global__ void __launch_bounds__(1024)
 k_tail(TailK<T> K) {
     extern __shared__ unsigned char smem_raw[];
@@ -1102,6 +1212,36 @@ k_tail(TailK<T> K) {
         const T *x = sm + K.off[l][5];
         const int cxn = L.nx >> 1, cyn = L.ny >> 1, czn = L.nz >> 1;
         T *bc = sm + K.off[l + 1][4];
+        if constexpr (TRI) {
+            T *r = sm + K.off_r;
+            for (int e = tid; e < L.nx * L.ny * L.nz; e += NT) {
+                int i, j, k;
+                split(e, L.nx, L.ny, i, j, k);
+                r[e] = (b[e] + nb_sum(L, x, i, j, k, e)) - diag_of<T, DGI>(L, i, j, k, e) * x[e];
+            }
+            __syncthreads();
+            const T w[4] = {(T)0.25, (T)0.75, (T)0.75, (T)0.25};
+            for (int e = tid; e < cxn * cyn * czn; e += NT) {
+                int I, J, Kz;
+                split(e, cxn, cyn, I, J, Kz);
+                T acc = 0;
+                for (int c3 = 0; c3 < 4; ++c3) {
+                    const int kf = tri_tap(Kz, c3, L.nz, K.pz);
+                    T accy = 0;
+                    for (int b3 = 0; b3 < 4; ++b3) {
+                        const int jf = tri_tap(J, b3, L.ny, K.py);
+                        const T *row = r + L.nx * (jf + L.ny * kf);
+                        T accx = 0;
+                        for (int a = 0; a < 4; ++a) accx += w[a] * row[tri_tap(I, a, L.nx, K.px)];
+                        accy += w[b3] * accx;
+                    }
+                    acc += w[c3] * accy;
+                }
+                bc[e] = acc * (T)0.125;
+            }
+            __syncthreads();
+            continue;
+        }
         for (int e = tid; e < cxn * cyn * czn; e += NT) {
             int I, J, Kz;
             split(e, cxn, cyn, I, J, Kz);
@@ -1140,7 +1280,25 @@ k_tail(TailK<T> K) {
         for (int e = tid; e < fx * fy * fz; e += NT) {
             int i, j, k;
             split(e, fx, fy, i, j, k);
-            xf[e] += xc[(i >> 1) + cx * ((j >> 1) + cy * (k >> 1))];
+            if constexpr (TRI) {
+                const int cz = fz >> 1;
+                const int ia[2] = {i >> 1, tri_other(i, cx, K.px)};
+                const int ja[2] = {j >> 1, tri_other(j, cy, K.py)};
+                const int ka[2] = {k >> 1, tri_other(k, cz, K.pz)};
+                const T w[2] = {(T)0.75, (T)0.25};
+                T acc = 0;
+                for (int c3 = 0; c3 < 2; ++c3) {
+                    T accy = 0;
+                    for (int b3 = 0; b3 < 2; ++b3) {
+                        const T *row = xc + cx * (ja[b3] + cy * ka[c3]);
+                        accy += w[b3] * (w[0] * row[ia[0]] + w[1] * row[ia[1]]);
+                    }
+                    acc += w[c3] * accy;
+                }
+                xf[e] += acc;
+            } else {
+                xf[e] += xc[(i >> 1) + cx * ((j >> 1) + cy * (k >> 1))];
+            }
         }
         __syncthreads();
         for (int q = 0; q < K.nu_post; ++q) {
@@ -1785,6 +1943,7 @@ struct FastState {
     int singular = 1;
     int exact = 0;   // every A7 is the coupling sum of outer_diag, bit for bit
     const double *built_for = nullptr;
+    int built_mixed = 0;   // precision of the hierarchy last built for built_for
     Hier<double> h64;
     Hier<float> h32;
     DevBuf b32;      // FP32 right-hand side of the mixed V-cycle
@@ -1992,9 +2151,11 @@ static SweepShape sweep_shape(int nx, int ny, int nz, int max_rows, int planes, 
 
 // LDS bytes of a tail that starts at level lt (and its offsets in K)
 template <class T>
-static size_t tail_layout(const Hier<T> &H, int lt, bool dgi, TailK<T> *K) {
+static size_t tail_layout(const Hier<T> &H, int lt, bool dgi, bool tri, TailK<T> *K) {
     const int nl = (int)H.lv.size();
     size_t off = 0;
+    if (K) K->off_r = 0;
+    if (tri && lt < nl) off += (size_t)H.lv[lt].n;   // residual of the tail's top level
     for (int l = lt; l < nl; ++l) {
         const size_t n = (size_t)H.lv[l].n;
         int o[6];
@@ -2022,6 +2183,8 @@ static size_t tail_layout(const Hier<T> &H, int lt, bool dgi, TailK<T> *K) {
 struct Cycle {
     int nu_pre = 2, nu_post = 2;
     bool dgi = false, fused = true;
+    bool tri = false;          // trilinear transfers
+    int per[3] = {0, 0, 0};    // periodic axes (trilinear transfers)
     int rows = 0, planes = 0;
     long long tail_cells = 4096;
     size_t lds_max = 65536;   // LDS bytes one workgroup may use
@@ -2178,7 +2341,8 @@ static int vcycle(Hier<T> &H, const T *fineb, const Cycle &cy, Prof *prof, hipSt
             if (H.lv[l].n <= cy.tail_cells) { lt = l; break; }
         }
         // the tail must fit the workgroup's LDS
-        while (lt < nl && (nl - lt > kTailMax || tail_layout(H, lt, dgi, (TailK<T> *)nullptr) > cy.lds_max))
+        while (lt < nl &&
+               (nl - lt > kTailMax || tail_layout(H, lt, dgi, cy.tri, (TailK<T> *)nullptr) > cy.lds_max))
             ++lt;
     }
     for (int l = 0; l < (lt < nl ? lt : nl - 1); ++l) {
@@ -2196,7 +2360,13 @@ static int vcycle(Hier<T> &H, const T *fineb, const Cycle &cy, Prof *prof, hipSt
             }
         }
         Level<T> &C = H.lv[l + 1];
-        if (L.nx >= 32 && L.ny >= 16) {
+        if (cy.tri) {
+            // residual into the free iterate buffer, then full weighting
+            const TriK t{L.nx, L.ny, L.nz, cy.per[0], cy.per[1], cy.per[2]};
+            if (dgi) k_resid_field<T, T, true><<<cgrid3(L.nx, L.ny, L.nz), kB3, 0, s>>>(L.k(), b, L.x, L.xt);
+            else k_resid_field<T, T, false><<<cgrid3(L.nx, L.ny, L.nz), kB3, 0, s>>>(L.k(), b, L.x, L.xt);
+            k_restrict_tri<T><<<cgrid3(C.nx, C.ny, C.nz), kB3, 0, s>>>(t, L.xt, C.b);
+        } else if (L.nx >= 32 && L.ny >= 16) {
             const TGrid rg = tgrid(L.nx, L.ny, L.nz, 1024, true);
             if (dgi) k_resid_restrict_tiled<T, true><<<rg.blocks, dim3(64, 16, 1), 0, s>>>(L.k(), rg, b, L.x, C.b);
             else k_resid_restrict_tiled<T, false><<<rg.blocks, dim3(64, 16, 1), 0, s>>>(L.k(), rg, b, L.x, C.b);
@@ -2215,9 +2385,15 @@ static int vcycle(Hier<T> &H, const T *fineb, const Cycle &cy, Prof *prof, hipSt
         K.b0 = H.lv[lt].b;
         K.x0 = H.lv[lt].x;
         K.minv = H.minv.template as<T>();
-        const size_t lds = tail_layout(H, lt, dgi, &K);
-        if (dgi) k_tail<T, true><<<1, 1024, lds, s>>>(K);
-        else k_tail<T, false><<<1, 1024, lds, s>>>(K);
+        K.px = cy.per[0]; K.py = cy.per[1]; K.pz = cy.per[2];
+        const size_t lds = tail_layout(H, lt, dgi, cy.tri, &K);
+        if (cy.tri) {
+            if (dgi) k_tail<T, true, true><<<1, 1024, lds, s>>>(K);
+            else k_tail<T, false, true><<<1, 1024, lds, s>>>(K);
+        } else {
+            if (dgi) k_tail<T, true, false><<<1, 1024, lds, s>>>(K);
+            else k_tail<T, false, false><<<1, 1024, lds, s>>>(K);
+        }
     } else {
         Level<T> &Bt = H.lv.back();
         const T *b = nl == 1 ? fineb : Bt.b;
@@ -2227,9 +2403,15 @@ static int vcycle(Hier<T> &H, const T *fineb, const Cycle &cy, Prof *prof, hipSt
         Level<T> &L = H.lv[l];
         const Level<T> &C = H.lv[l + 1];
         const T *b = l == 0 ? fineb : L.b;
-        // the first fused post-sweep adds the correction as it loads x
-        const bool fold = nu_post > 0 && sweep_fused(L, cy);
-        if (!fold) k_prolong_add<T><<<cgrid3(C.nx, C.ny, C.nz), kB3, 0, s>>>(L.nx, L.ny, L.nz, L.x, C.x);
+        // the first fused post-sweep adds a piecewise-constant correction
+        // as it loads x; the trilinear one is a pass of its own
+        const bool fold = nu_post > 0 && sweep_fused(L, cy) && !cy.tri;
+        if (cy.tri) {
+            const TriK t{L.nx, L.ny, L.nz, cy.per[0], cy.per[1], cy.per[2]};
+            k_prolong_tri_add<T><<<cgrid3(L.nx, L.ny, L.nz), kB3, 0, s>>>(t, L.x, C.x);
+        } else if (!fold) {
+            k_prolong_add<T><<<cgrid3(C.nx, C.ny, C.nz), kB3, 0, s>>>(L.nx, L.ny, L.nz, L.x, C.x);
+        }
         const bool with_dot = l == 0 && dot && fused && sweep_dot_ok(L, cy);
         for (int q = 0; q < nu_post; ++q) {
             if (fused) {
@@ -2274,10 +2456,11 @@ static int fast_ensure(pps_handle *h) {
             f->cus = v;
         // the tail kernel may use all of it
         const int mx = (int)f->lds_max;
-        (void)hipFuncSetAttribute((const void *)k_tail<float, true>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
-        (void)hipFuncSetAttribute((const void *)k_tail<float, false>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
-        (void)hipFuncSetAttribute((const void *)k_tail<double, true>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
-        (void)hipFuncSetAttribute((const void *)k_tail<double, false>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
+        const void *tails[8] = {(const void *)k_tail<float, true, false>, (const void *)k_tail<float, false, false>,
+                                (const void *)k_tail<double, true, false>, (const void *)k_tail<double, false, false>,
+                                (const void *)k_tail<float, true, true>, (const void *)k_tail<float, false, true>,
+                                (const void *)k_tail<double, true, true>, (const void *)k_tail<double, false, true>};
+        for (const void *kt : tails) (void)hipFuncSetAttribute(kt, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
         (void)hipGetLastError();
         f->built = 1;
     }
@@ -2362,6 +2545,7 @@ static int fast_setup(pps_handle *h, const double *dA, const double *dp, bool mi
     }
     PPS_HIP(hipGetLastError());
     f->built_for = dA;
+    f->built_mixed = mixed ? 1 : 0;
     f->op_in_A = own ? 0 : 1;
     return PPS_OK;
 }
@@ -2395,6 +2579,8 @@ static int outer_iterate(pps_handle *h, Hier<T> &H, bool mixed, double tol_abs, 
     cy.tail_cells = h->opt.tail_cells == 0 ? 4096 : (h->opt.tail_cells < 0 ? 0 : h->opt.tail_cells);
     cy.lds_max = f->lds_max;
     cy.cus = f->cus;
+    cy.tri = h->opt.transfer == PPS_TRANSFER_TRILINEAR;
+    for (int a = 0; a < 3; ++a) cy.per[a] = f->d.per[a];
 
     // r = b - M x (unless the setup pass just computed it)
     if (!f->have_r) {
@@ -2557,7 +2743,7 @@ int fast_adjoint(pps_handle *h, const double *dA, const double *dg, double *dlam
     const long long n = d.cells();
     // rebuild only if the operator changed; the fold of wall ghosts into
     // the right-hand side is irrelevant here (the adjoint ghosts are zero)
-    if (f->built_for != dA || (mixed ? !f->mixed_built : !f->fp64_built)) {
+    if (f->built_for != dA || f->built_mixed != (mixed ? 1 : 0)) {
         PPS_TRY(fast_setup(h, dA, dlam, mixed, s));
     }
     Vecs V = vecs(f);
@@ -2582,10 +2768,17 @@ int fast_adjoint(pps_handle *h, const double *dA, const double *dg, double *dlam
     PPS_HIP(hipStreamSynchronize(s));
     const double tol = rtol * std::sqrt(h->pinned[0] / (double)n);
     f->have_r = 0;   // the outer iteration recomputes r (and its FP32 copy) for g
+    h->hist.clear();
+    if (h->pinned[0] == 0.0) {
+        // g is zero (after the mean-free projection): lam = 0 exactly
+        PPS_HIP(hipMemsetAsync(dlam, 0, sizeof(double) * d.padded(), s));
+        PPS_HIP(hipStreamSynchronize(s));
+        *it = 0;
+        return PPS_OK;
+    }
     int k = 0;
     double last = 0.0;
     int rc;
-    h->hist.clear();
     if (mixed) rc = outer_iterate<float>(h, f->h32, true, tol, maxit, &k, &last, true, s);
     else rc = outer_iterate<double>(h, f->h64, false, tol, maxit, &k, &last, true, s);
     if (rc != PPS_OK) return rc;

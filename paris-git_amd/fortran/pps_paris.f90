@@ -30,7 +30,7 @@ module pps_paris
   type, bind(C), public :: pps_options_t
      integer(c_int) :: method, outer, nrelax, sym_mg, nu_pre, nu_post
      integer(c_int) :: check_setup, multigrid, smoother, sweep_rows, sweep_planes
-     integer(c_int) :: tail_cells
+     integer(c_int) :: tail_cells, transfer
   end type pps_options_t
 
   interface

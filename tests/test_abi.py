@@ -81,3 +81,32 @@ def test_fortran_module_compiles():
     r = subprocess.run([flang, "-c", src, "-J", out, "-o", os.path.join(out, "m.o")],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_wrong_dtype_or_size_is_refused_before_the_call(pps):
+    """The Python mirror checks dtype and element count of every array it
+    passes: a float32 array, or one whose size does not match the grid's
+    extents, raises instead of reaching a kernel that would index past it."""
+    import torch
+    g = pps.Grid.make(8)
+    with pytest.raises(TypeError):
+        pps._ptr(np.zeros(10, dtype=np.float32))
+    with pytest.raises(TypeError):
+        pps._ptr(torch.zeros(10, dtype=torch.float32))
+    with pytest.raises(ValueError):
+        pps._ptr(np.zeros(g.shape_A[1:]), pps._nA(g), "A")
+    with pytest.raises(ValueError):
+        pps._ptr(torch.zeros((g.nz, g.ny, g.nx), dtype=torch.float64), pps._np(g), "p")
+    with pytest.raises(ValueError):
+        pps._ptr(torch.zeros((4, 4), dtype=torch.float64).t(), 16)
+    with pytest.raises(TypeError):
+        pps._iptr(np.zeros(g.shape_p, dtype=np.int64), pps._np(g))
+    assert pps._ptr(np.zeros(g.shape_A), pps._nA(g), "A").value
+    # device entry points refuse host or non-float64 tensors
+    with pytest.raises(TypeError):
+        pps._need_gpu(torch.zeros(4, dtype=torch.float64))
+    with pytest.raises(TypeError):
+        pps.check_setup(g, torch.zeros(g.shape_A, dtype=torch.float32))
+    # spacing arrays must have n + 2 ng entries
+    with pytest.raises(ValueError):
+        pps._spacing_arrays(g, *[np.ones(g.nx) for _ in range(6)])
