@@ -119,7 +119,9 @@ def gpu_run(pps, torch, grid, A, method, tol, steps, warmup, maxit=200):
 
 
 def reference_iterations(pps, torch, grid, A, tol, cap):
-    """Iterations the reference algorithm needs (its GPU restatement)."""
+    """Iterations the reference algorithm needs (its GPU restatement), run
+    for at most `cap` V-cycles; returns (count, extrapolated, history,
+    wall ms of that capped run)."""
     s = pps.Solver(grid, method="reference")
     p = torch.zeros(grid.shape_p, dtype=torch.float64, device="cuda")
     t0 = time.perf_counter()
@@ -310,9 +312,10 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="drops256")
-    ap.add_argument("--cpu-cycles", type=int, default=4)
+    ap.add_argument("--cpu-cycles", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--ref-cap", type=int, default=3000)
+    ap.add_argument("--ref-cap", type=int, default=6000,
+                    help="V-cycles the reference method may run to count its iterations")
     ap.add_argument("--only", choices=("both", "mixed", "fp64"), default="both",
                     help="time one form only (for counter runs)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -397,6 +400,9 @@ def main():
         "unit": "GB/s",
         "frac": round(m["smoother_GBps"] / HBM_PEAK_GBPS, 4),
         "traffic": measured_traffic(),
+        "traffic_source": ("profiles/k_rb_sweep_drops256_traffic.md: rocprofv3 --pmc FETCH_SIZE and "
+                           "WRITE_SIZE of this kernel in 'bench.py --only mixed --steps 1', calibrated "
+                           "by 'bench.py --copy-calib 1024' (not measured in this run)"),
         "launches": m["launches"],
         "avg_launch_us": round(m["kernel_ms"] / max(m["launches"], 1) * 1e3, 2),
         "bytes_per_launch": round(m["kernel_bytes"] / max(m["launches"], 1)),
@@ -408,7 +414,13 @@ def main():
         print(f"[bench] reference algorithm iteration count on the GPU (cap {args.ref_cap})",
               file=sys.stderr, flush=True)
         nref, extrap, hist, ref_ms = reference_iterations(pps, torch, grid, A, tol, args.ref_cap)
-        ref_info = {"vcycles": nref, "extrapolated": extrap, "gpu_ms": ref_ms}
+        ref_info = {"vcycles_to_solve": nref, "extrapolated": extrap, "cap": args.ref_cap,
+                    "vcycles_run": len(hist),
+                    "rel_residual_at_end": (hist[-1] / brms) if hist else None,
+                    "run_wall_ms": ref_ms,
+                    "run": ("the reference method on the GPU, stopped at 1e-10 or at the cap; "
+                            "run_wall_ms is that run's wall time (the whole solve when not "
+                            "extrapolated)")}
         print(f"[bench] reference algorithm: {nref:.0f} V-cycles; CPU sample of "
               f"{args.cpu_cycles} cycles", file=sys.stderr, flush=True)
         A_host = np.ascontiguousarray(A.cpu().numpy())

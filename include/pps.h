@@ -49,7 +49,7 @@ enum pps_status {
     PPS_ERR_BAD_ARG = 8,
     PPS_ERR_HIP = 9,               /* HIP runtime failure (out of memory, ...) */
     PPS_ERR_NO_DEVICE = 10,
-    PPS_ERR_UNSUPPORTED = 11       /* e.g. pressure boundary codes 5/6 in assembly */
+    PPS_ERR_UNSUPPORTED = 11       /* e.g. a method or option an entry point lacks */
 };
 
 /* ---- solver methods --------------------------------------------------- */
@@ -178,6 +178,25 @@ int pps_solve_project_device(pps_handle *h, const double *dA, double *dp,
                              const double *wmask, const double *rho, double dt,
                              double *u_out, double *v_out, double *w_out,
                              void *stream);
+/* The projection step from the fields, without forming A: SetupPoisson +
+ * Poisson_BCs (modules.f90:2722-2892; codes 0-4) are evaluated inside the
+ * fast solve's setup pass, the same rows bit for bit as
+ * pps_setup_poisson_device (volume_source is S_vol, uniform spacings of
+ * the grid), then the solve runs as pps_solve_device (norm 2), and when
+ * u_out, v_out, w_out are given (all three, not aliasing u, v, w)
+ * project_velocity (paris.f90:898-908) is applied in its last residual
+ * pass, as pps_solve_project_device does.  Replaces the SetupPoisson ->
+ * NewSolver -> project_velocity sequence of paris.f90:499, 898-908 on one
+ * device.  Fast methods only, opt.check_setup 0, no cell widths (else
+ * PPS_ERR_UNSUPPORTED).  Same p, it and tres2 as assembling A and calling
+ * pps_solve_device; no initial-guess reuse beyond dp itself. */
+int pps_solve_fields_device(pps_handle *h, const double *u, const double *v,
+                            const double *w, const double *umask,
+                            const double *vmask, const double *wmask,
+                            const double *rho, double dt, double volume_source,
+                            double *dp, double maxError, int maxit, int *it,
+                            double *tres2, double *u_out, double *v_out,
+                            double *w_out, void *stream);
 
 /* Residual history of the last solve: tres2 after each outer iteration.
  * Returns the number of entries written (<= cap). */

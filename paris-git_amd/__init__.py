@@ -144,6 +144,8 @@ def _lib():
         "pps_new_solver": (_I, [_P, _P, _P, _D, _D, _I, _IP, _IP, _I]),
         "pps_new_solver_mg": (_I, [_P, _P, _P, _D, _D, _I, _IP, _IP, _I, _DP]),
         "pps_solve_device": (_I, [_P, _P, _P, _D, _D, _I, _I, _IP, _DP, _P]),
+        "pps_solve_fields_device": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _D, _D, _P, _D, _I, _IP,
+                                         _DP, _P, _P, _P, _P]),
         "pps_solve_project_device": (_I, [_P, _P, _P, _D, _D, _I, _I, _IP, _DP, _P, _P, _P, _P, _P,
                                           _P, _P, _D, _P, _P, _P, _P]),
         "pps_last_history": (_I, [_P, _DP, _I]),
@@ -188,7 +190,8 @@ def exported_symbols():
     """Names of the C-ABI functions declared in include/pps.h."""
     return [
         "pps_default_options", "pps_init", "pps_finalize", "pps_set_options", "pps_set_cell_widths",
-        "pps_new_solver", "pps_new_solver_mg", "pps_solve_device", "pps_solve_project_device",
+        "pps_new_solver", "pps_new_solver_mg", "pps_solve_device",
+        "pps_solve_project_device", "pps_solve_fields_device",
         "pps_last_history",
         "pps_last_timing", "pps_adjoint_device", "pps_face_gradient_device",
         "pps_pressure_masks_device", "pps_linfunc_device", "pps_setup_poisson_device",
@@ -437,6 +440,29 @@ class Solver:
                                      ctypes.byref(it), ctypes.byref(t2), _stream(stream))
         _check(rc, "pps_solve_device",
                allow=(OK, ERR_NOT_CONVERGED) if allow_maxit else (OK,))
+        return it.value, t2.value
+
+    def solve_fields(self, fields, p, maxError, dt=1.0, volume_source=0.0, maxit=50,
+                     project=False, stream=None, allow_maxit=True):
+        """Assemble the system from the fields inside the solve (A is never
+        formed) and solve in place on p; fields = dict(u, v, w, um, vm, wm,
+        rho) of padded device tensors.  With project=True also returns the
+        corrected velocities.  Returns (it, tres2[, u_out, v_out, w_out])."""
+        import torch
+        f = fields
+        _need_gpu(p, f["u"], f["v"], f["w"], f["um"], f["vm"], f["wm"], f["rho"])
+        g = self.grid
+        npad = _np(g)
+        outs = [torch.empty_like(f[c]) for c in ("u", "v", "w")] if project else [None] * 3
+        it, t2 = ctypes.c_int(), ctypes.c_double()
+        rc = _lib().pps_solve_fields_device(
+            self._h, *[_ptr(f[c], npad, c) for c in ("u", "v", "w", "um", "vm", "wm", "rho")],
+            dt, volume_source, _ptr(p, npad, "p"), maxError, maxit, ctypes.byref(it), ctypes.byref(t2),
+            *[_ptr(o) if o is not None else None for o in outs], _stream(stream))
+        _check(rc, "pps_solve_fields_device",
+               allow=(OK, ERR_NOT_CONVERGED) if allow_maxit else (OK,))
+        if project:
+            return (it.value, t2.value) + tuple(outs)
         return it.value, t2.value
 
     def solve_project(self, A, p, u, v, w, umask, vmask, wmask, rho, dt, maxError, beta=1.2,

@@ -24,7 +24,7 @@ extern "C" const char *pps_status_string(int st) {
     case PPS_ERR_BAD_ARG: return "bad argument";
     case PPS_ERR_HIP: return "HIP runtime error";
     case PPS_ERR_NO_DEVICE: return "no HIP device";
-    case PPS_ERR_UNSUPPORTED: return "unsupported boundary condition";
+    case PPS_ERR_UNSUPPORTED: return "unsupported by this entry point (method, option or boundary condition)";
     default: return "unknown status";
     }
 }

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "pps_internal.h"
+#include "pps_row.h"
 
 namespace pps {
 
@@ -338,146 +339,17 @@ k_masks(DK d, int b0, int b1, int b2, int b3, int b4, int b5, double *__restrict
 // SetupPoisson + Poisson_BCs (codes 3..6) + solid rule
 // coordinate arrays are indexed by padded index (Fortran index - 1)
 // ---------------------------------------------------------------------
-// Boundary codes of the six faces (x-, y-, z-, x+, y+, z+: bdry_cond) and
-// BoundaryPressure (-x, +x, -y, +y, -z, +z: modules.f90:878-883).
-struct BcK {
-    int b[6];
-    double bp[6];
-};
-
-// Poisson_BCs (modules.f90:2755-2892) applied to the row of cell (i, j, k),
-// block by block in the reference's statement order, so that where one cell
-// lies on several faces the later block overwrites what an earlier one set,
-// as the whole-plane assignments of the reference do.  Codes 3 / 4: the
-// outward coupling is dropped (flux condition).  Code 5: P_0 - P_inner / 3 =
-// 2/3 P_b, corners adding the conditions of both faces (pbc: which of x-, x+,
-// y-, y+ carry code 5).  Code 6 (x+ only): P_b and du/dn at the face (du: the
-// cell's (u(i) - u(i-1)) / dx(i)).  The z+ block of code 5 writes its
-// y-coupling corner entries into the ks plane (modules.f90:2884-2887): the
-// four corner cells of the first plane take them last.
-__device__ __forceinline__ void poisson_bcs(const BcK &bc, int nx, int ny, int nz, int i, int j, int k,
-                                            double du, double a[8]) {
-    const int ie = nx - 1, je = ny - 1, ke = nz - 1;
-    const double c23 = 2.0 / 3.0, c13 = 1.0 / 3.0;
-    const double pbc0 = bc.b[0] == 5 ? 1.0 : 0.0, pbc1 = bc.b[3] == 5 ? 1.0 : 0.0;
-    const double pbc2 = bc.b[1] == 5 ? 1.0 : 0.0, pbc3 = bc.b[4] == 5 ? 1.0 : 0.0;
-    // x- (2768-2781)
-    if (i == 0) {
-        if (bc.b[0] == 3) { a[6] = a[6] - a[0]; a[0] = 0.0; }
-        else if (bc.b[0] == 5) {
-            a[7] = c23 * bc.bp[0];
-            a[6] = 1.0;
-            for (int s = 0; s < 6; ++s) a[s] = 0.0;
-            a[1] = c13;
-        }
-    }
-    // x+ (2784-2801)
-    if (i == ie) {
-        if (bc.b[3] == 4 || bc.b[3] == 3) { a[6] = a[6] - a[1]; a[1] = 0.0; }
-        else if (bc.b[3] == 5) {
-            a[7] = c23 * bc.bp[1];
-            a[6] = 1.0;
-            for (int s = 1; s < 6; ++s) a[s] = 0.0;
-            a[0] = c13;
-        } else if (bc.b[3] == 6) {
-            a[6] = a[6] - a[0] * 2.0 / 3.0 + a[1] * 5.0 / 3.0;
-            a[7] = a[7] + 8.0 / 3.0 * a[1] * bc.bp[1] + du;
-            a[0] = a[0] * 1.0 / 3.0;
-            a[1] = 0.0;
-        }
-    }
-    // y- (2804-2822), y+ (2824-2843)
-    for (int side = 0; side < 2; ++side) {
-        if (j != (side ? je : 0)) continue;
-        const int code = bc.b[side ? 4 : 1];
-        const int out = side ? 3 : 2;
-        if (code == 3) { a[6] = a[6] - a[out]; a[out] = 0.0; }
-        else if (code == 5) {
-            const double P = bc.bp[side ? 3 : 2];
-            a[7] = c23 * P;
-            if (i == 0) a[7] = c23 * (P + bc.bp[0] * pbc0);
-            if (i == ie) a[7] = c23 * (P + bc.bp[1] * pbc1);
-            a[6] = 1.0;
-            if (i == 0) a[6] = 1.0 + pbc0;
-            if (i == ie) a[6] = 1.0 + pbc1;
-            for (int s = 0; s < 6; ++s) a[s] = 0.0;
-            if (i == 0) a[1] = c13 * pbc0;
-            if (i == ie) a[0] = c13 * pbc1;
-            a[side ? 2 : 3] = c13;
-        }
-    }
-    // z- (2845-2867), z+ (2869-2891)
-    for (int side = 0; side < 2; ++side) {
-        const int code = bc.b[side ? 5 : 2];
-        if (k == (side ? ke : 0)) {
-            const int out = side ? 5 : 4;
-            if (code == 3) { a[6] = a[6] - a[out]; a[out] = 0.0; }
-            else if (code == 5) {
-                const double P = bc.bp[side ? 5 : 4];
-                a[7] = c23 * P;
-                if (i == 0 && j == 0) a[7] = c23 * ((P + bc.bp[0] * pbc0) + bc.bp[2] * pbc2);
-                if (i == ie && j == 0) a[7] = c23 * ((P + bc.bp[1] * pbc1) + bc.bp[2] * pbc2);
-                if (i == 0 && j == je) a[7] = c23 * ((P + bc.bp[0] * pbc0) + bc.bp[3] * pbc3);
-                if (i == ie && j == je) a[7] = c23 * ((P + bc.bp[1] * pbc1) + bc.bp[3] * pbc3);
-                a[6] = 1.0;
-                if (i == 0 && j == 0) a[6] = (1.0 + pbc0) + pbc2;
-                if (i == ie && j == 0) a[6] = (1.0 + pbc1) + pbc2;
-                if (i == 0 && j == je) a[6] = (1.0 + pbc0) + pbc3;
-                if (i == ie && j == je) a[6] = (1.0 + pbc1) + pbc3;
-                for (int s = 0; s < 6; ++s) a[s] = 0.0;
-                if (i == 0 && (j == 0 || j == je)) a[1] = c13 * pbc0;
-                if (i == ie && (j == 0 || j == je)) a[0] = c13 * pbc1;
-            }
-        }
-        // the y-coupling corner entries of both blocks land in the ks plane
-        if (code == 5 && k == 0) {
-            if ((i == 0 || i == ie) && j == 0) a[3] = c13 * pbc2;
-            if ((i == 0 || i == ie) && j == je) a[2] = c13 * pbc3;
-        }
-        if (code == 5 && k == (side ? ke : 0)) a[side ? 4 : 5] = c13;
-    }
-}
-
 __global__ void __launch_bounds__(256)
-k_setup_poisson(DK d, const double *__restrict__ u, const double *__restrict__ v,
-                const double *__restrict__ w, const double *__restrict__ um,
-                const double *__restrict__ vm, const double *__restrict__ wm,
-                const double *__restrict__ rho, const double *__restrict__ ax,
-                const double *__restrict__ ay, const double *__restrict__ az, double dt,
-                double vs, BcK bc, double *__restrict__ A, int *__restrict__ flag) {
+k_setup_poisson(DK d, RowK R, double *__restrict__ A, int *__restrict__ flag) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int k = blockIdx.z;
     if (i >= d.nx || j >= d.ny) return;
     const long long n = (long long)d.nx * d.ny * d.nz;
     const long long c = i + (long long)d.nx * (j + (long long)d.ny * k);
-    const long long q = padi(d, i, j, k);
     const long long sy = d.X, sz = (long long)d.X * d.Y;
-    // ax = {dx[0..X), dxh[0..X)}; padded index I = i + ng
-    const int NX = d.nx + 2 * d.ng, NY = d.ny + 2 * d.ng, NZ = d.nz + 2 * d.ng;
-    const double *dx = ax, *dxh = ax + NX, *dy = ay, *dyh = ay + NY, *dz = az, *dzh = az + NZ;
-    const int I = i + d.ng, J = j + d.ng, K = k + d.ng;
-    const double r0 = rho[q];
     double a[8];
-    a[0] = 2.0 * dt * um[q - 1] / (dx[I] * dxh[I - 1] * (rho[q - 1] + r0));
-    a[1] = 2.0 * dt * um[q] / (dx[I] * dxh[I] * (rho[q + 1] + r0));
-    a[2] = 2.0 * dt * vm[q - sy] / (dy[J] * dyh[J - 1] * (rho[q - sy] + r0));
-    a[3] = 2.0 * dt * vm[q] / (dy[J] * dyh[J] * (rho[q + sy] + r0));
-    a[4] = 2.0 * dt * wm[q - sz] / (dz[K] * dzh[K - 1] * (rho[q - sz] + r0));
-    a[5] = 2.0 * dt * wm[q] / (dz[K] * dzh[K] * (rho[q + sz] + r0));
-    a[6] = ((((a[0] + a[1]) + a[2]) + a[3]) + a[4]) + a[5];
-    const double dudx = (u[q] - u[q - 1]) / dx[I];
-    a[7] = -(((vs + dudx) + (v[q] - v[q - sy]) / dy[J]) + (w[q] - w[q - sz]) / dz[K]);
-    poisson_bcs(bc, d.nx, d.ny, d.nz, i, j, k, dudx, a);
-    // check_and_debug_Poisson (modules.f90:2905-2945)
-    if (a[6] < 1e-50) {
-        const bool solid = um[q - 1] < 0.5 && um[q] < 0.5 && vm[q - sy] < 0.5 && vm[q] < 0.5 &&
-                           wm[q - sz] < 0.5 && wm[q] < 0.5;
-        const double mx = fmax(fmax(fmax(a[0], a[1]), fmax(a[2], a[3])), fmax(a[4], a[5]));
-        const double mn = fmin(fmin(fmin(a[0], a[1]), fmin(a[2], a[3])), fmin(a[4], a[5]));
-        if (!solid || a[7] > 1e-50 || mx > 1e-50 || mn < 0.0) atomicOr(flag, 1);
-        a[6] = 1.0;
-    }
+    if (setup_row(R, d.nx, d.ny, d.nz, d.ng, sy, sz, i, j, k, padi(d, i, j, k), a)) atomicOr(flag, 1);
 #pragma unroll
     for (int s = 0; s < 8; ++s) A[s * n + c] = a[s];
 }
@@ -801,8 +673,8 @@ static int setup_poisson_with(const pps_grid *g, const double *u, const double *
         bc.bp[f] = bp ? bp[f] : 0.0;
     }
     PPS_HIP(hipMemsetAsync(red_flag(h), 0, sizeof(int), s));
-    k_setup_poisson<<<cgrid(d), kBlk, 0, s>>>(dk(d), u, v, w, um, vm, wm, rho, ax, ay, az, dt,
-                                              volume_source, bc, A, red_flag(h));
+    const RowK R{u, v, w, um, vm, wm, rho, ax, ay, az, dt, volume_source, bc};
+    k_setup_poisson<<<cgrid(d), kBlk, 0, s>>>(dk(d), R, A, red_flag(h));
     PPS_HIP(hipGetLastError());
     int flag = 0;
     PPS_HIP(hipMemcpyAsync(&flag, red_flag(h), sizeof(int), hipMemcpyDeviceToHost, s));
@@ -949,6 +821,17 @@ int handle_spacing(pps_handle *h, const double *dh[3], hipStream_t s) {
     return PPS_OK;
 }
 
+// {dx, dxh} of each axis of the handle's grid on the device (padded), the
+// layout RowK reads
+static int handle_axes(pps_handle *h, const double *ax[3], hipStream_t s) {
+    const double *dh[3];
+    PPS_TRY(handle_spacing(h, dh, s));
+    const pps_grid &g = h->grid;
+    const int nt[3] = {g.nx + 2 * g.ng, g.ny + 2 * g.ng, g.nz + 2 * g.ng};
+    for (int a = 0; a < 3; ++a) ax[a] = dh[a] - nt[a];
+    return PPS_OK;
+}
+
 }  // namespace pps
 
 // ax / ay / az: {dx, dxh} of each axis on the device, padded
@@ -1056,4 +939,56 @@ extern "C" int pps_synth_fields_device(const pps_grid *g, const double *drops, i
     PPS_HIP(hipStreamSynchronize(s));
     dd.release();
     return PPS_OK;
+}
+
+// SetupPoisson + Poisson_BCs (codes 0-4) fused into the fast solve's setup
+// pass, and optionally project_velocity fused into its last residual pass:
+// the 8-slot A is never formed.
+extern "C" int pps_solve_fields_device(pps_handle *h, const double *u, const double *v,
+                                       const double *w, const double *umask, const double *vmask,
+                                       const double *wmask, const double *rho, double dt,
+                                       double volume_source, double *dp, double maxError, int maxit,
+                                       int *it, double *tres2, double *u_out, double *v_out,
+                                       double *w_out, void *stream) {
+    if (!h || !u || !v || !w || !umask || !vmask || !wmask || !rho || !dp || !it || !tres2 || maxit < 1)
+        return PPS_ERR_BAD_ARG;
+    const bool proj = u_out || v_out || w_out;
+    if (proj && (!u_out || !v_out || !w_out || u_out == u || v_out == v || w_out == w))
+        return PPS_ERR_BAD_ARG;
+    const int method = h->opt.method;
+    if ((method != PPS_MG_FP64 && method != PPS_MG_MIXED) || h->opt.check_setup || h->have_widths)
+        return PPS_ERR_UNSUPPORTED;
+    // codes 5 / 6 give non-symmetric rows, which the fast methods refuse
+    for (int f = 0; f < 6; ++f)
+        if (h->grid.bdry_cond[f] < 0 || h->grid.bdry_cond[f] > 4) return PPS_ERR_UNSUPPORTED;
+    (void)hipSetDevice(h->device);
+    hipStream_t s = as_stream(stream);
+    const Dims D = dims_of(h->grid);
+    const double *ax[3];
+    PPS_TRY(handle_axes(h, ax, s));
+    BcK bc;
+    for (int f = 0; f < 6; ++f) {
+        bc.b[f] = h->grid.bdry_cond[f];
+        bc.bp[f] = 0.0;
+    }
+    const RowK R{u, v, w, umask, vmask, wmask, rho, ax[0], ax[1], ax[2], dt, volume_source, bc};
+    ProjK P{};
+    if (proj) {
+        const double *dh[3];
+        PPS_TRY(handle_spacing(h, dh, s));
+        P.u = u; P.v = v; P.w = w;
+        P.um = umask; P.vm = vmask; P.wm = wmask; P.rho = rho;
+        P.dxh = dh[0]; P.dyh = dh[1]; P.dzh = dh[2];
+        P.uo = u_out; P.vo = v_out; P.wo = w_out;
+        P.dt = dt;
+        P.lx = h->grid.bdry_cond[0] != 1;
+        P.ly = h->grid.bdry_cond[1] != 1;
+        P.lz = h->grid.bdry_cond[2] != 1;
+        const double *const src[3] = {u, v, w};
+        double *const dst[3] = {u_out, v_out, w_out};
+        PPS_TRY(copy_ghosts(D, src, dst, s));
+    }
+    const int rc = fast_solve(h, nullptr, dp, maxError, maxit, 2, it, tres2, s, false, proj ? &P : nullptr, &R);
+    if (rc != PPS_OK) return rc;
+    return *it > maxit ? PPS_ERR_NOT_CONVERGED : PPS_OK;
 }

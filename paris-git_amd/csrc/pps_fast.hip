@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "pps_internal.h"
+#include "pps_row.h"
 
 namespace pps {
 
@@ -1702,8 +1703,68 @@ k_unpack(PackK d, const double *__restrict__ x, double *__restrict__ p) {
 // 2 dt mask / (dx dxh (rho + rho)) with the width dx of the row's own cell
 // (modules.f90:2737-2742), so dx(i) A2(i) = dx(i+1) A1(i+1).  b, r and the
 // operator are those of the weighted rows; the b^2 partial is unweighted.
+//
+// The rows come from a row source: RowsA reads the caller's 8-slot A;
+// RowsF assembles each row from the fields (SetupPoisson + Poisson_BCs +
+// the solid rule, the expressions the assembly kernel uses), so that A is
+// never written.  A source gives the row of a cell, and for the symmetry
+// check the -x / -y / -z coupling of its +x / +y / +z neighbour (wrapped on
+// a periodic axis; only asked for where that neighbour exists).
+struct CellAt {
+    int i, j, k;
+    long long c, q;     // interior and padded index
+    bool lastx, lasty, lastz;
+};
+struct RowsA {
+    const double *A;
+    long long n;
+    double a4n;   // the -z coupling of the next plane, carried down the march
+    __device__ void begin(long long c0) { a4n = A[4 * n + c0]; }
+    __device__ int row(const CellAt &e, double a[8]) {
+#pragma unroll
+        for (int s = 0; s < 8; ++s) a[s] = s == 4 ? a4n : A[s * n + e.c];
+        return 0;
+    }
+    __device__ double nb_x(const CellAt &e, long long ex) const { return A[e.c + ex]; }
+    __device__ double nb_y(const CellAt &e, long long ey) const { return A[2 * n + e.c + ey]; }
+    // the next plane's -z coupling (wrapped): read every step and carried
+    __device__ double nb_z(const CellAt &e, long long ez) {
+        a4n = A[4 * n + e.c + ez];
+        return a4n;
+    }
+};
+struct RowsF {
+    RowK R;
+    int nx, ny, nz, ng;
+    long long Py, Pz;
+    __device__ void begin(long long) {}
+    __device__ int row(const CellAt &e, double a[8]) {
+        return setup_row(R, nx, ny, nz, ng, Py, Pz, e.i, e.j, e.k, e.q, a);
+    }
+    // For boundary codes 0-4 a neighbour's lower coupling is SetupPoisson's
+    // expression: Poisson_BCs changes only couplings across the domain
+    // boundary (codes 3/4), which this check never asks for, and the solid
+    // rule changes only the diagonal.
+    __device__ double nb_x(const CellAt &e, long long) const {
+        const int I = (e.lastx ? 0 : e.i + 1) + ng;
+        const long long q = e.lastx ? e.q - (nx - 1) : e.q + 1;
+        return face_coupling(R.dt, R.um[q - 1], R.ax[I], R.ax[nx + 2 * ng + I - 1], R.rho[q - 1], R.rho[q]);
+    }
+    __device__ double nb_y(const CellAt &e, long long) const {
+        const int J = (e.lasty ? 0 : e.j + 1) + ng;
+        const long long q = e.lasty ? e.q - (long long)(ny - 1) * Py : e.q + Py;
+        return face_coupling(R.dt, R.vm[q - Py], R.ay[J], R.ay[ny + 2 * ng + J - 1], R.rho[q - Py], R.rho[q]);
+    }
+    __device__ double nb_z(const CellAt &e, long long) {
+        const int K = (e.lastz ? 0 : e.k + 1) + ng;
+        const long long q = e.lastz ? e.q - (long long)(nz - 1) * Pz : e.q + Pz;
+        return face_coupling(R.dt, R.wm[q - Pz], R.az[K], R.az[nz + 2 * ng + K - 1], R.rho[q - Pz], R.rho[q]);
+    }
+};
+
+template <class Src>
 __global__ void __launch_bounds__(256)
-k_setup(PackK d, int zc, const double *__restrict__ A, const double *__restrict__ p,
+k_setup(PackK d, int zc, Src src, const double *__restrict__ p,
         double *__restrict__ cx, double *__restrict__ cy, double *__restrict__ cz,
         double *__restrict__ dg, double *__restrict__ bo, double *__restrict__ x,
         double *__restrict__ r, float *__restrict__ f32, float *__restrict__ r32,
@@ -1726,13 +1787,14 @@ k_setup(PackK d, int zc, const double *__restrict__ A, const double *__restrict_
         // symmetry check of plane k-1 loaded it), carried down the march
         const long long q0 = (i + d.ng) + Py * (j + d.ng) + Pz * (k0 + d.ng);
         double pb = p[q0 - Pz], pc = p[q0];
-        double a4n = A[4 * n + i + sy * j + sz * k0];
+        src.begin(i + sy * j + sz * k0);
         for (int k = k0; k < k1; ++k) {
             const long long c = i + sy * j + sz * k;
             const long long q = (i + d.ng) + Py * (j + d.ng) + Pz * (k + d.ng);
+            const bool lastz = k == d.nz - 1;
+            const CellAt e{i, j, k, c, q, lastx, lasty, lastz};
             double a[8];
-#pragma unroll
-            for (int s = 0; s < 8; ++s) a[s] = s == 4 ? a4n : A[s * n + c];
+            if (src.row(e, a)) f |= 1;
             bool nan = false;
 #pragma unroll
             for (int s = 0; s < 8; ++s) nan |= a[s] != a[s];
@@ -1751,7 +1813,6 @@ k_setup(PackK d, int zc, const double *__restrict__ A, const double *__restrict_
             const double pw = p[q - 1], pe = p[q + 1], ps = p[q - Py], pn = p[q + Py], pt = p[q + Pz];
             const double rv = a[7] - a[6] * pc + a[0] * pw + a[1] * pe + a[2] * ps + a[3] * pn +
                               a[4] * pb + a[5] * pt;
-            const bool lastz = k == d.nz - 1;
             double rhs = a[7], kept = 0.0;
             double vx = a[1], vy = a[3], vz = a[5];
             if (!d.px) {
@@ -1772,13 +1833,12 @@ k_setup(PackK d, int zc, const double *__restrict__ A, const double *__restrict_
             } else {
                 kept += a[4] + a[5];
             }
-            if (!lastx || d.px) { if (asym(A[c + ex_] * wxn, a[1])) f |= 2; }
-            if (!lasty || d.py) { if (asym(A[2 * n + c + ey_] * wyn, a[3])) f |= 2; }
+            if (!lastx || d.px) { if (asym(src.nb_x(e, ex_) * wxn, a[1])) f |= 2; }
+            if (!lasty || d.py) { if (asym(src.nb_y(e, ey_) * wyn, a[3])) f |= 2; }
             {
-                // the +z neighbour's -z coupling: the next plane's a[4]
-                const long long e = lastz ? c - (long long)(d.nz - 1) * sz : c + sz;
-                a4n = A[4 * n + e];
-                if ((!lastz || d.pz) && asym(a4n * wzn, a[5])) f |= 2;
+                // the +z neighbour's -z coupling (the next plane's a[4])
+                const double zn = src.nb_z(e, lastz ? -(long long)(d.nz - 1) * sz : sz);
+                if ((!lastz || d.pz) && asym(zn * wzn, a[5])) f |= 2;
             }
             if (fabs(a[6] - kept) > 1e-12 * fabs(a[6])) f |= 4;
             {
@@ -2489,8 +2549,10 @@ static OpK opk(FastState *f) {
     return OpK{f->d.nx, f->d.ny, f->d.nz, o, o + n, o + 2 * n, o + 3 * n, px, py, pz};
 }
 
-// pack A, detect symmetry/singularity, build the hierarchy for `mixed`
-static int fast_setup(pps_handle *h, const double *dA, const double *dp, bool mixed, hipStream_t s) {
+// pack A (or, given rows, the system assembled from the fields), detect
+// symmetry/singularity, build the hierarchy for `mixed`
+static int fast_setup(pps_handle *h, const double *dA, const double *dp, bool mixed, hipStream_t s,
+                      const RowK *rows = nullptr) {
     FastState *f = h->fast;
     const Dims &d = f->d;
     const long long n = d.cells();
@@ -2511,16 +2573,23 @@ static int fast_setup(pps_handle *h, const double *dA, const double *dp, bool mi
     const ZGrid zg = zgrid(d.nx, d.ny, d.nz);
     PPS_HIP(hipMemsetAsync(flags, 0, sizeof(int), s));
     // mixed mode: the outer kernels read the operator from A itself, unless
-    // the rows are weighted (cell widths set)
+    // the rows are weighted (cell widths set) or there is no A
     double *o64 = f->op.as<double>();
     const double *w = h->widths.as<double>();
     const double *wx = h->have_widths ? w : nullptr;
     const double *wy = wx ? w + d.nx : nullptr, *wz = wx ? w + d.nx + d.ny : nullptr;
-    const bool own = !mixed || wx;
-    k_setup<<<zg.grid, dim3(64, 4, 1), 0, s>>>(packk(d), zg.zc, dA, dp, own ? o64 : nullptr,
-                                                own ? o64 + n : nullptr, own ? o64 + 2 * n : nullptr,
-                                                own ? o64 + 3 * n : nullptr, V.b, V.x, V.r, f32, r32,
-                                                part, flags, wx, wy, wz);
+    const bool own = !mixed || wx || rows;
+    double *ocx = own ? o64 : nullptr, *ocy = own ? o64 + n : nullptr;
+    double *ocz = own ? o64 + 2 * n : nullptr, *odg = own ? o64 + 3 * n : nullptr;
+    if (rows) {
+        const RowsF src{*rows, d.nx, d.ny, d.nz, d.ng, (long long)d.X(), (long long)d.X() * d.Y()};
+        k_setup<RowsF><<<zg.grid, dim3(64, 4, 1), 0, s>>>(packk(d), zg.zc, src, dp, ocx, ocy, ocz, odg, V.b,
+                                                          V.x, V.r, f32, r32, part, flags, wx, wy, wz);
+    } else {
+        const RowsA src{dA, n, 0.0};
+        k_setup<RowsA><<<zg.grid, dim3(64, 4, 1), 0, s>>>(packk(d), zg.zc, src, dp, ocx, ocy, ocz, odg, V.b,
+                                                          V.x, V.r, f32, r32, part, flags, wx, wy, wz);
+    }
     k_scal_rr<<<1, 1024, 0, s>>>(part, zg.blocks, 1, S);
     PPS_HIP(hipGetLastError());
     int fl = 0;
@@ -2544,7 +2613,8 @@ static int fast_setup(pps_handle *h, const double *dA, const double *dp, bool mi
         hier_build(H, f->singular, s);
     }
     PPS_HIP(hipGetLastError());
-    f->built_for = dA;
+    // an operator assembled from fields is not kept for a later adjoint
+    f->built_for = rows ? nullptr : dA;
     f->built_mixed = mixed ? 1 : 0;
     f->op_in_A = own ? 0 : 1;
     return PPS_OK;
@@ -2649,14 +2719,16 @@ static int outer_iterate(pps_handle *h, Hier<T> &H, bool mixed, double tol_abs, 
 // relative: maxError bounds ||r||_2 / ||b||_2 (b the folded right-hand
 // side) instead of sqrt(sum r^2 / N)
 int fast_solve(pps_handle *h, const double *dA, double *dp, double maxError, int maxit, int norm,
-               int *it, double *tres2, hipStream_t s, bool relative, const ProjK *proj) {
+               int *it, double *tres2, hipStream_t s, bool relative, const ProjK *proj,
+               const RowK *rows) {
+    if (rows && norm != 2) return PPS_ERR_BAD_ARG;
     if (proj && norm != 2) return PPS_ERR_BAD_ARG;
     PPS_TRY(fast_ensure(h));
     FastState *f = h->fast;
     const bool mixed = h->opt.method == PPS_MG_MIXED;
     h->hist.clear();
     PPS_HIP(hipEventRecord(h->ev[0], s));
-    PPS_TRY(fast_setup(h, dA, dp, mixed, s));
+    PPS_TRY(fast_setup(h, dA, dp, mixed, s, rows));
     PPS_HIP(hipEventRecord(h->ev[1], s));
     const Dims &d = f->d;
     const double N = (double)d.cells();

@@ -159,9 +159,13 @@ struct ProjK {
     int lx, ly, lz;
 };
 
+// rows: assemble the system from the fields instead of reading dA (which
+// may then be null; norm 2 only)
+struct RowK;
 int fast_solve(pps_handle *h, const double *dA, double *dp, double maxError,
                int maxit, int norm, int *it, double *tres2, hipStream_t s,
-               bool relative = false, const ProjK *proj = nullptr);
+               bool relative = false, const ProjK *proj = nullptr,
+               const RowK *rows = nullptr);
 int fast_adjoint(pps_handle *h, const double *dA, const double *dg,
                  double *dlam, double rtol, int maxit, int *it, hipStream_t s);
 void fast_destroy(FastState *f);

@@ -260,9 +260,11 @@ def test_adjoint_gradient_on_zero_faces(pps):
 def test_adjoint_128_directional_vs_oracle_solver(pps, orc):
     """The 128^3 adjoint workload against the FP64 CPU solver: a central
     difference of J along a random relative perturbation v of every open
-    face, each J from the oracle's CG (Jacobi-PCG to a 1e-15 relative
-    residual) on the 8-slot system of the perturbed faces, against
-    <dJ/dc, v> from the adjoint solve and the face kernel."""
+    face, each J from the oracle's CG (Jacobi-PCG, a fixed 2000 iterations,
+    which reach its rounding floor of about 1e-15 relative residual; the two
+    solves then run the same arithmetic sequence, so their rounding errors
+    largely cancel in the difference) on the 8-slot system of the perturbed
+    faces, against <dJ/dc, v> from the adjoint solve and the face kernel."""
     _say("adjoint 128^3 directional derivative vs oracle solver")
     grid, A, f = pps.make_problem("sphere128")
     og = orc.OrcGrid.make(128)
@@ -283,9 +285,9 @@ def test_adjoint_128_directional_vs_oracle_solver(pps, orc):
     def oracle_p(cc):
         Ah = np.ascontiguousarray(pps.faces_to_A(grid, cc, w, b).cpu().numpy())
         ph = np.zeros(og.shape_p)
-        its, rel = orc.pcg_solve(og, Ah, ph, rtol=1e-15, maxit=200000)
+        its, rel = orc.pcg_solve(og, Ah, ph, rtol=0.0, maxit=2000)
         _say(f"   oracle PCG {its} iterations, rel. residual {rel:.2e}")
-        assert rel <= 2e-15
+        assert rel <= 1e-14
         return torch.from_numpy(og.interior(ph).copy()).cuda()
 
     pm = [oracle_p(cf + sgn * eps * v) for sgn in (1.0, -1.0)]
