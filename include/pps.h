@@ -49,7 +49,9 @@ enum pps_status {
     PPS_ERR_BAD_ARG = 8,
     PPS_ERR_HIP = 9,               /* HIP runtime failure (out of memory, ...) */
     PPS_ERR_NO_DEVICE = 10,
-    PPS_ERR_UNSUPPORTED = 11       /* e.g. a method or option an entry point lacks */
+    PPS_ERR_UNSUPPORTED = 11,      /* e.g. a method or option an entry point lacks */
+    PPS_ERR_SCRATCH = 12           /* the grid needs more than the fixed scratch array
+                                      of pps_solve_device_noalloc */
 };
 
 /* ---- solver methods --------------------------------------------------- */
@@ -197,6 +199,22 @@ int pps_solve_fields_device(pps_handle *h, const double *u, const double *v,
                             double *dp, double maxError, int maxit, int *it,
                             double *tres2, double *u_out, double *v_out,
                             double *w_out, void *stream);
+/* pps_solve_device without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory during a
+ * solve (tight serving loops, harnesses that forbid hipMalloc).  The
+ * solver's buffers (outer vectors, operator copy, reduction partials, the
+ * V-cycle hierarchy) are taken from one fixed 128 MiB array that each GPU
+ * holds from the library's first use until the process ends; pps_init
+ * looks its address up.  Limits: methods PPS_MG_FP64 / PPS_MG_MIXED, norm 2,
+ * opt.check_setup 0, maxit >= 1 (else PPS_ERR_UNSUPPORTED / PPS_ERR_BAD_ARG);
+ * grids up to about 1.1 million cells (96^3 in either precision), larger
+ * ones return PPS_ERR_SCRATCH before any work is queued.  Every call on
+ * one GPU shares the array and nothing carries over between calls; calls
+ * that overlap in time on one GPU (several threads or streams) race.
+ * Results equal pps_solve_device's bit for bit. */
+int pps_solve_device_noalloc(pps_handle *h, const double *dA, double *dp,
+                             double maxError, double beta, int maxit, int norm,
+                             int *it, double *tres2, void *stream);
 
 /* Residual history of the last solve: tres2 after each outer iteration.
  * Returns the number of entries written (<= cap). */

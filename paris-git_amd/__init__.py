@@ -40,6 +40,7 @@ ERR_BAD_ARG = 8
 ERR_HIP = 9
 ERR_NO_DEVICE = 10
 ERR_UNSUPPORTED = 11
+ERR_SCRATCH = 12
 
 # enum pps_method / pps_outer
 MG_REFERENCE = 0
@@ -144,6 +145,7 @@ def _lib():
         "pps_new_solver": (_I, [_P, _P, _P, _D, _D, _I, _IP, _IP, _I]),
         "pps_new_solver_mg": (_I, [_P, _P, _P, _D, _D, _I, _IP, _IP, _I, _DP]),
         "pps_solve_device": (_I, [_P, _P, _P, _D, _D, _I, _I, _IP, _DP, _P]),
+        "pps_solve_device_noalloc": (_I, [_P, _P, _P, _D, _D, _I, _I, _IP, _DP, _P]),
         "pps_solve_fields_device": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _D, _D, _P, _D, _I, _IP,
                                          _DP, _P, _P, _P, _P]),
         "pps_solve_project_device": (_I, [_P, _P, _P, _D, _D, _I, _I, _IP, _DP, _P, _P, _P, _P, _P,
@@ -190,7 +192,7 @@ def exported_symbols():
     """Names of the C-ABI functions declared in include/pps.h."""
     return [
         "pps_default_options", "pps_init", "pps_finalize", "pps_set_options", "pps_set_cell_widths",
-        "pps_new_solver", "pps_new_solver_mg", "pps_solve_device",
+        "pps_new_solver", "pps_new_solver_mg", "pps_solve_device", "pps_solve_device_noalloc",
         "pps_solve_project_device", "pps_solve_fields_device",
         "pps_last_history",
         "pps_last_timing", "pps_adjoint_device", "pps_face_gradient_device",
@@ -439,6 +441,21 @@ class Solver:
                                      beta, maxit, norm,
                                      ctypes.byref(it), ctypes.byref(t2), _stream(stream))
         _check(rc, "pps_solve_device",
+               allow=(OK, ERR_NOT_CONVERGED) if allow_maxit else (OK,))
+        return it.value, t2.value
+
+    def solve_noalloc(self, A, p, maxError, beta=1.2, maxit=50, norm=2, stream=None,
+                      allow_maxit=True):
+        """solve() through pps_solve_device_noalloc: no device allocation
+        (fast methods, norm 2, grids up to about 1.1 million cells);
+        returns (it, tres2)."""
+        _need_gpu(A, p)
+        it, t2 = ctypes.c_int(), ctypes.c_double()
+        g = self.grid
+        rc = _lib().pps_solve_device_noalloc(self._h, _ptr(A, _nA(g), "A"), _ptr(p, _np(g), "p"),
+                                             maxError, beta, maxit, norm,
+                                             ctypes.byref(it), ctypes.byref(t2), _stream(stream))
+        _check(rc, "pps_solve_device_noalloc",
                allow=(OK, ERR_NOT_CONVERGED) if allow_maxit else (OK,))
         return it.value, t2.value
 

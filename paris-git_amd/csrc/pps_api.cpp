@@ -25,6 +25,7 @@ extern "C" const char *pps_status_string(int st) {
     case PPS_ERR_HIP: return "HIP runtime error";
     case PPS_ERR_NO_DEVICE: return "no HIP device";
     case PPS_ERR_UNSUPPORTED: return "unsupported by this entry point (method, option or boundary condition)";
+    case PPS_ERR_SCRATCH: return "grid too large for the no-allocation solve's fixed scratch array";
     default: return "unknown status";
     }
 }
@@ -87,6 +88,15 @@ extern "C" int pps_init(const pps_grid *grid, const pps_options *opt, pps_handle
     for (auto &e : h->ev) {
         if (hipEventCreate(&e) != hipSuccess) { pps_finalize(h); return PPS_ERR_HIP; }
     }
+    // scalar read-back buffer, and the device's scratch array address (the
+    // no-allocation solve then allocates nothing of its own)
+    if (hipHostMalloc((void **)&h->pinned, 16 * sizeof(double), 0) != hipSuccess) {
+        h->pinned = nullptr;
+        pps_finalize(h);
+        return PPS_ERR_HIP;
+    }
+    const int rc = fast_scratch_setup(h->device);
+    if (rc != PPS_OK) { pps_finalize(h); return rc; }
     *out = h;
     return PPS_OK;
 }

@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "pps_internal.h"
@@ -2014,6 +2015,9 @@ struct FastState {
     int have_r = 0;  // the setup pass already wrote r (and its norms) for x
     size_t lds_max = 65536;  // LDS bytes per workgroup on this device
     int cus = 256;           // compute units on this device
+    // no-allocation solve: every device buffer is a piece of this arena
+    // (the scratch array) instead of a hipMalloc
+    Arena *arena = nullptr;
 };
 
 void fast_invalidate(FastState *f) {
@@ -2058,7 +2062,7 @@ static int plan_levels(const Dims &d, std::vector<int> &dims) {
 }
 
 template <class T>
-static int hier_alloc(Hier<T> &H, const std::vector<int> &dims, bool own_fine) {
+static int hier_alloc(Hier<T> &H, const std::vector<int> &dims, bool own_fine, Arena *arena) {
     const int nl = (int)dims.size() / 3;
     H.lv.resize(nl);
     H.bufs.resize(nl);
@@ -2071,7 +2075,7 @@ static int hier_alloc(Hier<T> &H, const std::vector<int> &dims, bool own_fine) {
         // operator, right-hand side and iterate are borrowed from the outer
         // iteration
         const int planes = own ? 7 : 1;
-        PPS_TRY(H.bufs[l].ensure(sizeof(T) * planes * L.n));
+        PPS_TRY(H.bufs[l].ensure(sizeof(T) * planes * L.n, arena));
         T *base = H.bufs[l].template as<T>();
         if (own) {
             L.cx = base; L.cy = base + L.n; L.cz = base + 2 * L.n; L.dg = base + 3 * L.n;
@@ -2082,8 +2086,8 @@ static int hier_alloc(Hier<T> &H, const std::vector<int> &dims, bool own_fine) {
         }
     }
     H.nb = (int)H.lv.back().n;
-    PPS_TRY(H.minv.ensure(sizeof(T) * H.nb * H.nb));
-    PPS_TRY(H.work.ensure(sizeof(double) * H.nb * H.nb));
+    PPS_TRY(H.minv.ensure(sizeof(T) * H.nb * H.nb, arena));
+    PPS_TRY(H.work.ensure(sizeof(double) * H.nb * H.nb, arena));
     return PPS_OK;
 }
 
@@ -2502,10 +2506,11 @@ static int fast_ensure(pps_handle *h) {
     }
     FastState *f = h->fast;
     const long long n = f->d.cells();
-    PPS_TRY(f->op.ensure(sizeof(double) * 4 * n));
-    PPS_TRY(f->vec.ensure(sizeof(double) * 7 * n));
-    PPS_TRY(f->red.ensure(sizeof(double) * (4 * kRedBlocks + 64)));
-    PPS_TRY(f->flags.ensure(64));
+    Arena *a = f->arena;
+    PPS_TRY(f->op.ensure(sizeof(double) * 4 * n, a));
+    PPS_TRY(f->vec.ensure(sizeof(double) * 7 * n, a));
+    PPS_TRY(f->red.ensure(sizeof(double) * (4 * kRedBlocks + 64), a));
+    PPS_TRY(f->flags.ensure(64, a));
     if (!h->pinned) PPS_HIP(hipHostMalloc((void **)&h->pinned, 16 * sizeof(double), 0));
     if (!f->built) {
         int v = 0;
@@ -2562,8 +2567,8 @@ static int fast_setup(pps_handle *h, const double *dA, const double *dp, bool mi
     PPS_TRY(plan_levels(d, dims));
     float *f32 = nullptr, *r32 = nullptr;
     if (mixed) {
-        if (!f->mixed_built) { PPS_TRY(hier_alloc(f->h32, dims, true)); f->mixed_built = 1; }
-        PPS_TRY(f->b32.ensure(sizeof(float) * n));
+        if (!f->mixed_built) { PPS_TRY(hier_alloc(f->h32, dims, true, f->arena)); f->mixed_built = 1; }
+        PPS_TRY(f->b32.ensure(sizeof(float) * n, f->arena));
         f32 = f->h32.lv[0].cx;   // cx, cy, cz, dg are contiguous
         r32 = f->b32.as<float>();
     }
@@ -2605,7 +2610,7 @@ static int fast_setup(pps_handle *h, const double *dA, const double *dp, bool mi
         hier_build(f->h32, f->singular, s);
     } else {
         Hier<double> &H = f->h64;
-        if (!f->fp64_built) { PPS_TRY(hier_alloc(H, dims, false)); f->fp64_built = 1; }
+        if (!f->fp64_built) { PPS_TRY(hier_alloc(H, dims, false, f->arena)); f->fp64_built = 1; }
         Level<double> &L0 = H.lv[0];
         L0.cx = o64; L0.cy = o64 + n; L0.cz = o64 + 2 * n; L0.dg = o64 + 3 * n;
         L0.b = nullptr;
@@ -2868,4 +2873,100 @@ int fast_adjoint(pps_handle *h, const double *dA, const double *dg, double *dlam
     return last < tol ? PPS_OK : PPS_ERR_NOT_CONVERGED;
 }
 
+// ---------------------------------------------------------------------
+// No-allocation solve.  Every device buffer of the fast solve (outer
+// vectors, FP64 operator, reduction partials, the V-cycle hierarchy) is
+// carved from one fixed array, `scratch`, instead of hipMalloc.  Each GPU
+// holds its own copy of the array (kScratchBytes) from the library's load
+// until the process ends; its address on each device is looked up once,
+// by pps_init with that device current, and kept in scratch_at.
+// ---------------------------------------------------------------------
+constexpr size_t kScratchBytes = (size_t)128 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static void *scratch_at[kMaxDevices];
+static std::mutex scratch_mu;
+
+int fast_scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return PPS_ERR_NO_DEVICE;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (scratch_at[device]) return PPS_OK;
+    void *ptr = nullptr;
+    PPS_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return PPS_ERR_HIP;
+    scratch_at[device] = ptr;
+    return PPS_OK;
+}
+
+// Bytes the no-allocation solve carves for a grid, in the order and with
+// the alignment fast_ensure / fast_setup / hier_alloc take them.
+static size_t noalloc_bytes(const Dims &d, bool mixed) {
+    std::vector<int> dims;
+    if (plan_levels(d, dims) != PPS_OK) return 0;
+    const size_t n = (size_t)d.cells();
+    size_t tot = 0;
+    auto take = [&](size_t b) { tot = ((tot + 255) & ~(size_t)255) + b; };
+    take(sizeof(double) * 4 * n);
+    take(sizeof(double) * 7 * n);
+    take(sizeof(double) * (4 * (size_t)kRedBlocks + 64));
+    take(64);
+    const size_t S = mixed ? sizeof(float) : sizeof(double);
+    const int nl = (int)dims.size() / 3;
+    size_t nb = 0;
+    for (int l = 0; l < nl; ++l) {
+        const size_t nlc = (size_t)dims[3 * l] * dims[3 * l + 1] * dims[3 * l + 2];
+        take(S * ((l > 0 || mixed) ? 7 : 1) * nlc);
+        nb = nlc;
+    }
+    take(S * nb * nb);
+    take(sizeof(double) * nb * nb);
+    if (mixed) take(sizeof(float) * n);
+    return tot;
+}
+
 }  // namespace pps
+
+using namespace pps;
+
+// pps_solve_device for the fast methods (norm 2) without any device
+// allocation: the solver's buffers live in the scratch array.  Calls on one
+// GPU share that array; calls that overlap in time on one GPU race.
+extern "C" int pps_solve_device_noalloc(pps_handle *h, const double *dA, double *dp,
+                                        double maxError, double beta, int maxit, int norm,
+                                        int *it, double *tres2, void *stream) {
+    (void)beta;   // Gauss-Seidel smoothing: the fast methods take no SOR weight
+    if (!h || !dA || !dp || !it || !tres2 || maxit < 1) return PPS_ERR_BAD_ARG;
+    const int method = h->opt.method;
+    if ((method != PPS_MG_FP64 && method != PPS_MG_MIXED) || norm != 2 || h->opt.check_setup)
+        return PPS_ERR_UNSUPPORTED;
+    if (h->device < 0 || h->device >= kMaxDevices || !h->pinned) return PPS_ERR_BAD_ARG;
+    void *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(scratch_mu);
+        base = scratch_at[h->device];
+    }
+    if (!base) return PPS_ERR_HIP;   // pps_init looks the address up
+    const Dims d = dims_of(h->grid);
+    const bool mixed = method == PPS_MG_MIXED;
+    const size_t need = noalloc_bytes(d, mixed);
+    if (need == 0) return PPS_ERR_BAD_SIZE;
+    if (need > kScratchBytes) return PPS_ERR_SCRATCH;
+    PPS_HIP(hipSetDevice(h->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // a solver state whose buffers are pieces of the scratch array, used
+    // for this call only (nothing of an earlier call is trusted: another
+    // handle may have written the array since)
+    Arena arena;
+    arena.base = static_cast<char *>(base);
+    arena.cap = kScratchBytes;
+    FastState local;
+    local.d = d;
+    local.arena = &arena;
+    FastState *keep = h->fast;
+    h->fast = &local;
+    const int rc = fast_solve(h, dA, dp, maxError, maxit, 2, it, tres2, s);
+    h->fast = keep;
+    if (rc != PPS_OK) return rc;
+    return *it > maxit ? PPS_ERR_NOT_CONVERGED : PPS_OK;
+}

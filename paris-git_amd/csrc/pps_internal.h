@@ -48,21 +48,47 @@ inline Dims dims_of(const pps_grid &g) {
     return d;
 }
 
-// Device buffer owned by the handle (freed with the handle).
+// Bump allocator over a fixed device region (the no-allocation solve's
+// scratch array): hands out 256-byte aligned pieces, never frees.
+struct Arena {
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+    void *take(size_t n) {
+        const size_t at = (used + 255) & ~(size_t)255;
+        if (at > cap || n > cap - at) return nullptr;
+        used = at + n;
+        return base + at;
+    }
+};
+
+// Device buffer owned by the handle (freed with the handle), or a piece of
+// an Arena (borrowed: never freed, never reallocated).
 struct DevBuf {
     void *ptr = nullptr;
     size_t bytes = 0;
+    bool borrowed = false;
     int ensure(size_t n) {
         if (n <= bytes) return PPS_OK;
+        if (borrowed) return PPS_ERR_BAD_SIZE;
         if (ptr) (void)hipFree(ptr);
         ptr = nullptr; bytes = 0;
         if (hipMalloc(&ptr, n) != hipSuccess) return PPS_ERR_HIP;
         bytes = n;
         return PPS_OK;
     }
+    // at least n bytes: from the arena when one is given, else ensure(n)
+    int ensure(size_t n, Arena *a) {
+        if (!a) return ensure(n);
+        if (n <= bytes) return PPS_OK;
+        if (ptr && !borrowed) return PPS_ERR_BAD_ARG;   // never mix the two kinds
+        void *p = a->take(n);
+        if (!p) return PPS_ERR_SCRATCH;
+        ptr = p; bytes = n; borrowed = true;
+        return PPS_OK;
+    }
     void release() {
-        if (ptr) (void)hipFree(ptr);
-        ptr = nullptr; bytes = 0;
+        if (ptr && !borrowed) (void)hipFree(ptr);
+        ptr = nullptr; bytes = 0; borrowed = false;
     }
     template <class T> T *as() const { return static_cast<T *>(ptr); }
 };
@@ -171,6 +197,9 @@ int fast_adjoint(pps_handle *h, const double *dA, const double *dg,
 void fast_destroy(FastState *f);
 // forget the built operator (the next solve sets it up again)
 void fast_invalidate(FastState *f);
+// look up the scratch array of pps_solve_device_noalloc on `device` (the
+// current device), once per device
+int fast_scratch_setup(int device);
 
 // shared helpers (pps_assemble.hip); results are read back through the
 // handle's pinned scalars, so each call synchronises the stream.
