@@ -84,7 +84,8 @@ def test_drops512_forward_backward(pps, orc):
         d = (p - p.mean()) - (p64 - p64.mean())
         err = float(d.abs().max() / (p64 - p64.mean()).abs().max())
         _say(f"   mixed vs fp64 max-norm rel. difference {err:.2e}")
-        assert err <= 1e-8
+        # measured 5.3e-15 on an MI355X
+        assert err <= 1e-13
 
         # a directional derivative: central differences of J along a random
         # relative perturbation of every open face, against <dJ/dc, v>

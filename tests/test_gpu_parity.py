@@ -247,7 +247,9 @@ def test_sphere128_forms_agree(pps, orc):
         ps[method] = _mf(og.interior(ph))
     d = np.abs(ps["mixed"] - ps["fp64"]).max() / np.abs(ps["fp64"]).max()
     _say(f"   max-norm difference mixed vs fp64 {d:.2e}")
-    assert d <= 1e-8
+    # measured 1.3e-15 on an MI355X: both forms stop at the same FP64
+    # residual of the same CG recursion
+    assert d <= 1e-13
 
 
 @pytest.mark.parametrize("method,tol", [("fp64", 1e-12), ("mixed", 2e-5)])
