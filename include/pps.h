@@ -405,10 +405,12 @@ int pps_last_timing(pps_handle *h, double *solve_ms, double *setup_ms);
 
 /* Measurement hooks (no reference counterpart; the reference's analogue is
  * its wall-clock timer bucket "Poisson for pressure", modules.f90:252-296).
- * When enabled, every finest-level smoother launch of the fast methods is
- * bracketed by a HIP event pair on the solver's stream; read returns the
- * summed kernel milliseconds, the launch count and the summed algorithmic
- * bytes since the last read, and resets them. */
+ * When enabled (on = 1), every timed finest-level smoother launch of the
+ * fast methods (the plain sweep: not the first of a cycle, not the one that
+ * adds the coarse correction or forms the dot products) carries a HIP event
+ * pair, the kernel's own start and end (on = N > 1: every N-th such launch);
+ * read returns the summed kernel milliseconds, the launch count and the
+ * summed algorithmic bytes since the last read, and resets them. */
 int pps_profile_enable(pps_handle *h, int on);
 int pps_profile_read(pps_handle *h, double *kernel_ms, long long *launches,
                      double *bytes);

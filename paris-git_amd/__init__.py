@@ -526,9 +526,11 @@ class Solver:
         _lib().pps_last_timing(self._h, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
 
-    def profile(self, on=True):
-        """Bracket every finest-level smoother launch with HIP events."""
-        _check(_lib().pps_profile_enable(self._h, int(on)), "pps_profile_enable")
+    def profile(self, on=True, every=1):
+        """Time finest-level smoother launches with HIP events (the kernel's
+        own start and end); every > 1 times one launch in `every`."""
+        _check(_lib().pps_profile_enable(self._h, (max(int(every), 1) if on else 0)),
+               "pps_profile_enable")
 
     def profile_read(self):
         """(kernel_ms, launches, algorithmic_bytes) since the last read."""

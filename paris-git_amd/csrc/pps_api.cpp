@@ -88,6 +88,14 @@ extern "C" int pps_init(const pps_grid *grid, const pps_options *opt, pps_handle
     for (auto &e : h->ev) {
         if (hipEventCreate(&e) != hipSuccess) { pps_finalize(h); return PPS_ERR_HIP; }
     }
+    if (hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess) {
+        h->side = nullptr;
+        pps_finalize(h);
+        return PPS_ERR_HIP;
+    }
+    for (auto &e : h->sev) {
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) { pps_finalize(h); return PPS_ERR_HIP; }
+    }
     // scalar read-back buffer, and the device's scratch array address (the
     // no-allocation solve then allocates nothing of its own)
     if (hipHostMalloc((void **)&h->pinned, 16 * sizeof(double), 0) != hipSuccess) {
@@ -95,6 +103,12 @@ extern "C" int pps_init(const pps_grid *grid, const pps_options *opt, pps_handle
         pps_finalize(h);
         return PPS_ERR_HIP;
     }
+    if (hipHostMalloc((void **)&h->mbox, sizeof(pps::Mailbox), hipHostMallocCoherent) != hipSuccess) {
+        h->mbox = nullptr;
+        pps_finalize(h);
+        return PPS_ERR_HIP;
+    }
+    h->mbox->seq = 0;
     const int rc = fast_scratch_setup(h->device);
     if (rc != PPS_OK) { pps_finalize(h); return rc; }
     *out = h;
@@ -104,6 +118,7 @@ extern "C" int pps_init(const pps_grid *grid, const pps_options *opt, pps_handle
 extern "C" int pps_finalize(pps_handle *h) {
     if (!h) return PPS_ERR_BAD_ARG;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->side) (void)hipStreamSynchronize(h->side);
     ref_destroy(h->ref);
     fast_destroy(h->fast);
     h->hA.release();
@@ -115,10 +130,17 @@ extern "C" int pps_finalize(pps_handle *h) {
     h->asplit.release();
     h->red.release();
     if (h->pinned) (void)hipHostFree(h->pinned);
+    if (h->mbox) (void)hipHostFree(h->mbox);
     for (auto &e : h->prof.ev)
         if (e) (void)hipEventDestroy(e);
     for (auto &e : h->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto &e : h->sev)
+        if (e) (void)hipEventDestroy(e);
+    if (h->side) {
+        (void)hipStreamSynchronize(h->side);
+        (void)hipStreamDestroy(h->side);
+    }
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return PPS_OK;
@@ -442,6 +464,8 @@ extern "C" int pps_profile_enable(pps_handle *h, int on) {
         for (auto &e : h->prof.ev) PPS_HIP(hipEventCreate(&e));
     }
     h->prof.on = on ? 1 : 0;
+    h->prof.every = on > 1 ? on : 1;
+    h->prof.seen = 0;
     return PPS_OK;
 }
 

@@ -27,9 +27,12 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <utility>
 #include <vector>
 
 #include "pps_internal.h"
+
+#include <hip/hip_ext.h>
 #include "pps_row.h"
 
 namespace pps {
@@ -233,7 +236,11 @@ template <class T, bool FIRST, int PX, bool FINE, bool DGI, bool PROL, bool PF, 
           bool LOP = false>
 __global__ void __launch_bounds__(LOP ? kLopThreads : 1024)
 k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *__restrict__ xout,
-           int c1, int zc, int oy, int ytiles, const T *__restrict__ ec, double *__restrict__ part) {
+           int c1, int zc, int oy, int ytiles, const T *__restrict__ ec, double *__restrict__ part,
+           const double *__restrict__ skip) {
+    // a first sweep queued ahead of the outer iteration's convergence test
+    // does nothing once that test has stopped the iteration
+    if (FIRST && skip && *skip != 0.0) return;
     extern __shared__ unsigned char smem_raw[];
     T *sm = reinterpret_cast<T *>(smem_raw);
     const int nx = L.nx, ny = L.ny, nz = L.nz;
@@ -1319,7 +1326,7 @@ k_tail(TailK<T> K) {
 // ---------------------------------------------------------------------
 // FP64 outer iteration kernels.  Scalars live on the device:
 // ---------------------------------------------------------------------
-enum { S_RZ = 0, S_ZMEAN, S_BETA, S_DQ, S_ALPHA, S_RR, S_BB, S_FIRST, S_N };
+enum { S_RZ = 0, S_ZMEAN, S_BETA, S_DQ, S_ALPHA, S_RR, S_BB, S_FIRST, S_STOP, S_N };
 
 // 2.5-D launch shape for z-marching kernels: (64, 4) threads over (x, y),
 // each thread walks `zc` consecutive planes; at most kRedBlocks blocks so
@@ -1660,13 +1667,29 @@ k_sub_mean(long long n, double *__restrict__ b, const double *__restrict__ part,
 }
 
 __global__ void __launch_bounds__(1024)
-k_scal_rr(const double *__restrict__ part, int nb, int with_bb, double *__restrict__ S) {
+k_scal_rr(const double *__restrict__ part, int nb, int with_bb, double *__restrict__ S,
+          Mailbox *mb = nullptr, unsigned long long seq = 0, const int *flags = nullptr,
+          double N = 1.0, double tol = 0.0) {
     const double rr = sum_partials(part, nb);
     double bb = 0.0;
     if (with_bb) bb = sum_partials(part + kRedBlocks, nb);
     if (threadIdx.x == 0) {
         S[S_RR] = rr;
         if (with_bb) S[S_BB] = bb;
+        // the outer iteration's stopping test, sqrt(rr / N) < tol (tol > 0),
+        // for the first sweep already queued behind this kernel; the host
+        // takes the same decision from the mailbox
+        const double stop = (tol > 0.0 && sqrt(rr / N) < tol) ? 1.0 : 0.0;
+        if (tol > 0.0) S[S_STOP] = stop;
+        if (mb) {
+            // the values, then seq with a system-scope release: a host that
+            // reads seq (acquire) sees them
+            mb->rr = rr;
+            mb->bb = bb;
+            mb->flags = flags ? (double)*flags : 0.0;
+            mb->pad = stop;
+            __hip_atomic_store(&mb->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
     }
 }
 
@@ -2013,6 +2036,11 @@ struct FastState {
     // compact operator is not written)
     int op_in_A = 0;
     int have_r = 0;  // the setup pass already wrote r (and its norms) for x
+    // deferred setup: the setup pass's flags are still to be read (at the
+    // next synchronisation), and the coarse hierarchy is being built on
+    // the handle's side stream (the solve stream waits for sev[1] before
+    // the first coarse-level kernel)
+    int flags_pending = 0, coarse_pending = 0;
     size_t lds_max = 65536;  // LDS bytes per workgroup on this device
     int cus = 256;           // compute units on this device
     // no-allocation solve: every device buffer is a piece of this arena
@@ -2259,32 +2287,44 @@ struct Cycle {
 // swap).  Algorithmic bytes: the operator planes (3 couplings, plus the
 // diagonal unless dgi), b, the old iterate (unless FIRST) and the new
 // iterate, once each.
+// Launch of a fused sweep; with ev, the kernel's own start and end are
+// recorded in ev[0], ev[1] (hipExtLaunchKernelGGL: the timestamps of the
+// dispatch itself, no marker packets between kernels).
+template <class... KArgs, class... Args>
+static void launch_sweep(void (*k)(KArgs...), const SweepShape &sh, hipStream_t s, hipEvent_t *ev,
+                         Args... args) {
+    if (ev)
+        hipExtLaunchKernelGGL(k, sh.grid, sh.block, (std::uint32_t)sh.lds, s, ev[0], ev[1], 0, args...);
+    else
+        hipLaunchKernelGGL(k, sh.grid, sh.block, sh.lds, s, args...);
+}
+
 template <class T, bool FIRST, bool DGI, bool PROL, int PX>
 static void sweep_lop(const SweepShape &sh, const LevelK<T> &K, const Level<T> &L, int l, const T *b,
-                      int c1, const T *ec, double *dot, hipStream_t s) {
+                      int c1, const T *ec, double *dot, hipStream_t s, hipEvent_t *ev, const double *skip) {
     if constexpr (!FIRST) {
         if (l == 0 && dot) {
-            k_rb_sweep<T, FIRST, PX, true, DGI, PROL, true, true, true><<<sh.grid, sh.block, sh.lds, s>>>(
-                K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec, dot);
+            launch_sweep(k_rb_sweep<T, FIRST, PX, true, DGI, PROL, true, true, true>, sh, s, ev,
+                K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec, dot, skip);
             return;
         }
     }
     if (l == 0)
-        k_rb_sweep<T, FIRST, PX, true, DGI, PROL, true, false, true><<<sh.grid, sh.block, sh.lds, s>>>(
-            K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec, nullptr);
+        launch_sweep(k_rb_sweep<T, FIRST, PX, true, DGI, PROL, true, false, true>, sh, s, ev,
+            K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec, nullptr, skip);
     else
-        k_rb_sweep<T, FIRST, PX, false, DGI, PROL, true, false, true><<<sh.grid, sh.block, sh.lds, s>>>(
-            K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec, nullptr);
+        launch_sweep(k_rb_sweep<T, FIRST, PX, false, DGI, PROL, true, false, true>, sh, s, ev,
+            K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec, nullptr, skip);
 }
 template <class T, bool FIRST, bool DGI, bool PROL, bool PF>
 static void sweep_launch2(const SweepShape &sh, const Level<T> &L, int l, const T *b, int c1,
-                          const T *ec, double *dot, hipStream_t s) {
+                          const T *ec, double *dot, hipStream_t s, hipEvent_t *ev, const double *skip) {
     const LevelK<T> K = L.k();
     if constexpr (sizeof(T) == 4 && DGI) {
         // couplings staged through LDS
         if (sh.lop) {
-            if (sh.px == 2) sweep_lop<T, FIRST, DGI, PROL, 2>(sh, K, L, l, b, c1, ec, dot, s);
-            else sweep_lop<T, FIRST, DGI, PROL, 1>(sh, K, L, l, b, c1, ec, dot, s);
+            if (sh.px == 2) sweep_lop<T, FIRST, DGI, PROL, 2>(sh, K, L, l, b, c1, ec, dot, s, ev, skip);
+            else sweep_lop<T, FIRST, DGI, PROL, 1>(sh, K, L, l, b, c1, ec, dot, s, ev, skip);
             return;
         }
         // two pairs per thread, the LDS buffers not fitting (no run-ahead
@@ -2292,49 +2332,49 @@ static void sweep_launch2(const SweepShape &sh, const Level<T> &L, int l, const 
         if (sh.px == 2) {
             if constexpr (!FIRST) {
                 if (l == 0 && dot) {
-                    k_rb_sweep<T, FIRST, 2, true, DGI, PROL, false, true><<<sh.grid, sh.block, sh.lds, s>>>(
-                        K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec, dot);
+                    launch_sweep(k_rb_sweep<T, FIRST, 2, true, DGI, PROL, false, true>, sh, s, ev,
+                        K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec, dot, skip);
                     return;
                 }
             }
             // the first sweep reads no iterate, which leaves registers for
             // the run-ahead loads (115 VGPRs; 89 against 95 us at 256^3)
             if (l == 0)
-                k_rb_sweep<T, FIRST, 2, true, DGI, PROL, FIRST, false><<<sh.grid, sh.block, sh.lds, s>>>(
-                    K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec, nullptr);
+                launch_sweep(k_rb_sweep<T, FIRST, 2, true, DGI, PROL, FIRST, false>, sh, s, ev,
+                    K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec, nullptr, skip);
             else
-                k_rb_sweep<T, FIRST, 2, false, DGI, PROL, false, false><<<sh.grid, sh.block, sh.lds, s>>>(
-                    K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec, nullptr);
+                launch_sweep(k_rb_sweep<T, FIRST, 2, false, DGI, PROL, false, false>, sh, s, ev,
+                    K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec, nullptr, skip);
             return;
         }
     }
     if constexpr (!FIRST) {
         if (l == 0 && dot) {
-            k_rb_sweep<T, FIRST, 1, true, DGI, PROL, PF, true><<<sh.grid, sh.block, sh.lds, s>>>(
-                K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec, dot);
+            launch_sweep(k_rb_sweep<T, FIRST, 1, true, DGI, PROL, PF, true>, sh, s, ev,
+                K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec, dot, skip);
             return;
         }
     }
     if (l == 0)
-        k_rb_sweep<T, FIRST, 1, true, DGI, PROL, PF, false><<<sh.grid, sh.block, sh.lds, s>>>(
-            K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec, nullptr);
+        launch_sweep(k_rb_sweep<T, FIRST, 1, true, DGI, PROL, PF, false>, sh, s, ev,
+            K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec, nullptr, skip);
     else
-        k_rb_sweep<T, FIRST, 1, false, DGI, PROL, PF, false><<<sh.grid, sh.block, sh.lds, s>>>(
-            K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec, nullptr);
+        launch_sweep(k_rb_sweep<T, FIRST, 1, false, DGI, PROL, PF, false>, sh, s, ev,
+            K, b, L.x, L.xt, c1, sh.zc, sh.oy, sh.ytiles, ec, nullptr, skip);
 }
 // Loads run ahead in FP32 only: the FP64 kernel with both operator sets
 // exceeds 128 VGPRs and spills (226 us against 198 us at 256^3).
 template <class T, bool FIRST, bool DGI, bool PROL>
 static void sweep_launch1(const SweepShape &sh, const Level<T> &L, int l, const T *b, int c1,
-                          const T *ec, double *dot, hipStream_t s) {
-    sweep_launch2<T, FIRST, DGI, PROL, sizeof(T) == 4>(sh, L, l, b, c1, ec, dot, s);
+                          const T *ec, double *dot, hipStream_t s, hipEvent_t *ev, const double *skip) {
+    sweep_launch2<T, FIRST, DGI, PROL, sizeof(T) == 4>(sh, L, l, b, c1, ec, dot, s, ev, skip);
 }
 template <class T, bool DGI>
 static void sweep_launch(const SweepShape &sh, const Level<T> &L, int l, const T *b, int c1, bool first,
-                         const T *ec, double *dot, hipStream_t s) {
-    if (first) sweep_launch1<T, true, DGI, false>(sh, L, l, b, c1, nullptr, nullptr, s);
-    else if (ec) sweep_launch1<T, false, DGI, true>(sh, L, l, b, c1, ec, dot, s);
-    else sweep_launch1<T, false, DGI, false>(sh, L, l, b, c1, nullptr, dot, s);
+                         const T *ec, double *dot, hipStream_t s, hipEvent_t *ev, const double *skip) {
+    if (first) sweep_launch1<T, true, DGI, false>(sh, L, l, b, c1, nullptr, nullptr, s, ev, skip);
+    else if (ec) sweep_launch1<T, false, DGI, true>(sh, L, l, b, c1, ec, dot, s, ev, skip);
+    else sweep_launch1<T, false, DGI, false>(sh, L, l, b, c1, nullptr, dot, s, ev, skip);
 }
 
 // Whether level l's sweeps run as one fused kernel (else two colour passes).
@@ -2363,7 +2403,8 @@ static int sweep_blocks(const Level<T> &L, const Cycle &cy) {
 // only); dot: partial sums of b x, x, b of the result (sweep_dot_ok only).
 template <class T>
 static void rb_sweep(Level<T> &L, int l, const T *b, int c1, bool first, const Cycle &cy, Prof *prof,
-                     hipStream_t s, const T *ec = nullptr, double *dot = nullptr) {
+                     hipStream_t s, const T *ec = nullptr, double *dot = nullptr,
+                     const double *skip = nullptr) {
     const bool dgi = cy.dgi;
     const SweepShape sh = sweep_shape<T>(L.nx, L.ny, L.nz, cy.rows, cy.planes, cy.cus, cy.dgi, cy.lds_max);
     if (!sweep_fused(L, cy)) {
@@ -2376,23 +2417,35 @@ static void rb_sweep(Level<T> &L, int l, const T *b, int c1, bool first, const C
     // nor the one that adds the coarse correction
     const double bytes = (double)L.n * sizeof(T) * (dgi ? 6.0 : 7.0);
     hipEvent_t *e = (l == 0 && !first && !ec && !dot && prof) ? prof->begin(bytes) : nullptr;
-    if (e) (void)hipEventRecord(e[0], s);
-    if (dgi) sweep_launch<T, true>(sh, L, l, b, c1, first, ec, dot, s);
-    else sweep_launch<T, false>(sh, L, l, b, c1, first, ec, dot, s);
-    if (e) (void)hipEventRecord(e[1], s);
+    if (dgi) sweep_launch<T, true>(sh, L, l, b, c1, first, ec, dot, s, e, skip);
+    else sweep_launch<T, false>(sh, L, l, b, c1, first, ec, dot, s, e, skip);
     T *t = L.x;
     L.x = L.xt;
     L.xt = t;
+}
+
+// The finest level's first pre-sweep of a V-cycle on its own, queued ahead
+// by the outer iteration; it returns at once when *skip is non-zero.
+template <class T>
+static void queue_first_sweep(Hier<T> &H, const T *fineb, const Cycle &cy, hipStream_t s, const double *skip) {
+    rb_sweep<T>(H.lv[0], 0, fineb, 0, true, cy, nullptr, s, nullptr, nullptr, skip);
+}
+// Whether the look-ahead applies: the finest level runs fused sweeps (one
+// kernel per sweep) and the cycle has a level below it.
+template <class T>
+static bool can_queue_first(const Hier<T> &H, const Cycle &cy) {
+    return cy.fused && H.lv.size() > 1 && sweep_fused(H.lv[0], cy);
 }
 
 // one V-cycle: x_0 = V(b_0).  fineb: the fine right-hand side (the FP64
 // residual in FP64 mode, its FP32 copy in mixed mode).  dot: when not null
 // and the last fine sweep is a fused one that can, that sweep writes the
 // block sums of b_0 x_0, x_0 and b_0 there (as k_dot_rz lays them out);
-// returns their block count, 0 if it did not.
+// returns their block count, 0 if it did not.  first_done: the finest
+// level's first pre-sweep is already queued (queue_first_sweep).
 template <class T>
 static int vcycle(Hier<T> &H, const T *fineb, const Cycle &cy, Prof *prof, hipStream_t s,
-                  double *dot = nullptr) {
+                  double *dot = nullptr, hipEvent_t coarse_ready = nullptr, bool first_done = false) {
     const int nl = (int)H.lv.size();
     const int nu_pre = cy.nu_pre, nu_post = cy.nu_post;
     const bool dgi = cy.dgi, fused = cy.fused;
@@ -2413,7 +2466,7 @@ static int vcycle(Hier<T> &H, const T *fineb, const Cycle &cy, Prof *prof, hipSt
         Level<T> &L = H.lv[l];
         const T *b = l == 0 ? fineb : L.b;
         if (fused) {
-            rb_sweep(L, l, b, 0, true, cy, prof, s);
+            if (!(l == 0 && first_done)) rb_sweep(L, l, b, 0, true, cy, prof, s);
             for (int q = 1; q < nu_pre; ++q) rb_sweep(L, l, b, 0, false, cy, prof, s);
         } else {
             rb_pass(L, l, b, 0, true, dgi, prof, s);
@@ -2438,7 +2491,15 @@ static int vcycle(Hier<T> &H, const T *fineb, const Cycle &cy, Prof *prof, hipSt
             if (dgi) k_resid_restrict<T, T, true><<<cgrid3(C.nx, C.ny, C.nz), kB3, 0, s>>>(L.k(), b, L.x, C.b);
             else k_resid_restrict<T, T, false><<<cgrid3(C.nx, C.ny, C.nz), kB3, 0, s>>>(L.k(), b, L.x, C.b);
         }
+        // the coarse operators (built on the side stream) are needed from
+        // here on: the finest level's smoothing and restriction only wrote
+        // the coarse right-hand side
+        if (l == 0 && coarse_ready) {
+            (void)hipStreamWaitEvent(s, coarse_ready, 0);
+            coarse_ready = nullptr;
+        }
     }
+    if (coarse_ready) (void)hipStreamWaitEvent(s, coarse_ready, 0);
     if (lt < nl) {
         TailK<T> K;
         K.nl = nl - lt;
@@ -2554,10 +2615,37 @@ static OpK opk(FastState *f) {
     return OpK{f->d.nx, f->d.ny, f->d.nz, o, o + n, o + 2 * n, o + 3 * n, px, py, pz};
 }
 
+// Waits until the mailbox carries the last requested seq (a k_scal_rr on
+// stream s wrote it).  Spins on the coherent host word; now and then asks
+// whether the stream has drained, so that a failed launch ends the wait.
+static int wait_mailbox(pps_handle *h, hipStream_t s) {
+    const unsigned long long want = h->mseq;
+    for (long long spin = 1;; ++spin) {
+        if (__atomic_load_n(&h->mbox->seq, __ATOMIC_ACQUIRE) == want) return PPS_OK;
+        if ((spin & 255) == 0) {
+            const hipError_t e = hipStreamQuery(s);
+            if (e == hipSuccess) {
+                if (__atomic_load_n(&h->mbox->seq, __ATOMIC_ACQUIRE) == want) return PPS_OK;
+                std::fprintf(stderr, "libpps: the stream drained without the expected reduction\n");
+                return PPS_ERR_HIP;
+            }
+            if (e != hipErrorNotReady) PPS_HIP(e);
+        }
+        __builtin_ia32_pause();
+    }
+}
+
 // pack A (or, given rows, the system assembled from the fields), detect
 // symmetry/singularity, build the hierarchy for `mixed`
+//
+// defer: do not wait for the setup pass.  Its flags are copied to the
+// pinned scalars and read at the next synchronisation (outer_iterate), and
+// the coarse hierarchy is built on the side stream for a singular system
+// (every PARIS wall / periodic system is), overlapping the start of the
+// first V-cycle; outer_iterate builds it again on the solve stream if the
+// flags say the system is not singular.
 static int fast_setup(pps_handle *h, const double *dA, const double *dp, bool mixed, hipStream_t s,
-                      const RowK *rows = nullptr) {
+                      const RowK *rows = nullptr, bool defer = false) {
     FastState *f = h->fast;
     const Dims &d = f->d;
     const long long n = d.cells();
@@ -2595,28 +2683,45 @@ static int fast_setup(pps_handle *h, const double *dA, const double *dp, bool mi
         k_setup<RowsA><<<zg.grid, dim3(64, 4, 1), 0, s>>>(packk(d), zg.zc, src, dp, ocx, ocy, ocz, odg, V.b,
                                                           V.x, V.r, f32, r32, part, flags, wx, wy, wz);
     }
-    k_scal_rr<<<1, 1024, 0, s>>>(part, zg.blocks, 1, S);
+    // deferred: r.r, b.b and the flags go to the mailbox as well
+    if (defer) k_scal_rr<<<1, 1024, 0, s>>>(part, zg.blocks, 1, S, h->mbox, ++h->mseq, flags);
+    else k_scal_rr<<<1, 1024, 0, s>>>(part, zg.blocks, 1, S);
     PPS_HIP(hipGetLastError());
-    int fl = 0;
-    PPS_HIP(hipMemcpyAsync(&fl, flags, sizeof(int), hipMemcpyDeviceToHost, s));
-    PPS_HIP(hipStreamSynchronize(s));
-    if (fl & 1) return PPS_ERR_NAN_OR_TINY_DIAG;
-    if (fl & 2) return PPS_ERR_NONSYMMETRIC;
-    f->singular = (fl & 4) ? 0 : 1;
-    f->exact = (fl & 8) ? 0 : 1;
-    f->have_r = 1;
-
-    if (mixed) {
-        hier_build(f->h32, f->singular, s);
+    // the hierarchy of the chosen precision, built on stream st
+    auto build = [&](int singular, hipStream_t st) -> int {
+        if (mixed) {
+            hier_build(f->h32, singular, st);
+        } else {
+            Hier<double> &H = f->h64;
+            if (!f->fp64_built) { PPS_TRY(hier_alloc(H, dims, false, f->arena)); f->fp64_built = 1; }
+            Level<double> &L0 = H.lv[0];
+            L0.cx = o64; L0.cy = o64 + n; L0.cz = o64 + 2 * n; L0.dg = o64 + 3 * n;
+            L0.b = nullptr;
+            L0.x = V.z;
+            hier_build(H, singular, st);
+        }
+        return PPS_OK;
+    };
+    if (defer) {
+        PPS_HIP(hipEventRecord(h->sev[0], s));
+        PPS_HIP(hipStreamWaitEvent(h->side, h->sev[0], 0));
+        PPS_TRY(build(1, h->side));
+        PPS_HIP(hipEventRecord(h->sev[1], h->side));
+        f->singular = 1;   // provisional until the flags are read
+        f->exact = 1;
+        f->flags_pending = 1;
+        f->coarse_pending = 1;
     } else {
-        Hier<double> &H = f->h64;
-        if (!f->fp64_built) { PPS_TRY(hier_alloc(H, dims, false, f->arena)); f->fp64_built = 1; }
-        Level<double> &L0 = H.lv[0];
-        L0.cx = o64; L0.cy = o64 + n; L0.cz = o64 + 2 * n; L0.dg = o64 + 3 * n;
-        L0.b = nullptr;
-        L0.x = V.z;
-        hier_build(H, f->singular, s);
+        int fl = 0;
+        PPS_HIP(hipMemcpyAsync(&fl, flags, sizeof(int), hipMemcpyDeviceToHost, s));
+        PPS_HIP(hipStreamSynchronize(s));
+        if (fl & 1) return PPS_ERR_NAN_OR_TINY_DIAG;
+        if (fl & 2) return PPS_ERR_NONSYMMETRIC;
+        f->singular = (fl & 4) ? 0 : 1;
+        f->exact = (fl & 8) ? 0 : 1;
+        PPS_TRY(build(f->singular, s));
     }
+    f->have_r = 1;
     PPS_HIP(hipGetLastError());
     // an operator assembled from fields is not kept for a later adjoint
     f->built_for = rows ? nullptr : dA;
@@ -2644,6 +2749,41 @@ static int outer_iterate(pps_handle *h, Hier<T> &H, bool mixed, double tol_abs, 
     float *b32 = mixed ? f->b32.as<float>() : nullptr;
     const double N = (double)n;
     const bool cg = h->opt.outer == PPS_OUTER_CG;
+
+    // r = b - M x (unless the setup pass just computed it)
+    if (!f->have_r) {
+        if (f->exact) k_outer_resid<float, true><<<zg.grid, zb, 0, s>>>(O, zg.zc, V.b, V.x, V.r, b32, part);
+        else k_outer_resid<float, false><<<zg.grid, zb, 0, s>>>(O, zg.zc, V.b, V.x, V.r, b32, part);
+        k_scal_rr<<<1, 1024, 0, s>>>(part, zg.blocks, 1, S, h->mbox, ++h->mseq);
+    }
+    f->have_r = 0;
+    k_fill<<<1, 64, 0, s>>>(S + S_FIRST, 1, 1.0);
+    k_fill<<<1, 64, 0, s>>>(S + S_ALPHA, 1, 0.0);   // no pending x update
+    k_fill<<<1, 64, 0, s>>>(S + S_STOP, 1, 0.0);
+    PPS_HIP(hipGetLastError());
+    // r.r of the setup pass or of the residual above
+    PPS_TRY(wait_mailbox(h, s));
+    const double rr0 = h->mbox->rr;
+    if (f->flags_pending) {
+        // a deferred setup: its flags have arrived with r.r
+        f->flags_pending = 0;
+        const int fl = (int)h->mbox->flags;
+        if (fl & 3) {
+            PPS_HIP(hipStreamSynchronize(h->side));
+            f->coarse_pending = 0;
+            return (fl & 1) ? PPS_ERR_NAN_OR_TINY_DIAG : PPS_ERR_NONSYMMETRIC;
+        }
+        f->singular = (fl & 4) ? 0 : 1;
+        f->exact = (fl & 8) ? 0 : 1;
+        if (!f->singular) {
+            // built for a singular system on the side stream: build again
+            // with the stored diagonals, after that build
+            PPS_HIP(hipStreamWaitEvent(s, h->sev[1], 0));
+            hier_build(H, 0, s);
+            PPS_HIP(hipGetLastError());
+            f->coarse_pending = 0;
+        }
+    }
     Cycle cy;
     cy.nu_pre = h->opt.nu_pre < 1 ? 1 : h->opt.nu_pre;
     cy.nu_post = h->opt.nu_post;
@@ -2656,27 +2796,29 @@ static int outer_iterate(pps_handle *h, Hier<T> &H, bool mixed, double tol_abs, 
     cy.cus = f->cus;
     cy.tri = h->opt.transfer == PPS_TRANSFER_TRILINEAR;
     for (int a = 0; a < 3; ++a) cy.per[a] = f->d.per[a];
-
-    // r = b - M x (unless the setup pass just computed it)
-    if (!f->have_r) {
-        if (f->exact) k_outer_resid<float, true><<<zg.grid, zb, 0, s>>>(O, zg.zc, V.b, V.x, V.r, b32, part);
-        else k_outer_resid<float, false><<<zg.grid, zb, 0, s>>>(O, zg.zc, V.b, V.x, V.r, b32, part);
-        k_scal_rr<<<1, 1024, 0, s>>>(part, zg.blocks, 1, S);
-    }
-    f->have_r = 0;
-    k_fill<<<1, 64, 0, s>>>(S + S_FIRST, 1, 1.0);
-    k_fill<<<1, 64, 0, s>>>(S + S_ALPHA, 1, 0.0);   // no pending x update
-    PPS_HIP(hipGetLastError());
-    PPS_HIP(hipMemcpyAsync(h->pinned, S + S_RR, sizeof(double), hipMemcpyDeviceToHost, s));
-    PPS_HIP(hipStreamSynchronize(s));
-    double t2 = std::sqrt(h->pinned[0] / N);
+    double t2 = std::sqrt(rr0 / N);
     *last = t2;
     int k = 1, cur = 0;
-    if (t2 == 0.0) { *it = 1; return PPS_OK; }   // exact solution: nothing to do
+    if (t2 == 0.0) {   // exact solution: nothing to do
+        if (f->coarse_pending) PPS_HIP(hipStreamSynchronize(h->side));
+        f->coarse_pending = 0;
+        *it = 1;
+        return PPS_OK;
+    }
+    // Look-ahead: the next cycle's first fine sweep (it reads only the new
+    // right-hand side) is queued behind each iteration's convergence test,
+    // before the host reads that test, so the device does not idle while
+    // the host decides; once the test stops the iteration the sweep returns
+    // at once (S_STOP), and the device and the host take the same decision.
+    const T *fb = mixed ? (const T *)(const void *)b32 : (const T *)(const void *)V.r;
+    const bool ahead = can_queue_first(H, cy);
+    bool queued = false;
     for (k = 1; k <= maxit; ++k) {
         // z = V(r)
-        const T *fb = mixed ? (const T *)(const void *)b32 : (const T *)(const void *)V.r;
-        const int ndot = vcycle<T>(H, fb, cy, &h->prof, s, cg ? part : nullptr);
+        const int ndot = vcycle<T>(H, fb, cy, &h->prof, s, cg ? part : nullptr,
+                                   f->coarse_pending ? h->sev[1] : nullptr, queued);
+        f->coarse_pending = 0;
+        queued = false;
         const T *z = H.lv[0].x;   // the sweeps swap buffers: read after the cycle
         if (cg) {
             // (r, z), sum z, sum r: from the last sweep (with r as the cycle
@@ -2696,23 +2838,33 @@ static int outer_iterate(pps_handle *h, Hier<T> &H, bool mixed, double tol_abs, 
                                                                            part);
             k_scal_alpha<<<1, 1024, 0, s>>>(part, tg.blocks, S);
             k_update<float><<<nb, 256, 0, s>>>(n, V.r, V.q, b32, S, part);
-            k_scal_rr<<<1, 1024, 0, s>>>(part, nb, 0, S);
+            k_scal_rr<<<1, 1024, 0, s>>>(part, nb, 0, S, h->mbox, ++h->mseq, nullptr, N, tol_abs);
             cur ^= 1;
         } else {
             k_zsum<T><<<nb, 256, 0, s>>>(n, z, part);
             k_add_z<T><<<nb, 256, 0, s>>>(n, V.x, z, part, nb, f->singular);
             if (f->exact) k_outer_resid<float, true><<<zg.grid, zb, 0, s>>>(O, zg.zc, V.b, V.x, V.r, b32, part);
             else k_outer_resid<float, false><<<zg.grid, zb, 0, s>>>(O, zg.zc, V.b, V.x, V.r, b32, part);
-            k_scal_rr<<<1, 1024, 0, s>>>(part, zg.blocks, 0, S);
+            k_scal_rr<<<1, 1024, 0, s>>>(part, zg.blocks, 0, S, h->mbox, ++h->mseq, nullptr, N, tol_abs);
+        }
+        if (ahead && k < maxit) {
+            queue_first_sweep(H, fb, cy, s, S + S_STOP);
+            queued = true;
         }
         PPS_HIP(hipGetLastError());
-        PPS_HIP(hipMemcpyAsync(h->pinned, S + S_RR, sizeof(double), hipMemcpyDeviceToHost, s));
-        PPS_HIP(hipStreamSynchronize(s));
-        t2 = std::sqrt(h->pinned[0] / N);
+        PPS_TRY(wait_mailbox(h, s));
+        t2 = std::sqrt(h->mbox->rr / N);
+        const bool stop = h->mbox->pad != 0.0;
         if (record) h->hist.push_back(t2);
         *last = t2;
         if (!(t2 == t2) || t2 > 1e300) { *it = k; return PPS_ERR_DIVERGED; }
-        if (t2 < tol_abs) break;
+        if (stop) {
+            // the queued sweep returned without writing: undo its swap of
+            // the finest level's iterate buffers
+            if (queued) std::swap(H.lv[0].x, H.lv[0].xt);
+            queued = false;
+            break;
+        }
     }
     // the last iteration's x += alpha d
     if (cg) k_axpy_d<T><<<nb, 256, 0, s>>>(n, V.x, (const T *)(const void *)V.d[cur], S);
@@ -2733,7 +2885,7 @@ int fast_solve(pps_handle *h, const double *dA, double *dp, double maxError, int
     const bool mixed = h->opt.method == PPS_MG_MIXED;
     h->hist.clear();
     PPS_HIP(hipEventRecord(h->ev[0], s));
-    PPS_TRY(fast_setup(h, dA, dp, mixed, s, rows));
+    PPS_TRY(fast_setup(h, dA, dp, mixed, s, rows, true));
     PPS_HIP(hipEventRecord(h->ev[1], s));
     const Dims &d = f->d;
     const double N = (double)d.cells();

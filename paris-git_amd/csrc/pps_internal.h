@@ -107,6 +107,8 @@ struct FastState;                  // fast MG (pps_fast.hip)
 // collected after the solve has synchronised.
 struct Prof {
     int on = 0;
+    int every = 1;          // time every `every`-th eligible launch
+    long long seen = 0;     // eligible launches since enabled
     std::vector<hipEvent_t> ev;
     std::vector<double> pend_bytes;
     int used = 0;
@@ -116,6 +118,7 @@ struct Prof {
     // or nullptr when not profiling / out of events
     hipEvent_t *begin(double b) {
         if (!on || used + 2 > (int)ev.size()) return nullptr;
+        if (seen++ % every) return nullptr;
         pend_bytes.push_back(b);
         hipEvent_t *e = &ev[used];
         used += 2;
@@ -135,6 +138,14 @@ struct Prof {
     }
 };
 
+// Scalars a reduction kernel writes straight into coherent host memory
+// (rr = r.r, bb = b.b, the setup flags), then seq with a system-scope
+// release, so the host can spin on seq instead of a stream synchronisation.
+struct Mailbox {
+    double rr, bb, flags, pad;
+    unsigned long long seq;
+};
+
 }  // namespace pps
 
 struct pps_handle {
@@ -143,6 +154,11 @@ struct pps_handle {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // second stream (default priority, non-blocking) on which the fast
+    // solve builds its coarse hierarchy while the first fine sweeps run,
+    // and two ordering events (no timing) between it and the solve stream
+    hipStream_t side = nullptr;
+    hipEvent_t sev[2] = {nullptr, nullptr};
     pps::RefState *ref = nullptr;
     pps::FastState *fast = nullptr;
     std::vector<double> hist;
@@ -159,6 +175,8 @@ struct pps_handle {
     double wmin = 1.0;
     pps::DevBuf red;               // reduction partials + results
     double *pinned = nullptr;      // 16 pinned host doubles for scalar readback
+    pps::Mailbox *mbox = nullptr;  // coherent host memory (pps_init)
+    unsigned long long mseq = 0;   // seq of the last mailbox write requested
     pps::Prof prof;                // fine-level smoother timing
 };
 
