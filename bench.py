@@ -35,13 +35,26 @@ HBM_PEAK_GBPS = 8000.0
 METRIC = "ms per pressure solve to 1e-10 rel. residual + HBM GB/s; mixed vs FP64 GPU MG"
 
 
-def level_dims(n3):
-    """Level sizes of the fast hierarchy (mirrors the library's plan)."""
-    nx, ny, nz = n3
-    dims = [(nx, ny, nz)]
-    while nx * ny * nz > 64 and nx % 2 == 0 and ny % 2 == 0 and nz % 2 == 0:
-        nx, ny, nz = nx // 2, ny // 2, nz // 2
-        dims.append((nx, ny, nz))
+def level_dims(n3, periodic=(0, 0, 0), lengths=(1.0, 1.0, 1.0)):
+    """Level sizes of the fast hierarchy (the library's plan: every axis is
+    halved, ceil(n / 2), unless its cells are more than 1.5 times as long as
+    the shortest, it is periodic with an odd size, or it has one cell; a
+    level with an odd periodic axis is the coarsest; stop at 64 cells)."""
+    n = list(n3)
+    h = [lengths[a] / n[a] for a in range(3)]
+    dims = [tuple(n)]
+    while n[0] * n[1] * n[2] > 64:
+        if any(periodic[a] and n[a] % 2 for a in range(3)):
+            break
+        hmin = min(h)
+        f = [2 if (n[a] > 1 and h[a] <= 1.5 * hmin) else 1 for a in range(3)]
+        if max(f) == 1:
+            break
+        for a in range(3):
+            if f[a] == 2:
+                n[a] = (n[a] + 1) // 2
+                h[a] *= 2.0
+        dims.append(tuple(n))
     return dims
 
 

@@ -19,46 +19,73 @@ struct FaceK {
     int px, py, pz;
 };
 
-// Threads own (i, j) columns of 64 x 4 cells and march through zc planes;
-// p and lam of the +z plane are loaded once and carried to the next step,
-// so each plane of p and lam is fetched from HBM once (the +x / +y
-// neighbours are the same wavefront's or the next row's lines, served by
-// the caches).  Algorithmic bytes: p, lam read (16 B), three gradient
-// planes written (24 B) per cell.
-__global__ void __launch_bounds__(256)
+// Workgroups of 64 x 16 threads own (i, j) columns and march through zc
+// planes.  Each plane of p and lam is fetched once: the +z values are
+// loaded one plane ahead and carried, the +x neighbour comes from the next
+// lane of the wavefront (a shuffle; the last lane of a row segment and the
+// last cell of a row load it), the +y neighbour from the next thread row
+// through LDS (the last thread row and the last row of the grid load it).
+// Threads past the grid work on a clamped cell and store nothing.
+// Algorithmic bytes: p, lam read (16 B), three gradient planes written
+// (24 B) per cell.
+constexpr int kFgRows = 16;
+__global__ void __launch_bounds__(64 * kFgRows)
 k_face_grad(FaceK d, int zc, const double *__restrict__ p, const double *__restrict__ lam,
             const double *__restrict__ wx, const double *__restrict__ wy,
             const double *__restrict__ wz, double *__restrict__ g) {
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * 4 + threadIdx.y;
+    __shared__ double sp[2][kFgRows][64], sl[2][kFgRows][64];
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i = blockIdx.x * 64 + tx, j = blockIdx.y * kFgRows + ty;
     const int k0 = blockIdx.z * zc;
     const int k1 = min(k0 + zc, d.nz);
-    if (i >= d.nx || j >= d.ny) return;
+    const bool in = i < d.nx && j < d.ny;
+    const int ic = min(i, d.nx - 1), jc = min(j, d.ny - 1);
     const long long n = (long long)d.nx * d.ny * d.nz;
     const long long Py = d.X, Pz = (long long)d.X * d.Y;
-    const long long col = (i + d.ng) + Py * (j + d.ng);
+    const long long col = (ic + d.ng) + Py * (jc + d.ng);
     // +x / +y neighbour offsets (wrapped on the last cell of a periodic
     // axis; unused on a wall face)
-    const bool ox = i < d.nx - 1 || d.px, oy = j < d.ny - 1 || d.py;
-    const long long ex = i < d.nx - 1 ? 1 : -(long long)(d.nx - 1);
-    const long long ey = j < d.ny - 1 ? Py : -(long long)(d.ny - 1) * Py;
-    const double gxw = ox ? -wx[i] : 0.0, gyw = oy ? -wy[j] : 0.0;
-    const long long cc = i + (long long)d.nx * j, sz = (long long)d.nx * d.ny;
+    const bool ox = ic < d.nx - 1 || d.px, oy = jc < d.ny - 1 || d.py;
+    const long long ex = ic < d.nx - 1 ? 1 : -(long long)(d.nx - 1);
+    const long long ey = jc < d.ny - 1 ? Py : -(long long)(d.ny - 1) * Py;
+    const double gxw = ox ? -wx[ic] : 0.0, gyw = oy ? -wy[jc] : 0.0;
+    // neighbours this thread loads itself instead of taking them from
+    // another thread
+    const bool xl = tx == 63 || ic == d.nx - 1;
+    const bool yl = ty == kFgRows - 1 || jc == d.ny - 1;
+    const long long cc = ic + (long long)d.nx * jc, sz = (long long)d.nx * d.ny;
     long long q = col + Pz * (k0 + d.ng);
     double pc = p[q], lc = lam[q];
     for (int k = k0; k < k1; ++k) {
+        const int buf = k & 1;
+        sp[buf][ty][tx] = pc;
+        sl[buf][ty][tx] = lc;
         // +z neighbour: the next plane, or the first one (periodic z)
         const bool oz = k < d.nz - 1 || d.pz;
         const long long qz = k < d.nz - 1 ? q + Pz : q - (long long)(d.nz - 1) * Pz;
-        const double pz = p[qz], lz = lam[qz];
-        const double pxe = p[q + ex], lxe = lam[q + ex];
-        const double pye = p[q + ey], lye = lam[q + ey];
-        const long long c = cc + sz * k;
-        g[c] = gxw * (lc - lxe) * (pc - pxe);
-        g[n + c] = gyw * (lc - lye) * (pc - pye);
-        g[2 * n + c] = oz ? -wz[k] * (lc - lz) * (pc - pz) : 0.0;
-        pc = pz;
-        lc = lz;
+        const double pzn = p[qz], lzn = lam[qz];
+        double pxe = __shfl_down(pc, 1, 64), lxe = __shfl_down(lc, 1, 64);
+        if (xl) {
+            pxe = p[q + ex];
+            lxe = lam[q + ex];
+        }
+        __syncthreads();
+        double pye, lye;
+        if (yl) {
+            pye = p[q + ey];
+            lye = lam[q + ey];
+        } else {
+            pye = sp[buf][ty + 1][tx];
+            lye = sl[buf][ty + 1][tx];
+        }
+        if (in) {
+            const long long c = cc + sz * k;
+            g[c] = gxw * (lc - lxe) * (pc - pxe);
+            g[n + c] = gyw * (lc - lye) * (pc - pye);
+            g[2 * n + c] = oz ? -wz[k] * (lc - lzn) * (pc - pzn) : 0.0;
+        }
+        pc = pzn;
+        lc = lzn;
         q += Pz;
     }
 }
@@ -122,14 +149,14 @@ extern "C" int pps_face_gradient_device(const pps_grid *grid, const double *dp, 
     FaceK d;
     d.nx = D.nx; d.ny = D.ny; d.nz = D.nz; d.ng = D.ng; d.X = D.X(); d.Y = D.Y();
     d.px = D.per[0]; d.py = D.per[1]; d.pz = D.per[2];
-    // about 16 blocks of 256 threads per CU over (x, y) columns x z chunks
-    const int gx = (d.nx + 63) / 64, gy = (d.ny + 3) / 4;
-    long long chunks = 4096 / ((long long)gx * gy);
+    // about 4 workgroups of 1024 threads per CU over (x, y) tiles x z chunks
+    const int gx = (d.nx + 63) / 64, gy = (d.ny + kFgRows - 1) / kFgRows;
+    long long chunks = 1024 / ((long long)gx * gy);
     if (chunks < 1) chunks = 1;
     if (chunks > d.nz) chunks = d.nz;
     const int zc = (int)((d.nz + chunks - 1) / chunks);
     chunks = (d.nz + zc - 1) / zc;
-    k_face_grad<<<dim3((unsigned)gx, (unsigned)gy, (unsigned)chunks), dim3(64, 4, 1), 0, s>>>(
+    k_face_grad<<<dim3((unsigned)gx, (unsigned)gy, (unsigned)chunks), dim3(64, kFgRows, 1), 0, s>>>(
         d, zc, dp, dlam, wx, wy, wz, dgrad);
     PPS_HIP(hipGetLastError());
     PPS_HIP(hipStreamSynchronize(s));

@@ -103,8 +103,18 @@ __global__ void __launch_bounds__(256)
 k_rb(LevelK<T> L, const B *__restrict__ b, T *__restrict__ x, int colour) {
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int k = blockIdx.z;
-    const int i = 2 * (int)(blockIdx.x * blockDim.x + threadIdx.x) + ((j + k + colour) & 1);
-    if (j >= L.ny || i >= L.nx) return;
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int i = 2 * t + ((j + k + colour) & 1);
+    if (j >= L.ny) return;
+    if (FIRST) {
+        // the other cell of the pair starts at zero: on a wall axis of odd
+        // size the wrapped neighbour across the wall (coupling 0) has the
+        // cell's own colour, and the next pass reads it (times 0) before
+        // writing it, so it must hold a finite value
+        const int io = 2 * t + ((j + k + colour + 1) & 1);
+        if (io < L.nx) x[io + (long long)L.nx * (j + (long long)L.ny * k)] = (T)0;
+    }
+    if (i >= L.nx) return;
     const long long c = i + (long long)L.nx * (j + (long long)L.ny * k);
     const T d = diag_of<T, DGI>(L, i, j, k, c);
     if (FIRST) {
@@ -685,6 +695,51 @@ k_prolong_add(int fx, int fy, int fz, T *__restrict__ xf, const T *__restrict__ 
 }
 
 // ---------------------------------------------------------------------
+// Transfers of an irregular level: a factor of 1 or 2 per axis (semi-
+// coarsening), and with factor 2 on an odd size the last coarse cell has
+// a single child.  R = P^T / (fx fy fz) with the level's nominal factors
+// for every coarse cell, so the V-cycle stays symmetric.
+// ---------------------------------------------------------------------
+struct Coarsening {
+    int fx, fy, fz;      // 1 or 2 per axis
+    int cnx, cny, cnz;   // coarse sizes, ceil(n / f)
+};
+
+// coarse b = (1 / (fx fy fz)) * the sum of the children's residuals; one
+// thread per coarse cell
+template <class T, class B, bool DGI>
+__global__ void __launch_bounds__(256)
+k_resid_restrict_g(LevelK<T> L, Coarsening g, const B *__restrict__ b, const T *__restrict__ x,
+                   T *__restrict__ bc) {
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int J = blockIdx.y * blockDim.y + threadIdx.y;
+    const int K = blockIdx.z;
+    if (I >= g.cnx || J >= g.cny || K >= g.cnz) return;
+    const int i0 = g.fx * I, j0 = g.fy * J, k0 = g.fz * K;
+    const int i1 = min(i0 + g.fx, L.nx), j1 = min(j0 + g.fy, L.ny), k1 = min(k0 + g.fz, L.nz);
+    T acc = 0;
+    for (int k = k0; k < k1; ++k)
+        for (int j = j0; j < j1; ++j)
+            for (int i = i0; i < i1; ++i) {
+                const long long c = i + (long long)L.nx * (j + (long long)L.ny * k);
+                acc += ((T)b[c] + nb_sum(L, x, i, j, k, c)) - diag_of<T, DGI>(L, i, j, k, c) * x[c];
+            }
+    bc[I + (long long)g.cnx * (J + (long long)g.cny * K)] = acc / (T)(g.fx * g.fy * g.fz);
+}
+
+// x_f += x_c of the parent cell; one thread per fine cell
+template <class T>
+__global__ void __launch_bounds__(256)
+k_prolong_add_g(int nx, int ny, int nz, Coarsening g, T *__restrict__ xf, const T *__restrict__ xc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= nx || j >= ny || k >= nz) return;
+    const long long p = i / g.fx + (long long)g.cnx * (j / g.fy + (long long)g.cny * (k / g.fz));
+    xf[i + (long long)nx * (j + (long long)ny * k)] += xc[p];
+}
+
+// ---------------------------------------------------------------------
 // Trilinear transfers (PPS_TRANSFER_TRILINEAR).  Along one axis, fine cell
 // i lies a quarter of a coarse cell from the centre of its parent I = i/2,
 // towards the coarse neighbour O = I - 1 (i even) or I + 1 (i odd):
@@ -937,6 +992,61 @@ k_coarsen(LevelK<Tf> F, Tc *__restrict__ ccx, Tc *__restrict__ ccy, Tc *__restri
                 ex += excess_of(F, i, j, k, fi(i, j, k));
             }
     cdg[c] = (Tc)(((vx + wx) + (vy + wy)) + (vz + wz) + 0.125 * ex);
+}
+
+// Coarse operator of an irregular level (the rule of k_coarsen with
+// per-axis factors): the coupling across a coarse x face is the sum of the
+// fine x faces on it (the +x couplings of the coarse cell's last children
+// in x) times 1 / (fy fz fx^2), i.e. their mean over the face area divided
+// by the square of the spacing factor (likewise y, z); the diagonal is the
+// coupling sum plus 1 / (fx fy fz) times the children's excess.
+template <class Tf, class Tc, bool DGI>
+__global__ void __launch_bounds__(256)
+k_coarsen_g(LevelK<Tf> F, Coarsening g, Tc *__restrict__ ccx, Tc *__restrict__ ccy, Tc *__restrict__ ccz,
+            Tc *__restrict__ cdg) {
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int J = blockIdx.y * blockDim.y + threadIdx.y;
+    const int K = blockIdx.z;
+    if (I >= g.cnx || J >= g.cny || K >= g.cnz) return;
+    const long long fsy = F.nx, fsz = (long long)F.nx * F.ny;
+    auto fi = [&](int i, int j, int k) { return i + fsy * j + fsz * k; };
+    auto lo = [](int C, int f) { return f * C; };
+    auto hi = [](int C, int f, int n) { return min(f * C + f, n); };
+    auto cpx = [&](int I_, int J_, int K_) {
+        const int i = hi(I_, g.fx, F.nx) - 1;
+        double a = 0.0;
+        for (int k = lo(K_, g.fz); k < hi(K_, g.fz, F.nz); ++k)
+            for (int j = lo(J_, g.fy); j < hi(J_, g.fy, F.ny); ++j) a += (double)F.cx[fi(i, j, k)];
+        return a;
+    };
+    auto cpy = [&](int I_, int J_, int K_) {
+        const int j = hi(J_, g.fy, F.ny) - 1;
+        double a = 0.0;
+        for (int k = lo(K_, g.fz); k < hi(K_, g.fz, F.nz); ++k)
+            for (int i = lo(I_, g.fx); i < hi(I_, g.fx, F.nx); ++i) a += (double)F.cy[fi(i, j, k)];
+        return a;
+    };
+    auto cpz = [&](int I_, int J_, int K_) {
+        const int k = hi(K_, g.fz, F.nz) - 1;
+        double a = 0.0;
+        for (int j = lo(J_, g.fy); j < hi(J_, g.fy, F.ny); ++j)
+            for (int i = lo(I_, g.fx); i < hi(I_, g.fx, F.nx); ++i) a += (double)F.cz[fi(i, j, k)];
+        return a;
+    };
+    const double sx = 1.0 / (double)(g.fy * g.fz * g.fx * g.fx);
+    const double sy = 1.0 / (double)(g.fx * g.fz * g.fy * g.fy);
+    const double sz = 1.0 / (double)(g.fx * g.fy * g.fz * g.fz);
+    const double vx = sx * cpx(I, J, K), vy = sy * cpy(I, J, K), vz = sz * cpz(I, J, K);
+    const long long c = I + (long long)g.cnx * (J + (long long)g.cny * K);
+    ccx[c] = (Tc)vx; ccy[c] = (Tc)vy; ccz[c] = (Tc)vz;
+    if (DGI) return;
+    const double wx = sx * cpx(wrap_m(I, g.cnx), J, K), wy = sy * cpy(I, wrap_m(J, g.cny), K),
+                 wz = sz * cpz(I, J, wrap_m(K, g.cnz));
+    double ex = 0.0;
+    for (int k = lo(K, g.fz); k < hi(K, g.fz, F.nz); ++k)
+        for (int j = lo(J, g.fy); j < hi(J, g.fy, F.ny); ++j)
+            for (int i = lo(I, g.fx); i < hi(I, g.fx, F.nx); ++i) ex += excess_of(F, i, j, k, fi(i, j, k));
+    cdg[c] = (Tc)(((vx + wx) + (vy + wy)) + (vz + wz) + ex / (double)(g.fx * g.fy * g.fz));
 }
 
 template <class Ti, class To>
@@ -2005,6 +2115,11 @@ template <class T> struct Level {
     long long n;
     T *cx, *cy, *cz, *dg, *b;
     T *x, *xt;   // current iterate and the other buffer of the out-of-place sweeps
+    // coarsening to the next level: factor 1 or 2 per axis, and whether it
+    // is the regular one (every factor 2, every size even: the fused
+    // restriction, prolongation and tail kernels apply)
+    Coarsening g{2, 2, 2, 1, 1, 1};
+    bool reg = true;
     LevelK<T> k() const { return LevelK<T>{nx, ny, nz, cx, cy, cz, dg}; }
 };
 
@@ -2073,19 +2188,44 @@ static int nblk(long long n) {
     return (int)b;
 }
 
-// level sizes: coarsen while every dim is even and the level has more
-// than 64 cells; the coarsest must have at most 512 cells
-static int plan_levels(const Dims &d, std::vector<int> &dims) {
-    int nx = d.nx, ny = d.ny, nz = d.nz;
+// Level sizes of the fast hierarchy.  From each level every axis is halved
+// (ceil(n / 2) coarse cells; on an odd size the last coarse cell has a
+// single child) except
+//   - an axis whose cells are more than 1.5 times as long as along the
+//     shortest one (semi-coarsening: the strongly coupled directions are
+//     coarsened first, which makes the coarse cells more nearly cubic);
+//   - a periodic axis of odd size (an odd periodic ring cannot be coloured
+//     red-black), and an axis of one cell.
+// A level with a periodic axis of odd size cannot be smoothed, so it is
+// the coarsest.  Coarsening stops at 64 cells or fewer; the coarsest level
+// must have at most 512 cells (its exact dense solve).  lengths: the box
+// (cell sizes length / n; a non-positive length counts as n).
+static int plan_levels(const Dims &d, const double lengths[3], std::vector<int> &dims) {
+    int n[3] = {d.nx, d.ny, d.nz};
     // z-marching reductions keep one partial slot per (64 x 4) column block
-    if ((long long)((nx + 63) / 64) * ((ny + 3) / 4) > kRedBlocks) return PPS_ERR_BAD_SIZE;
+    if ((long long)((n[0] + 63) / 64) * ((n[1] + 3) / 4) > kRedBlocks) return PPS_ERR_BAD_SIZE;
+    double h[3];
+    for (int a = 0; a < 3; ++a) h[a] = lengths[a] > 0.0 ? lengths[a] / n[a] : 1.0;
     dims.clear();
-    dims.push_back(nx); dims.push_back(ny); dims.push_back(nz);
-    while ((long long)nx * ny * nz > 64 && !(nx & 1) && !(ny & 1) && !(nz & 1)) {
-        nx >>= 1; ny >>= 1; nz >>= 1;
-        dims.push_back(nx); dims.push_back(ny); dims.push_back(nz);
+    for (int a = 0; a < 3; ++a) dims.push_back(n[a]);
+    while ((long long)n[0] * n[1] * n[2] > 64) {
+        bool podd = false;
+        for (int a = 0; a < 3; ++a) podd |= d.per[a] && (n[a] & 1);
+        if (podd) break;
+        const double hmin = std::fmin(h[0], std::fmin(h[1], h[2]));
+        bool any = false;
+        int f[3];
+        for (int a = 0; a < 3; ++a) {
+            f[a] = (n[a] > 1 && h[a] <= 1.5 * hmin) ? 2 : 1;
+            any |= f[a] == 2;
+        }
+        if (!any) break;
+        for (int a = 0; a < 3; ++a) {
+            if (f[a] == 2) { n[a] = (n[a] + 1) / 2; h[a] *= 2.0; }
+            dims.push_back(n[a]);
+        }
     }
-    if ((long long)nx * ny * nz > 512) return PPS_ERR_BAD_SIZE;
+    if ((long long)n[0] * n[1] * n[2] > 512) return PPS_ERR_BAD_SIZE;
     return PPS_OK;
 }
 
@@ -2098,6 +2238,11 @@ static int hier_alloc(Hier<T> &H, const std::vector<int> &dims, bool own_fine, A
         Level<T> &L = H.lv[l];
         L.nx = dims[3 * l]; L.ny = dims[3 * l + 1]; L.nz = dims[3 * l + 2];
         L.n = (long long)L.nx * L.ny * L.nz;
+        if (l + 1 < nl) {
+            const int cx = dims[3 * l + 3], cy = dims[3 * l + 4], cz = dims[3 * l + 5];
+            L.g = Coarsening{cx == L.nx ? 1 : 2, cy == L.ny ? 1 : 2, cz == L.nz ? 1 : 2, cx, cy, cz};
+            L.reg = L.g.fx == 2 && L.g.fy == 2 && L.g.fz == 2 && !(L.nx & 1) && !(L.ny & 1) && !(L.nz & 1);
+        }
         const bool own = l > 0 || own_fine;
         // coefficient planes (4) + b + x + xt, or only xt when the fine
         // operator, right-hand side and iterate are borrowed from the outer
@@ -2126,8 +2271,14 @@ static void hier_build(Hier<T> &H, int singular, hipStream_t s) {
     for (int l = 1; l < nl; ++l) {
         const Level<T> &F = H.lv[l - 1];
         Level<T> &C = H.lv[l];
-        if (singular) k_coarsen<T, T, true><<<cgrid3(C.nx, C.ny, C.nz), kB3, 0, s>>>(F.k(), C.cx, C.cy, C.cz, C.dg);
-        else k_coarsen<T, T, false><<<cgrid3(C.nx, C.ny, C.nz), kB3, 0, s>>>(F.k(), C.cx, C.cy, C.cz, C.dg);
+        if (!F.reg) {
+            if (singular) k_coarsen_g<T, T, true><<<cgrid3(C.nx, C.ny, C.nz), kB3, 0, s>>>(F.k(), F.g, C.cx, C.cy, C.cz, C.dg);
+            else k_coarsen_g<T, T, false><<<cgrid3(C.nx, C.ny, C.nz), kB3, 0, s>>>(F.k(), F.g, C.cx, C.cy, C.cz, C.dg);
+        } else if (singular) {
+            k_coarsen<T, T, true><<<cgrid3(C.nx, C.ny, C.nz), kB3, 0, s>>>(F.k(), C.cx, C.cy, C.cz, C.dg);
+        } else {
+            k_coarsen<T, T, false><<<cgrid3(C.nx, C.ny, C.nz), kB3, 0, s>>>(F.k(), C.cx, C.cy, C.cz, C.dg);
+        }
     }
     const size_t lds = sizeof(double) * ((size_t)H.nb * H.nb + 2 * (size_t)H.nb);
     if (lds <= 65536)
@@ -2457,9 +2608,15 @@ static int vcycle(Hier<T> &H, const T *fineb, const Cycle &cy, Prof *prof, hipSt
         for (int l = 1; l < nl; ++l) {
             if (H.lv[l].n <= cy.tail_cells) { lt = l; break; }
         }
-        // the tail must fit the workgroup's LDS
-        while (lt < nl &&
-               (nl - lt > kTailMax || tail_layout(H, lt, dgi, cy.tri, (TailK<T> *)nullptr) > cy.lds_max))
+        // the tail must fit the workgroup's LDS, and its transfers are the
+        // regular ones
+        auto tail_regular = [&](int from) {
+            for (int l = from; l + 1 < nl; ++l)
+                if (!H.lv[l].reg) return false;
+            return true;
+        };
+        while (lt < nl && (nl - lt > kTailMax || !tail_regular(lt) ||
+                           tail_layout(H, lt, dgi, cy.tri, (TailK<T> *)nullptr) > cy.lds_max))
             ++lt;
     }
     for (int l = 0; l < (lt < nl ? lt : nl - 1); ++l) {
@@ -2483,6 +2640,9 @@ static int vcycle(Hier<T> &H, const T *fineb, const Cycle &cy, Prof *prof, hipSt
             if (dgi) k_resid_field<T, T, true><<<cgrid3(L.nx, L.ny, L.nz), kB3, 0, s>>>(L.k(), b, L.x, L.xt);
             else k_resid_field<T, T, false><<<cgrid3(L.nx, L.ny, L.nz), kB3, 0, s>>>(L.k(), b, L.x, L.xt);
             k_restrict_tri<T><<<cgrid3(C.nx, C.ny, C.nz), kB3, 0, s>>>(t, L.xt, C.b);
+        } else if (!L.reg) {
+            if (dgi) k_resid_restrict_g<T, T, true><<<cgrid3(C.nx, C.ny, C.nz), kB3, 0, s>>>(L.k(), L.g, b, L.x, C.b);
+            else k_resid_restrict_g<T, T, false><<<cgrid3(C.nx, C.ny, C.nz), kB3, 0, s>>>(L.k(), L.g, b, L.x, C.b);
         } else if (L.nx >= 32 && L.ny >= 16) {
             const TGrid rg = tgrid(L.nx, L.ny, L.nz, 1024, true);
             if (dgi) k_resid_restrict_tiled<T, true><<<rg.blocks, dim3(64, 16, 1), 0, s>>>(L.k(), rg, b, L.x, C.b);
@@ -2530,10 +2690,12 @@ static int vcycle(Hier<T> &H, const T *fineb, const Cycle &cy, Prof *prof, hipSt
         const T *b = l == 0 ? fineb : L.b;
         // the first fused post-sweep adds a piecewise-constant correction
         // as it loads x; the trilinear one is a pass of its own
-        const bool fold = nu_post > 0 && sweep_fused(L, cy) && !cy.tri;
+        const bool fold = nu_post > 0 && sweep_fused(L, cy) && !cy.tri && L.reg;
         if (cy.tri) {
             const TriK t{L.nx, L.ny, L.nz, cy.per[0], cy.per[1], cy.per[2]};
             k_prolong_tri_add<T><<<cgrid3(L.nx, L.ny, L.nz), kB3, 0, s>>>(t, L.x, C.x);
+        } else if (!L.reg) {
+            k_prolong_add_g<T><<<cgrid3(L.nx, L.ny, L.nz), kB3, 0, s>>>(L.nx, L.ny, L.nz, L.g, L.x, C.x);
         } else if (!fold) {
             k_prolong_add<T><<<cgrid3(C.nx, C.ny, C.nz), kB3, 0, s>>>(L.nx, L.ny, L.nz, L.x, C.x);
         }
@@ -2652,7 +2814,14 @@ static int fast_setup(pps_handle *h, const double *dA, const double *dp, bool mi
     Vecs V = vecs(f);
     int *flags = f->flags.as<int>();
     std::vector<int> dims;
-    PPS_TRY(plan_levels(d, dims));
+    const double lengths[3] = {h->grid.xlength, h->grid.ylength, h->grid.zlength};
+    PPS_TRY(plan_levels(d, lengths, dims));
+    // the trilinear transfers need every level halved in every direction
+    if (h->opt.transfer == PPS_TRANSFER_TRILINEAR) {
+        for (size_t l = 0; l + 3 < dims.size(); l += 3)
+            for (int a = 0; a < 3; ++a)
+                if ((dims[l + a] & 1) || dims[l + 3 + a] * 2 != dims[l + a]) return PPS_ERR_UNSUPPORTED;
+    }
     float *f32 = nullptr, *r32 = nullptr;
     if (mixed) {
         if (!f->mixed_built) { PPS_TRY(hier_alloc(f->h32, dims, true, f->arena)); f->mixed_built = 1; }
@@ -3053,9 +3222,9 @@ int fast_scratch_setup(int device) {
 
 // Bytes the no-allocation solve carves for a grid, in the order and with
 // the alignment fast_ensure / fast_setup / hier_alloc take them.
-static size_t noalloc_bytes(const Dims &d, bool mixed) {
+static size_t noalloc_bytes(const Dims &d, const double lengths[3], bool mixed) {
     std::vector<int> dims;
-    if (plan_levels(d, dims) != PPS_OK) return 0;
+    if (plan_levels(d, lengths, dims) != PPS_OK) return 0;
     const size_t n = (size_t)d.cells();
     size_t tot = 0;
     auto take = [&](size_t b) { tot = ((tot + 255) & ~(size_t)255) + b; };
@@ -3101,7 +3270,8 @@ extern "C" int pps_solve_device_noalloc(pps_handle *h, const double *dA, double 
     if (!base) return PPS_ERR_HIP;   // pps_init looks the address up
     const Dims d = dims_of(h->grid);
     const bool mixed = method == PPS_MG_MIXED;
-    const size_t need = noalloc_bytes(d, mixed);
+    const double lengths[3] = {h->grid.xlength, h->grid.ylength, h->grid.zlength};
+    const size_t need = noalloc_bytes(d, lengths, mixed);
     if (need == 0) return PPS_ERR_BAD_SIZE;
     if (need > kScratchBytes) return PPS_ERR_SCRATCH;
     PPS_HIP(hipSetDevice(h->device));

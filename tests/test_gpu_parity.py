@@ -467,11 +467,12 @@ def test_fast_non_power_of_two(pps, orc, method, shape, bc):
 
 
 def test_fast_refuses_sizes_without_small_coarsest(pps):
-    """50^3 halves once to 25^3 (15,625 cells), too large for the exact
-    coarsest solve: the fast methods report a bad size (the reference and
-    SOR methods still apply)."""
+    """A fully periodic 50^3 box halves once to 25^3; an odd periodic axis
+    is not halved further, so the coarsest level has 15,625 cells, too many
+    for the exact coarsest solve: the fast methods report a bad size (the
+    reference and SOR methods still apply)."""
     _say("fast bad size")
-    g = pps.Grid.make(50)
+    g = pps.Grid.make(50, bdry_cond=(1,) * 6)
     A = torch.zeros((8,) + (50, 50, 50), dtype=torch.float64, device="cuda")
     A[0:6] = 1.0
     A[6] = 6.0
