@@ -234,8 +234,10 @@ def project_run(pps, torch, args):
     """Mixed solve followed by project_velocity: the fused entry point (the
     correction inside the last residual pass) against the solve alone and
     against the solve plus the standalone projection call (which also
-    uploads the grid spacing on every call).  CUDA events on the current
-    stream; medians over the timed steps."""
+    uploads the grid spacing on every call).  Also one time step's system
+    from the fields: SetupPoisson on the device then the solve, against the
+    solve that assembles its rows inside its setup pass (A never formed).
+    CUDA events on the current stream; medians over the timed steps."""
     import numpy as np
     print(f"[bench] building {args.workload}", file=sys.stderr, flush=True)
     grid, A, f = pps.make_problem(args.workload)
@@ -243,7 +245,7 @@ def project_run(pps, torch, args):
     tol = 1e-10 * brms
     dt = 1.0
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-    res = {"solve": [], "solve_then_project": [], "fused": []}
+    res = {"solve": [], "solve_then_project": [], "fused": [], "assemble_then_solve": [], "fields": []}
     with pps.Solver(grid, method="mixed") as s:
         p = torch.zeros(grid.shape_p, dtype=torch.float64, device="cuda")
         for step in range(args.warmup + args.steps):
@@ -255,6 +257,12 @@ def project_run(pps, torch, args):
                 if mode == "fused":
                     s.solve_project(A, p, f["u"], f["v"], f["w"], f["um"], f["vm"], f["wm"], f["rho"],
                                     dt, tol, 1.2, 100)
+                elif mode == "assemble_then_solve":
+                    A2 = pps.setup_poisson(grid, f["u"], f["v"], f["w"], f["um"], f["vm"], f["wm"], f["rho"], dt)
+                    s.solve(A2, p, tol, 1.2, 100)
+                    del A2
+                elif mode == "fields":
+                    s.solve_fields(f, p, tol, dt=dt, maxit=100)
                 else:
                     s.solve(A, p, tol, 1.2, 100)
                     if mode == "solve_then_project":

@@ -1600,8 +1600,10 @@ k_dot_rz(long long n, const double *__restrict__ r, const TZ *__restrict__ z, do
 // operator of plane s are requested at the start of step s, so the loads
 // overlap the arithmetic of the step; one barrier per plane (three
 // buffers: the plane written at step s+1 was last read at step s-2).
+// Registers held to 64 a thread (two blocks a CU): the FP64 form needs 66
+// otherwise, one block a CU, and ran 239 against 223 us at 256^3.
 template <class TZ, bool DGI>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8)))
 k_dq_tiled(OpK L, TGrid g, const TZ *__restrict__ z, const TZ *__restrict__ dold,
            TZ *__restrict__ dnew, double *__restrict__ q, double *__restrict__ x,
            const double *__restrict__ S, double *__restrict__ part) {
@@ -1729,7 +1731,7 @@ __global__ void __launch_bounds__(256)
 k_axpy_d(long long n, double *__restrict__ x, const TD *__restrict__ d, const double *__restrict__ S) {
     const double alpha = S[S_ALPHA];
     for (long long c = blockIdx.x * 256LL + threadIdx.x; c < n; c += (long long)gridDim.x * 256)
-        x[c] += alpha * (double)d[c];
+        x[c] = __fma_rn(alpha, (double)d[c], x[c]);
 }
 
 // refinement step: x += z - zmean ; partial sums of z
@@ -1888,26 +1890,36 @@ struct RowsF {
     RowK R;
     int nx, ny, nz, ng;
     long long Py, Pz;
+    double raw[3];   // the row's own +x, +y, +z couplings before Poisson_BCs
     __device__ void begin(long long) {}
     __device__ int row(const CellAt &e, double a[8]) {
-        return setup_row(R, nx, ny, nz, ng, Py, Pz, e.i, e.j, e.k, e.q, a);
+        return setup_row<true>(R, nx, ny, nz, ng, Py, Pz, e.i, e.j, e.k, e.q, a, raw);
     }
     // For boundary codes 0-4 a neighbour's lower coupling is SetupPoisson's
     // expression: Poisson_BCs changes only couplings across the domain
     // boundary (codes 3/4), which this check never asks for, and the solid
-    // rule changes only the diagonal.
+    // rule changes only the diagonal.  Inside the domain that expression
+    // differs from the row's own coupling of the same face only in the width
+    // of the cell (face_coupling(dt, mask, d, dh, r_nb, r_c) with d the
+    // neighbour's width and r_nb + r_c added the other way round, which
+    // rounds the same): with equal widths it is that coupling, bit for bit,
+    // and is not evaluated again.  Across a periodic boundary the neighbour's
+    // coupling reads the ghost mask and densities, so it is always formed.
     __device__ double nb_x(const CellAt &e, long long) const {
         const int I = (e.lastx ? 0 : e.i + 1) + ng;
+        if (!e.lastx && R.ax[I] == R.ax[e.i + ng]) return raw[0];
         const long long q = e.lastx ? e.q - (nx - 1) : e.q + 1;
         return face_coupling(R.dt, R.um[q - 1], R.ax[I], R.ax[nx + 2 * ng + I - 1], R.rho[q - 1], R.rho[q]);
     }
     __device__ double nb_y(const CellAt &e, long long) const {
         const int J = (e.lasty ? 0 : e.j + 1) + ng;
+        if (!e.lasty && R.ay[J] == R.ay[e.j + ng]) return raw[1];
         const long long q = e.lasty ? e.q - (long long)(ny - 1) * Py : e.q + Py;
         return face_coupling(R.dt, R.vm[q - Py], R.ay[J], R.ay[ny + 2 * ng + J - 1], R.rho[q - Py], R.rho[q]);
     }
     __device__ double nb_z(const CellAt &e, long long) {
         const int K = (e.lastz ? 0 : e.k + 1) + ng;
+        if (!e.lastz && R.az[K] == R.az[e.k + ng]) return raw[2];
         const long long q = e.lastz ? e.q - (long long)(nz - 1) * Pz : e.q + Pz;
         return face_coupling(R.dt, R.wm[q - Pz], R.az[K], R.az[nz + 2 * ng + K - 1], R.rho[q - Pz], R.rho[q]);
     }
@@ -2027,70 +2039,134 @@ __device__ __forceinline__ double proj_face(double vel, double mask, double dh, 
     return vel - dt * (2.0 * mask / dh) * (pe - pc) / (re + rc);
 }
 
-// Final check fused with the write-back (z-marching): r = b - M x with the
-// FP64 operator, partial sum r^2, and p = x on the interior with the
-// first ghost layer wrapped in periodic directions.  PROJ: also the
-// velocity correction of project_velocity into P.uo / P.vo / P.wo, from the
-// iterate already in registers (the +x / +y neighbours of the stencil, the
+// Final check fused with the write-back, on LDS tiles (64 x 16 cells, zc
+// planes a block; launch with tgrid()): x_f = x + alpha d, the last
+// iteration's deferred update (alpha = S[S_ALPHA]; dd null: x_f = x), then
+// r = b - M x_f with the FP64 operator, partial sum r^2, and p = x_f on the
+// interior with the first ghost layer wrapped in periodic directions.  x is
+// left as it was (a restart applies the update first).  x_f of each plane
+// (own cells plus a one-cell halo ring, coordinates wrapped) goes to one of
+// three LDS planes: the in-plane neighbours come from there, the vertical
+// ones from registers, and the raw x / d of plane s+2 are requested at the
+// start of step s.  PROJ: also the velocity correction of project_velocity
+// into P.uo / P.vo / P.wo from x_f (the +x / +y neighbours from LDS, the
 // +z one of the march; on a periodic axis the wrapped value is the ghost
-// the reference reads); faces left alone are copied.
-template <bool DGI, bool PROJ>
-__global__ void __launch_bounds__(256)
-k_check_unpack(OpK L, PackK d, int zc, const double *__restrict__ b, const double *__restrict__ x,
-               double *__restrict__ p, double *__restrict__ part, ProjK P, const double *__restrict__ wx,
-               const double *__restrict__ wy, const double *__restrict__ wz) {
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * 4 + threadIdx.y;
-    const int k0 = blockIdx.z * zc;
-    const int k1 = min(k0 + zc, L.nz);
+// the reference reads); faces left alone are copied.  Without PROJ the
+// registers are held to 64 a thread, two blocks a CU, so that one block's
+// loads overlap the other's plane step (256^3: 230 -> 214 us, a few bytes
+// of spill); the PROJ form would spill far more and keeps its registers.
+template <bool DGI, bool PROJ, class TD>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(PROJ ? 1 : 8)))
+k_check_tiled(OpK L, PackK d, TGrid g, const double *__restrict__ b, const double *__restrict__ x,
+              const TD *__restrict__ dd, const double *__restrict__ S, double *__restrict__ p,
+              double *__restrict__ part, ProjK P, const double *__restrict__ wx,
+              const double *__restrict__ wy, const double *__restrict__ wz) {
+    __shared__ double X[3][18][66];
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const int bid = xcd_order(blockIdx.x, gridDim.x);
+    const int tx = bid % g.tx, ty = (bid / g.tx) % g.ty, ch = bid / (g.tx * g.ty);
+    const int li = threadIdx.x, lj = threadIdx.y;
+    const int i = tx * 64 + li, j = ty * 16 + lj;
+    const int k0 = ch * g.zc, k1 = min(k0 + g.zc, nz);
+    const bool in = i < nx && j < ny;
+    const double al = dd ? S[S_ALPHA] : 0.0;
+    const bool upd = dd != nullptr;
+    const long long sy = nx, sz = (long long)nx * ny;
+    const long long Py = d.X, Pz = (long long)d.X * d.Y;
+    auto wrapn = [](int a, int n) { a %= n; return a < 0 ? a + n : a; };
+    auto wrapk = [&](int k) { return wrapn(k, nz); };
+    // own slot (li+1, lj+1) holds cell (wrap i, wrap j); threads 0..163 also
+    // fill one halo slot: rows -1 / 16 (66 slots each), columns -1 / 64
+    const long long own = wrapn(i, nx) + sy * wrapn(j, ny);
+    const int t = li + 64 * lj;
+    int hx = 0, hy = 0;
+    if (t < 66) { hx = t - 1; hy = -1; }
+    else if (t < 132) { hx = t - 67; hy = 16; }
+    else if (t < 148) { hx = -1; hy = t - 132; }
+    else { hx = 64; hy = t - 148; }
+    const bool halo = t < 164;
+    const long long hcol = wrapn(tx * 64 + hx, nx) + sy * wrapn(ty * 16 + hy, ny);
+    auto ldd = [&](long long c) -> TD { return upd ? dd[c] : (TD)0; };
+    auto xf = [&](double xv, TD dv) -> double { return upd ? __fma_rn(al, (double)dv, xv) : xv; };
+    const long long dxm = in ? (long long)(wrap_m(i, nx) - i) : 0;
+    const long long dym = in ? (long long)(wrap_m(j, ny) - j) * sy : 0;
+
+    // prologue: x_f at k0-1 (register), k0 (register + LDS); raw of k0+1
+    double x_m = 0.0, x_0 = 0.0, xo1 = 0.0, xh1 = 0.0;
+    TD do1 = 0, dh1 = 0;
+    {
+        const long long pm = sz * wrapk(k0 - 1), p0 = sz * wrapk(k0), p1 = sz * wrapk(k0 + 1);
+        x_m = xf(x[own + pm], ldd(own + pm));
+        x_0 = xf(x[own + p0], ldd(own + p0));
+        X[k0 % 3][lj + 1][li + 1] = x_0;
+        if (halo) X[k0 % 3][hy + 1][hx + 1] = xf(x[hcol + p0], ldd(hcol + p0));
+        xo1 = x[own + p1];
+        do1 = ldd(own + p1);
+        if (halo) {
+            xh1 = x[hcol + p1];
+            dh1 = ldd(hcol + p1);
+        }
+    }
+    double czm = in ? op_z(L, own + sz * wrapk(k0 - 1), wrapk(k0 - 1)) : 0.0;
+    // PROJ: the cell's density, carried up the march
+    double rc = (PROJ && in) ? P.rho[(i + d.ng) + Py * (j + d.ng) + Pz * (k0 + d.ng)] : 0.0;
+    __syncthreads();
     double rr = 0.0;
-    if (i < L.nx && j < L.ny) {
-        const long long sy = L.nx, sz = (long long)L.nx * L.ny;
-        const long long Py = d.X, Pz = (long long)d.X * d.Y;
-        const int dxm = wrap_m(i, L.nx) - i, dxp = wrap_p(i, L.nx) - i;
-        const long long dym = (long long)(wrap_m(j, L.ny) - j) * sy, dyp = (long long)(wrap_p(j, L.ny) - j) * sy;
-        const long long col = i + sy * j;
-        double xm = x[col + sz * wrap_m(k0, L.nz)];
-        double x0 = x[col + sz * k0];
-        // PROJ: the cell's density, carried up the march
-        double rc = PROJ ? P.rho[(i + d.ng) + Py * (j + d.ng) + Pz * (k0 + d.ng)] : 0.0;
-        for (int k = k0; k < k1; ++k) {
-            const long long c = col + sz * k;
-            const long long czm = col + sz * wrap_m(k, L.nz);
-            const double xp = x[col + sz * wrap_p(k, L.nz)];
-            const double vxm = op_x(L, c + dxm, wrap_m(i, L.nx)), vx = op_x(L, c, i);
-            const double vym = op_y(L, c + dym, wrap_m(j, L.ny)), vy = op_y(L, c, j);
-            const double vzm = op_z(L, czm, wrap_m(k, L.nz)), vz = op_z(L, c, k);
-            double s = vxm * x[c + dxm] + vx * x[c + dxp];
-            s += vym * x[c + dym] + vy * x[c + dyp];
-            s += vzm * xm + vz * xp;
-            double v = b[c] - (outer_diag<DGI>(L, c, vxm, vx, vym, vy, vzm, vz) * x0 - s);
-            if (wx) v /= wx[i] * wy[j] * wz[k];   // the caller's (unweighted) row
+    for (int s = k0; s < k1; ++s) {
+        // requests: raw x / d of plane s+2, operator of plane s
+        double xo2 = 0.0, xh2 = 0.0;
+        TD do2 = 0, dh2 = 0;
+        if (s + 1 < k1) {
+            const long long p2 = sz * wrapk(s + 2);
+            xo2 = x[own + p2];
+            do2 = ldd(own + p2);
+            if (halo) {
+                xh2 = x[hcol + p2];
+                dh2 = ldd(hcol + p2);
+            }
+        }
+        const long long c = own + sz * s;
+        double cxw = 0.0, cxe = 0.0, cys = 0.0, cyn = 0.0, czp = 0.0, dgc = 0.0, bv = 0.0;
+        if (in) {
+            cxw = op_x(L, c + dxm, wrap_m(i, nx)); cxe = op_x(L, c, i);
+            cys = op_y(L, c + dym, wrap_m(j, ny)); cyn = op_y(L, c, j);
+            czp = op_z(L, c, s);
+            dgc = outer_diag<DGI>(L, c, cxw, cxe, cys, cyn, czm, czp);
+            bv = b[c];
+        }
+        // x_f of plane s+1 into LDS
+        const int wb = (s + 1) % 3, rb = s % 3;
+        const double x_p = xf(xo1, do1);
+        X[wb][lj + 1][li + 1] = x_p;
+        if (halo) X[wb][hy + 1][hx + 1] = xf(xh1, dh1);
+        if (in) {
+            const double xe = X[rb][lj + 1][li + 2], xn = X[rb][lj + 2][li + 1];
+            double sm = cxw * X[rb][lj + 1][li] + cxe * xe;
+            sm += cys * X[rb][lj][li + 1] + cyn * xn;
+            sm += czm * x_m + czp * x_p;
+            double v = bv - (dgc * x_0 - sm);
+            if (wx) v /= wx[i] * wy[j] * wz[s];   // the caller's (unweighted) row
             rr += v * v;
-            const long long q = (i + d.ng) + Py * (j + d.ng) + Pz * (k + d.ng);
+            const long long q = (i + d.ng) + Py * (j + d.ng) + Pz * (s + d.ng);
             if constexpr (PROJ) {
                 const double re = P.rho[q + 1], rn = P.rho[q + Py], rt = P.rho[q + Pz];
                 const double ui = P.u[q], vi = P.v[q], wi = P.w[q];
-                P.uo[q] = i < L.nx - P.lx
-                              ? proj_face(ui, P.um[q], P.dxh[i + d.ng], x[c + dxp], x0, re, rc, P.dt)
-                              : ui;
-                P.vo[q] = j < L.ny - P.ly
-                              ? proj_face(vi, P.vm[q], P.dyh[j + d.ng], x[c + dyp], x0, rn, rc, P.dt)
-                              : vi;
-                P.wo[q] = k < L.nz - P.lz ? proj_face(wi, P.wm[q], P.dzh[k + d.ng], xp, x0, rt, rc, P.dt)
-                                          : wi;
+                P.uo[q] = i < nx - P.lx ? proj_face(ui, P.um[q], P.dxh[i + d.ng], xe, x_0, re, rc, P.dt) : ui;
+                P.vo[q] = j < ny - P.ly ? proj_face(vi, P.vm[q], P.dyh[j + d.ng], xn, x_0, rn, rc, P.dt) : vi;
+                P.wo[q] = s < nz - P.lz ? proj_face(wi, P.wm[q], P.dzh[s + d.ng], x_p, x_0, rt, rc, P.dt) : wi;
                 rc = rt;
             }
-            p[q] = x0;
-            if (d.px && i == d.nx - 1) p[q - d.nx] = x0;
-            if (d.px && i == 0) p[q + d.nx] = x0;
-            if (d.py && j == d.ny - 1) p[q - (long long)d.ny * Py] = x0;
-            if (d.py && j == 0) p[q + (long long)d.ny * Py] = x0;
-            if (d.pz && k == d.nz - 1) p[q - (long long)d.nz * Pz] = x0;
-            if (d.pz && k == 0) p[q + (long long)d.nz * Pz] = x0;
-            xm = x0;
-            x0 = xp;
+            p[q] = x_0;
+            if (d.px && i == d.nx - 1) p[q - d.nx] = x_0;
+            if (d.px && i == 0) p[q + d.nx] = x_0;
+            if (d.py && j == d.ny - 1) p[q - (long long)d.ny * Py] = x_0;
+            if (d.py && j == 0) p[q + (long long)d.ny * Py] = x_0;
+            if (d.pz && s == d.nz - 1) p[q - (long long)d.nz * Pz] = x_0;
+            if (d.pz && s == 0) p[q + (long long)d.nz * Pz] = x_0;
         }
+        x_m = x_0; x_0 = x_p; czm = czp;
+        xo1 = xo2; do1 = do2; xh1 = xh2; dh1 = dh2;
+        __syncthreads();
     }
     block_partial(rr, part);
 }
@@ -2918,10 +2994,14 @@ static int fast_setup(pps_handle *h, const double *dA, const double *dp, bool mi
 
 // The outer iteration on (b, x): PCG or refinement with the V-cycle of
 // hierarchy H.  Convergence: sqrt(rr/N) < tol_abs (norm 2).  Returns the
-// iteration count in *it (maxit+1 if not converged).
+// iteration count in *it (maxit+1 if not converged).  The last iteration's
+// x += alpha d is applied here, or, given pend, left to the caller: *pend is
+// then the direction d (null when nothing is pending), alpha S[S_ALPHA].
 template <class T>
 static int outer_iterate(pps_handle *h, Hier<T> &H, bool mixed, double tol_abs, int maxit,
-                         int *it, double *last, bool record, hipStream_t s) {
+                         int *it, double *last, bool record, hipStream_t s,
+                         const void **pend = nullptr) {
+    if (pend) *pend = nullptr;
     FastState *f = h->fast;
     const long long n = f->d.cells();
     const OpK O = opk(f);
@@ -3053,10 +3133,31 @@ static int outer_iterate(pps_handle *h, Hier<T> &H, bool mixed, double tol_abs, 
         }
     }
     // the last iteration's x += alpha d
-    if (cg) k_axpy_d<T><<<nb, 256, 0, s>>>(n, V.x, (const T *)(const void *)V.d[cur], S);
+    if (cg && pend) *pend = V.d[cur];
+    else if (cg) k_axpy_d<T><<<nb, 256, 0, s>>>(n, V.x, (const T *)(const void *)V.d[cur], S);
     PPS_HIP(hipGetLastError());
     *it = k;
     return PPS_OK;
+}
+
+// the outer scalars S (after the four partial-sum slots)
+static double *part_S(FastState *f) { return f->red.as<double>() + 4 * kRedBlocks; }
+
+// The final check (k_check_tiled) of the fast solve; dd: the pending
+// direction of the last x += alpha d (null for none).
+template <class TD>
+static void launch_check(FastState *f, const Dims &d, const TGrid &tg, const TD *dd, const double *S,
+                         double *dp, double *part, bool proj, const ProjK &P, const double *wx,
+                         const double *wy, const double *wz, hipStream_t s) {
+    const dim3 blk(64, 16, 1);
+    const Vecs V = vecs(f);
+    if (proj) {
+        auto k = f->exact ? k_check_tiled<true, true, TD> : k_check_tiled<false, true, TD>;
+        k<<<tg.blocks, blk, 0, s>>>(opk(f), packk(d), tg, V.b, V.x, dd, S, dp, part, P, wx, wy, wz);
+    } else {
+        auto k = f->exact ? k_check_tiled<true, false, TD> : k_check_tiled<false, false, TD>;
+        k<<<tg.blocks, blk, 0, s>>>(opk(f), packk(d), tg, V.b, V.x, dd, S, dp, part, P, wx, wy, wz);
+    }
 }
 
 // relative: maxError bounds ||r||_2 / ||b||_2 (b the folded right-hand
@@ -3084,6 +3185,8 @@ int fast_solve(pps_handle *h, const double *dA, double *dp, double maxError, int
     int total = 0, k = 0;
     double t2 = 0.0, last = 0.0;
     int rc = PPS_OK;
+    // norm 2: the last iteration's x += alpha d is fused into the final check
+    const void *pend = nullptr;
     // iterate on the recursive residual, then confirm with the true residual
     // b - M x (norm 2: the FP64 operator, fused with the write-back of p and,
     // given proj, with project_velocity into separate outputs, which a
@@ -3094,28 +3197,25 @@ int fast_solve(pps_handle *h, const double *dA, double *dp, double maxError, int
         const int budget = maxit - total;
         // weighted rows: the recursion sees W r, and |r| <= |W r| / min w
         const double tol_rec = h->have_widths ? maxError * h->wmin : maxError;
-        if (mixed) rc = outer_iterate<float>(h, f->h32, true, tol_rec, budget, &k, &last, true, s);
-        else rc = outer_iterate<double>(h, f->h64, false, tol_rec, budget, &k, &last, true, s);
+        const void **pp = norm == 2 ? &pend : nullptr;
+        if (mixed) rc = outer_iterate<float>(h, f->h32, true, tol_rec, budget, &k, &last, true, s, pp);
+        else rc = outer_iterate<double>(h, f->h64, false, tol_rec, budget, &k, &last, true, s, pp);
         if (rc != PPS_OK && rc != PPS_ERR_NOT_CONVERGED) break;
         const bool exhausted = k > budget;
         total += exhausted ? budget : k;
         if (norm == 2) {
-            const ZGrid zg = zgrid(d.nx, d.ny, d.nz);
+            const TGrid tg = tgrid(d.nx, d.ny, d.nz);
             double *part = f->red.as<double>();
             double *S = part + 4 * kRedBlocks;
             const double *w = h->widths.as<double>();
             const double *wx = h->have_widths ? w : nullptr;
             const double *wy = wx ? w + d.nx : nullptr, *wz = wx ? w + d.nx + d.ny : nullptr;
-            if (proj) {
-                auto kcu = f->exact ? k_check_unpack<true, true> : k_check_unpack<false, true>;
-                kcu<<<zg.grid, dim3(64, 4, 1), 0, s>>>(opk(f), packk(d), zg.zc, vecs(f).b, vecs(f).x, dp,
-                                                       part, *proj, wx, wy, wz);
-            } else {
-                auto kcu = f->exact ? k_check_unpack<true, false> : k_check_unpack<false, false>;
-                kcu<<<zg.grid, dim3(64, 4, 1), 0, s>>>(opk(f), packk(d), zg.zc, vecs(f).b, vecs(f).x, dp,
-                                                       part, ProjK{}, wx, wy, wz);
-            }
-            k_scal_rr<<<1, 1024, 0, s>>>(part, zg.blocks, 0, S);
+            const ProjK P = proj ? *proj : ProjK{};
+            if (mixed) launch_check<float>(f, d, tg, (const float *)pend, S, dp, part, proj != nullptr, P, wx,
+                                           wy, wz, s);
+            else launch_check<double>(f, d, tg, (const double *)pend, S, dp, part, proj != nullptr, P, wx, wy,
+                                      wz, s);
+            k_scal_rr<<<1, 1024, 0, s>>>(part, tg.blocks, 0, S);
             PPS_HIP(hipGetLastError());
             PPS_HIP(hipMemcpyAsync(h->pinned, S + S_RR, sizeof(double), hipMemcpyDeviceToHost, s));
             PPS_HIP(hipStreamSynchronize(s));
@@ -3129,6 +3229,14 @@ int fast_solve(pps_handle *h, const double *dA, double *dp, double maxError, int
         }
         if (t2 < maxError) break;
         if (exhausted) break;
+        // a restart iterates on from x: apply the pending update first
+        if (pend) {
+            const long long n = d.cells();
+            if (mixed) k_axpy_d<float><<<nblk(n), 256, 0, s>>>(n, vecs(f).x, (const float *)pend, part_S(f));
+            else k_axpy_d<double><<<nblk(n), 256, 0, s>>>(n, vecs(f).x, (const double *)pend, part_S(f));
+            PPS_HIP(hipGetLastError());
+            pend = nullptr;
+        }
     }
     PPS_HIP(hipEventRecord(h->ev[2], s));
     PPS_HIP(hipEventSynchronize(h->ev[2]));

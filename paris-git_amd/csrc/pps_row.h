@@ -27,10 +27,22 @@ struct BcK {
 // y-, y+ carry code 5).  Code 6 (x+ only): P_b and du/dn at the face (du: the
 // cell's (u(i) - u(i-1)) / dx(i)).  The z+ block of code 5 writes its
 // y-coupling corner entries into the ks plane (modules.f90:2884-2887): the
-// four corner cells of the first plane take them last.
+// four corner cells of the first plane take them last.  FLUX_ONLY: codes 0-4
+// only (the callers that refuse codes 5 and 6), which leaves the flux
+// condition of codes 3 / 4 in the same block order.
+template <bool FLUX_ONLY = false>
 __device__ __forceinline__ void poisson_bcs(const BcK &bc, int nx, int ny, int nz, int i, int j, int k,
                                             double du, double a[8]) {
     const int ie = nx - 1, je = ny - 1, ke = nz - 1;
+    if constexpr (FLUX_ONLY) {
+        if (i == 0 && bc.b[0] == 3) { a[6] = a[6] - a[0]; a[0] = 0.0; }
+        if (i == ie && (bc.b[3] == 4 || bc.b[3] == 3)) { a[6] = a[6] - a[1]; a[1] = 0.0; }
+        if (j == 0 && bc.b[1] == 3) { a[6] = a[6] - a[2]; a[2] = 0.0; }
+        if (j == je && bc.b[4] == 3) { a[6] = a[6] - a[3]; a[3] = 0.0; }
+        if (k == 0 && bc.b[2] == 3) { a[6] = a[6] - a[4]; a[4] = 0.0; }
+        if (k == ke && bc.b[5] == 3) { a[6] = a[6] - a[5]; a[5] = 0.0; }
+        return;
+    }
     const double c23 = 2.0 / 3.0, c13 = 1.0 / 3.0;
     const double pbc0 = bc.b[0] == 5 ? 1.0 : 0.0, pbc1 = bc.b[3] == 5 ? 1.0 : 0.0;
     const double pbc2 = bc.b[1] == 5 ? 1.0 : 0.0, pbc3 = bc.b[4] == 5 ? 1.0 : 0.0;
@@ -131,9 +143,13 @@ __device__ __forceinline__ double face_coupling(double dt, double mask, double d
 
 // Row of interior cell (i, j, k) (padded index q; padded strides sy, sz) into
 // a[0..7]; returns 1 when the solid rule meets a row that is not a closed
-// solid cell (the caller's error flag), else 0.
+// solid cell (the caller's error flag), else 0.  FLUX_ONLY: boundary codes
+// 0-4 only (poisson_bcs<true>).  raw (not null): the +x, +y, +z couplings
+// as SetupPoisson forms them, before Poisson_BCs.
+template <bool FLUX_ONLY = false>
 __device__ __forceinline__ int setup_row(const RowK &R, int nx, int ny, int nz, int ng, long long sy,
-                                         long long sz, int i, int j, int k, long long q, double a[8]) {
+                                         long long sz, int i, int j, int k, long long q, double a[8],
+                                         double *raw = nullptr) {
 #pragma clang fp contract(off)
     const int NX = nx + 2 * ng, NY = ny + 2 * ng, NZ = nz + 2 * ng;
     const double *dx = R.ax, *dxh = R.ax + NX, *dy = R.ay, *dyh = R.ay + NY, *dz = R.az, *dzh = R.az + NZ;
@@ -147,9 +163,10 @@ __device__ __forceinline__ int setup_row(const RowK &R, int nx, int ny, int nz, 
     a[4] = face_coupling(dt, wm[q - sz], dz[K], dzh[K - 1], rho[q - sz], r0);
     a[5] = face_coupling(dt, wm[q], dz[K], dzh[K], rho[q + sz], r0);
     a[6] = ((((a[0] + a[1]) + a[2]) + a[3]) + a[4]) + a[5];
+    if (raw) { raw[0] = a[1]; raw[1] = a[3]; raw[2] = a[5]; }
     const double dudx = (R.u[q] - R.u[q - 1]) / dx[I];
     a[7] = -(((R.vs + dudx) + (R.v[q] - R.v[q - sy]) / dy[J]) + (R.w[q] - R.w[q - sz]) / dz[K]);
-    poisson_bcs(R.bc, nx, ny, nz, i, j, k, dudx, a);
+    poisson_bcs<FLUX_ONLY>(R.bc, nx, ny, nz, i, j, k, dudx, a);
     int bad = 0;
     if (a[6] < 1e-50) {
         const bool solid = um[q - 1] < 0.5 && um[q] < 0.5 && vm[q - sy] < 0.5 && vm[q] < 0.5 &&
