@@ -11,6 +11,7 @@
 // a non-periodic axis, which couples to no cell) get 0.  A caller whose
 // system also carries masks (SetupPoisson's umask, ...) multiplies by them.
 #include "pps_internal.h"
+#include "pps_xcd.h"
 
 namespace pps {
 
@@ -20,23 +21,29 @@ struct FaceK {
 };
 
 // Workgroups of 64 x 16 threads own (i, j) columns and march through zc
-// planes.  Each plane of p and lam is fetched once: the +z values are
-// loaded one plane ahead and carried, the +x neighbour comes from the next
-// lane of the wavefront (a shuffle; the last lane of a row segment and the
-// last cell of a row load it), the +y neighbour from the next thread row
-// through LDS (the last thread row and the last row of the grid load it).
-// Threads past the grid work on a clamped cell and store nothing.
+// planes (a one-dimensional grid of gx x gy tiles x z chunks, numbered XCD
+// by XCD so that neighbouring tiles share an L2).  Each plane of p and lam
+// is fetched once: the +z values are requested two planes ahead and
+// carried, so a step waits on no load of its own but the +x / +y edge
+// neighbours; the +x neighbour comes from the next lane of the wavefront (a
+// shuffle; the last lane of a row segment and the last cell of a row load
+// it), the +y neighbour from the next thread row through LDS (the last
+// thread row and the last row of the grid load it).  The gradient planes
+// are written once, with non-temporal stores.  Threads past the grid work
+// on a clamped cell and store nothing.
 // Algorithmic bytes: p, lam read (16 B), three gradient planes written
 // (24 B) per cell.
 constexpr int kFgRows = 16;
 __global__ void __launch_bounds__(64 * kFgRows)
-k_face_grad(FaceK d, int zc, const double *__restrict__ p, const double *__restrict__ lam,
+k_face_grad(FaceK d, int gx, int gy, int zc, const double *__restrict__ p, const double *__restrict__ lam,
             const double *__restrict__ wx, const double *__restrict__ wy,
             const double *__restrict__ wz, double *__restrict__ g) {
     __shared__ double sp[2][kFgRows][64], sl[2][kFgRows][64];
+    const int bid = xcd_order(blockIdx.x, gridDim.x);
+    const int bx = bid % gx, by = (bid / gx) % gy, bz = bid / (gx * gy);
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int i = blockIdx.x * 64 + tx, j = blockIdx.y * kFgRows + ty;
-    const int k0 = blockIdx.z * zc;
+    const int i = bx * 64 + tx, j = by * kFgRows + ty;
+    const int k0 = bz * zc;
     const int k1 = min(k0 + zc, d.nz);
     const bool in = i < d.nx && j < d.ny;
     const int ic = min(i, d.nx - 1), jc = min(j, d.ny - 1);
@@ -54,39 +61,55 @@ k_face_grad(FaceK d, int zc, const double *__restrict__ p, const double *__restr
     const bool xl = tx == 63 || ic == d.nx - 1;
     const bool yl = ty == kFgRows - 1 || jc == d.ny - 1;
     const long long cc = ic + (long long)d.nx * jc, sz = (long long)d.nx * d.ny;
-    long long q = col + Pz * (k0 + d.ng);
+    // padded index of plane kk of this column: wrapped past the last plane
+    // of a periodic z axis, else clamped (the value is then not used)
+    auto plane = [&](int kk) -> long long {
+        const int kw = kk < d.nz ? kk : (d.pz ? kk % d.nz : d.nz - 1);
+        return col + Pz * (kw + d.ng);
+    };
+    long long q = plane(k0), q1 = plane(k0 + 1);
     double pc = p[q], lc = lam[q];
+    double p1 = p[q1], l1 = lam[q1];
     for (int k = k0; k < k1; ++k) {
         const int buf = k & 1;
-        sp[buf][ty][tx] = pc;
-        sl[buf][ty][tx] = lc;
-        // +z neighbour: the next plane, or the first one (periodic z)
-        const bool oz = k < d.nz - 1 || d.pz;
-        const long long qz = k < d.nz - 1 ? q + Pz : q - (long long)(d.nz - 1) * Pz;
-        const double pzn = p[qz], lzn = lam[qz];
-        double pxe = __shfl_down(pc, 1, 64), lxe = __shfl_down(lc, 1, 64);
+        // requests: plane k+2, and the edge neighbours of plane k
+        const long long q2 = plane(k + 2);
+        const double p2 = p[q2], l2 = lam[q2];
+        double pxe = 0.0, lxe = 0.0, pye = 0.0, lye = 0.0;
         if (xl) {
             pxe = p[q + ex];
             lxe = lam[q + ex];
         }
-        __syncthreads();
-        double pye, lye;
         if (yl) {
             pye = p[q + ey];
             lye = lam[q + ey];
-        } else {
+        }
+        sp[buf][ty][tx] = pc;
+        sl[buf][ty][tx] = lc;
+        const double pxs = __shfl_down(pc, 1, 64), lxs = __shfl_down(lc, 1, 64);
+        if (!xl) {
+            pxe = pxs;
+            lxe = lxs;
+        }
+        __syncthreads();
+        if (!yl) {
             pye = sp[buf][ty + 1][tx];
             lye = sl[buf][ty + 1][tx];
         }
         if (in) {
+            // +z neighbour: the next plane, or the first one (periodic z)
+            const bool oz = k < d.nz - 1 || d.pz;
             const long long c = cc + sz * k;
-            g[c] = gxw * (lc - lxe) * (pc - pxe);
-            g[n + c] = gyw * (lc - lye) * (pc - pye);
-            g[2 * n + c] = oz ? -wz[k] * (lc - lzn) * (pc - pzn) : 0.0;
+            __builtin_nontemporal_store(gxw * (lc - lxe) * (pc - pxe), g + c);
+            __builtin_nontemporal_store(gyw * (lc - lye) * (pc - pye), g + n + c);
+            __builtin_nontemporal_store(oz ? -wz[k] * (lc - l1) * (pc - p1) : 0.0, g + 2 * n + c);
         }
-        pc = pzn;
-        lc = lzn;
-        q += Pz;
+        pc = p1;
+        lc = l1;
+        p1 = p2;
+        l1 = l2;
+        q = q1;
+        q1 = q2;
     }
 }
 
@@ -156,8 +179,8 @@ extern "C" int pps_face_gradient_device(const pps_grid *grid, const double *dp, 
     if (chunks > d.nz) chunks = d.nz;
     const int zc = (int)((d.nz + chunks - 1) / chunks);
     chunks = (d.nz + zc - 1) / zc;
-    k_face_grad<<<dim3((unsigned)gx, (unsigned)gy, (unsigned)chunks), dim3(64, kFgRows, 1), 0, s>>>(
-        d, zc, dp, dlam, wx, wy, wz, dgrad);
+    k_face_grad<<<(unsigned)(gx * gy * chunks), dim3(64, kFgRows, 1), 0, s>>>(d, gx, gy, zc, dp, dlam, wx, wy,
+                                                                           wz, dgrad);
     PPS_HIP(hipGetLastError());
     PPS_HIP(hipStreamSynchronize(s));
     return PPS_OK;

@@ -34,6 +34,7 @@
 
 #include <hip/hip_ext.h>
 #include "pps_row.h"
+#include "pps_xcd.h"
 
 namespace pps {
 
@@ -182,16 +183,6 @@ template <int I> struct Ic { static constexpr int value = I; };
 // 1024 threads leave 128, and the LOP form spills there)
 constexpr int kLopCap = 12800;
 constexpr int kLopThreads = 768;
-
-// Workgroups go to the 8 XCDs round-robin by id (each XCD has its own L2).
-// Renumbers ids so that each XCD's workgroups are consecutive: neighbouring
-// tiles, which read each other's edge rows, then run on one XCD and find
-// those rows in its L2 (k_dq_tiled fetches 680 MB instead of 877 MB per
-// 256^3 launch).
-__device__ __forceinline__ int xcd_order(int b, int G) {
-    const int q = G >> 3, r = G & 7, x = b & 7, s = b >> 3;
-    return x < r ? x * (q + 1) + s : r * (q + 1) + (x - r) * q + s;
-}
 
 __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
