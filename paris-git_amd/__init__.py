@@ -153,6 +153,8 @@ def _lib():
         "pps_last_history": (_I, [_P, _DP, _I]),
         "pps_last_timing": (_I, [_P, _DP, _DP]),
         "pps_adjoint_device": (_I, [_P, _P, _P, _P, _D, _I, _IP, _P]),
+        "pps_solve_faces_device": (_I, [_P, _P, _P, _P, _P, _P, _P, _D, _I, _IP, _DP, _P]),
+        "pps_adjoint_faces_device": (_I, [_P, _P, _P, _P, _P, _P, _P, _D, _I, _IP, _P]),
         "pps_face_gradient_device": (_I, [_GP, _P, _P, _P, _P, _P, _P, _P]),
         "pps_pressure_masks_device": (_I, [_GP, _P, _P, _P, _P]),
         "pps_linfunc_device": (_I, [_GP, _P, _D, _D, _I, _I, _P, _P]),
@@ -196,6 +198,7 @@ def exported_symbols():
         "pps_solve_project_device", "pps_solve_fields_device",
         "pps_last_history",
         "pps_last_timing", "pps_adjoint_device", "pps_face_gradient_device",
+        "pps_solve_faces_device", "pps_adjoint_faces_device",
         "pps_pressure_masks_device", "pps_linfunc_device", "pps_setup_poisson_device",
         "pps_setup_poisson_bp_device", "pps_setup_poisson_coords_device",
         "pps_check_setup_device", "pps_check_poisson_setup_device",
@@ -515,6 +518,34 @@ class Solver:
         _check(rc, "pps_adjoint_device")
         return it.value
 
+    def solve_faces(self, cface, weights, b, p, maxError, maxit=50, stream=None, allow_maxit=True):
+        """faces_to_A(grid, cface, weights, b) then solve(), without forming
+        A: the rows come from the face values inside the setup pass (fast
+        methods).  Returns (it, tres2)."""
+        _need_gpu(cface, b, p)
+        it, t2 = ctypes.c_int(), ctypes.c_double()
+        g = self.grid
+        rc = _lib().pps_solve_faces_device(self._h, _ptr(cface, 3 * g.cells, "cface"),
+                                           *_face_weight_ptrs(g, weights), _ptr(b, g.cells, "b"),
+                                           _ptr(p, _np(g), "p"), maxError, maxit, ctypes.byref(it),
+                                           ctypes.byref(t2), _stream(stream))
+        _check(rc, "pps_solve_faces_device",
+               allow=(OK, ERR_NOT_CONVERGED) if allow_maxit else (OK,))
+        return it.value, t2.value
+
+    def adjoint_faces(self, cface, weights, g, lam, rtol=1e-12, maxit=100, stream=None):
+        """adjoint() for the system of face values cface: reuses the
+        hierarchy of the last solve_faces on the same cface tensor."""
+        _need_gpu(cface, g, lam)
+        it = ctypes.c_int()
+        gr = self.grid
+        rc = _lib().pps_adjoint_faces_device(self._h, _ptr(cface, 3 * gr.cells, "cface"),
+                                             *_face_weight_ptrs(gr, weights), _ptr(g, _np(gr), "g"),
+                                             _ptr(lam, _np(gr), "lam"), rtol, maxit, ctypes.byref(it),
+                                             _stream(stream))
+        _check(rc, "pps_adjoint_faces_device")
+        return it.value
+
     def history(self, cap=100000):
         buf = (ctypes.c_double * cap)()
         n = _lib().pps_last_history(self._h, buf, cap)
@@ -769,32 +800,46 @@ def _autograd():
         """p = M(c)^-1 b with M assembled from face values c.
 
         forward(cface (3,nz,ny,nx), b (nz,ny,nx), solver, weights, tol, maxit)
-        returns the interior pressure (nz,ny,nx).  backward solves the
-        adjoint M lam = grad_p on the same hierarchy (M is symmetric) and
-        forms grad_c = -w (lam_L - lam_R)(p_L - p_R) with the fused face
-        kernel; grad_b = lam.  The forward pressure is kept, not recomputed.
+        returns the interior pressure (nz,ny,nx).  With the fast methods the
+        8-slot system is never formed: the solve takes its rows from the
+        face values.  backward solves the adjoint M lam = grad_p on the same
+        hierarchy (M is symmetric) and forms grad_c = -w (lam_L - lam_R)
+        (p_L - p_R) with the fused face kernel; grad_b = lam.  The forward
+        pressure is kept, not recomputed.
         """
 
         @staticmethod
         def forward(ctx, cface, b, solver, weights, tol=1e-12, maxit=200):
             grid = solver.grid
-            A = faces_to_A(grid, cface.detach().contiguous(), weights, b.detach().contiguous())
+            cf = cface.detach().contiguous()
+            bc = b.detach().contiguous()
             p = torch.zeros(grid.shape_p, dtype=torch.float64, device=b.device)
-            bn = float(torch.sqrt((b.detach() ** 2).mean()))
-            solver.solve(A, p, tol * bn if bn > 0 else tol, 1.0, maxit, 2, allow_maxit=False)
+            bn = float(torch.sqrt((bc ** 2).mean()))
+            tol_abs = tol * bn if bn > 0 else tol
+            if solver.options.method in (METHODS["fp64"], METHODS["mixed"]):
+                solver.solve_faces(cf, weights, bc, p, tol_abs, maxit, allow_maxit=False)
+                A = None
+            else:
+                A = faces_to_A(grid, cf, weights, bc)
+                solver.solve(A, p, tol_abs, 1.0, maxit, 2, allow_maxit=False)
             ctx.solver, ctx.weights, ctx.tol, ctx.maxit = solver, weights, tol, maxit
-            ctx.save_for_backward(A, p)
+            ctx.A = A
+            ctx.save_for_backward(cf, p)
             return grid.interior(p).clone()
 
         @staticmethod
         def backward(ctx, grad_p):
-            A, p = ctx.saved_tensors
+            cf, p = ctx.saved_tensors
             solver = ctx.solver
             grid = solver.grid
-            g = torch.zeros(grid.shape_p, dtype=torch.float64, device=A.device)
+            g = torch.zeros(grid.shape_p, dtype=torch.float64, device=p.device)
             grid.interior(g).copy_(grad_p)
             lam = torch.empty_like(g)
-            solver.adjoint(A, g, lam, rtol=ctx.tol, maxit=ctx.maxit)
+            if ctx.A is None:
+                solver.adjoint_faces(cf, ctx.weights, g, lam, rtol=ctx.tol, maxit=ctx.maxit)
+            else:
+                solver.adjoint(ctx.A, g, lam, rtol=ctx.tol, maxit=ctx.maxit)
+            ctx.A = None
             gc = face_gradient(grid, p, lam, ctx.weights)
             return gc, grid.interior(lam).clone(), None, None, None, None
 

@@ -118,6 +118,9 @@ __global__ void __launch_bounds__(256)
 k_faces_to_A(FaceK d, const double *__restrict__ cf, const double *__restrict__ wx,
              const double *__restrict__ wy, const double *__restrict__ wz,
              const double *__restrict__ b, double *__restrict__ A) {
+    // the same products and sum as the fast solve's rows from face values
+    // (RowsC in pps_fast.hip), which form this system without writing it
+#pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int k = blockIdx.z;

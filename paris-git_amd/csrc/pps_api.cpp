@@ -481,6 +481,42 @@ extern "C" int pps_profile_read(pps_handle *h, double *kernel_ms, long long *lau
     return PPS_OK;
 }
 
+// The solve of the system that pps_faces_to_A_device would form, with its
+// rows taken from the face values inside the fast solve's setup pass.
+extern "C" int pps_solve_faces_device(pps_handle *h, const double *cface, const double *wx,
+                                      const double *wy, const double *wz, const double *b, double *dp,
+                                      double maxError, int maxit, int *it, double *tres2,
+                                      void *stream) {
+    if (!h || !cface || !wx || !wy || !wz || !b || !dp || !it || !tres2 || maxit < 1)
+        return PPS_ERR_BAD_ARG;
+    const int method = h->opt.method;
+    if ((method != PPS_MG_FP64 && method != PPS_MG_MIXED) || h->opt.check_setup) return PPS_ERR_UNSUPPORTED;
+    (void)hipSetDevice(h->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);   // NULL: the default stream
+    h->setup_ms = 0.0;
+    const FacesK F{cface, wx, wy, wz, b};
+    const int rc = fast_solve(h, nullptr, dp, maxError, maxit, 2, it, tres2, s, false, nullptr, nullptr, &F);
+    if (rc != PPS_OK) return rc;
+    return *it > maxit ? PPS_ERR_NOT_CONVERGED : PPS_OK;
+}
+
+// The adjoint of that system on the hierarchy of the last solve from the
+// same face values (set up again from them otherwise).
+extern "C" int pps_adjoint_faces_device(pps_handle *h, const double *cface, const double *wx,
+                                        const double *wy, const double *wz, const double *dg,
+                                        double *dlam, double rtol, int maxit, int *it, void *stream) {
+    if (!h || !cface || !wx || !wy || !wz || !dg || !dlam || !it || !(rtol > 0.0) || maxit < 1)
+        return PPS_ERR_BAD_ARG;
+    (void)hipSetDevice(h->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int saved = h->opt.method;
+    if (saved != PPS_MG_MIXED) h->opt.method = PPS_MG_FP64;
+    const FacesK F{cface, wx, wy, wz, nullptr};
+    const int rc = fast_adjoint(h, nullptr, dg, dlam, rtol, maxit, it, s, &F);
+    h->opt.method = saved;
+    return rc;
+}
+
 extern "C" int pps_adjoint_device(pps_handle *h, const double *dA, const double *dg, double *dlam,
                                   double rtol, int maxit, int *it, void *stream) {
     if (!h || !dA || !dg || !dlam || !it || !(rtol > 0.0) || maxit < 1) return PPS_ERR_BAD_ARG;

@@ -1916,6 +1916,41 @@ struct RowsF {
     }
 };
 
+// Rows from face values (the map of k_faces_to_A in pps_adjoint.hip, the
+// same products and the same sum, bit for bit): A1..A6 the weighted values
+// w c of the cell's six faces (0 through a wall), A7 their sum, A8 = b.
+// The system is symmetric by construction (a neighbour's lower coupling is
+// the same product as the row's upper one), so the check gets the row's
+// own value.
+struct RowsC {
+    FacesK F;
+    int nx, ny, nz, px, py, pz;
+    double up[3];
+    __device__ void begin(long long) {}
+    __device__ int row(const CellAt &e, double a[8]) {
+#pragma clang fp contract(off)
+        const long long n = (long long)nx * ny * nz, sy = nx, sz = (long long)nx * ny;
+        const double *cx = F.cf, *cy = F.cf + n, *cz = F.cf + 2 * n;
+        const int i = e.i, j = e.j, k = e.k;
+        const long long c = e.c;
+        a[0] = i > 0 ? F.wx[i - 1] * cx[c - 1] : (px ? F.wx[nx - 1] * cx[c + nx - 1] : 0.0);
+        a[1] = (i < nx - 1 || px) ? F.wx[i] * cx[c] : 0.0;
+        a[2] = j > 0 ? F.wy[j - 1] * cy[c - sy] : (py ? F.wy[ny - 1] * cy[c + (long long)(ny - 1) * sy] : 0.0);
+        a[3] = (j < ny - 1 || py) ? F.wy[j] * cy[c] : 0.0;
+        a[4] = k > 0 ? F.wz[k - 1] * cz[c - sz] : (pz ? F.wz[nz - 1] * cz[c + (long long)(nz - 1) * sz] : 0.0);
+        a[5] = (k < nz - 1 || pz) ? F.wz[k] * cz[c] : 0.0;
+        a[6] = ((((a[0] + a[1]) + a[2]) + a[3]) + a[4]) + a[5];
+        a[7] = F.b ? F.b[c] : 0.0;
+        up[0] = a[1];
+        up[1] = a[3];
+        up[2] = a[5];
+        return 0;
+    }
+    __device__ double nb_x(const CellAt &, long long) const { return up[0]; }
+    __device__ double nb_y(const CellAt &, long long) const { return up[1]; }
+    __device__ double nb_z(const CellAt &, long long) { return up[2]; }
+};
+
 template <class Src>
 __global__ void __launch_bounds__(256)
 k_setup(PackK d, int zc, Src src, const double *__restrict__ p,
@@ -2891,7 +2926,7 @@ static int wait_mailbox(pps_handle *h, hipStream_t s) {
 // first V-cycle; outer_iterate builds it again on the solve stream if the
 // flags say the system is not singular.
 static int fast_setup(pps_handle *h, const double *dA, const double *dp, bool mixed, hipStream_t s,
-                      const RowK *rows = nullptr, bool defer = false) {
+                      const RowK *rows = nullptr, bool defer = false, const FacesK *faces = nullptr) {
     FastState *f = h->fast;
     const Dims &d = f->d;
     const long long n = d.cells();
@@ -2924,12 +2959,16 @@ static int fast_setup(pps_handle *h, const double *dA, const double *dp, bool mi
     const double *w = h->widths.as<double>();
     const double *wx = h->have_widths ? w : nullptr;
     const double *wy = wx ? w + d.nx : nullptr, *wz = wx ? w + d.nx + d.ny : nullptr;
-    const bool own = !mixed || wx || rows;
+    const bool own = !mixed || wx || rows || faces;
     double *ocx = own ? o64 : nullptr, *ocy = own ? o64 + n : nullptr;
     double *ocz = own ? o64 + 2 * n : nullptr, *odg = own ? o64 + 3 * n : nullptr;
     if (rows) {
         const RowsF src{*rows, d.nx, d.ny, d.nz, d.ng, (long long)d.X(), (long long)d.X() * d.Y()};
         k_setup<RowsF><<<zg.grid, dim3(64, 4, 1), 0, s>>>(packk(d), zg.zc, src, dp, ocx, ocy, ocz, odg, V.b,
+                                                          V.x, V.r, f32, r32, part, flags, wx, wy, wz);
+    } else if (faces) {
+        const RowsC src{*faces, d.nx, d.ny, d.nz, d.per[0], d.per[1], d.per[2], {0.0, 0.0, 0.0}};
+        k_setup<RowsC><<<zg.grid, dim3(64, 4, 1), 0, s>>>(packk(d), zg.zc, src, dp, ocx, ocy, ocz, odg, V.b,
                                                           V.x, V.r, f32, r32, part, flags, wx, wy, wz);
     } else {
         const RowsA src{dA, n, 0.0};
@@ -2976,8 +3015,9 @@ static int fast_setup(pps_handle *h, const double *dA, const double *dp, bool mi
     }
     f->have_r = 1;
     PPS_HIP(hipGetLastError());
-    // an operator assembled from fields is not kept for a later adjoint
-    f->built_for = rows ? nullptr : dA;
+    // an operator assembled from fields is not kept for a later adjoint; one
+    // from face values is known by their address
+    f->built_for = rows ? nullptr : (faces ? faces->cf : dA);
     f->built_mixed = mixed ? 1 : 0;
     f->op_in_A = own ? 0 : 1;
     return PPS_OK;
@@ -3155,15 +3195,15 @@ static void launch_check(FastState *f, const Dims &d, const TGrid &tg, const TD 
 // side) instead of sqrt(sum r^2 / N)
 int fast_solve(pps_handle *h, const double *dA, double *dp, double maxError, int maxit, int norm,
                int *it, double *tres2, hipStream_t s, bool relative, const ProjK *proj,
-               const RowK *rows) {
-    if (rows && norm != 2) return PPS_ERR_BAD_ARG;
+               const RowK *rows, const FacesK *faces) {
+    if ((rows || faces) && norm != 2) return PPS_ERR_BAD_ARG;
     if (proj && norm != 2) return PPS_ERR_BAD_ARG;
     PPS_TRY(fast_ensure(h));
     FastState *f = h->fast;
     const bool mixed = h->opt.method == PPS_MG_MIXED;
     h->hist.clear();
     PPS_HIP(hipEventRecord(h->ev[0], s));
-    PPS_TRY(fast_setup(h, dA, dp, mixed, s, rows, true));
+    PPS_TRY(fast_setup(h, dA, dp, mixed, s, rows, true, faces));
     PPS_HIP(hipEventRecord(h->ev[1], s));
     const Dims &d = f->d;
     const double N = (double)d.cells();
@@ -3249,7 +3289,7 @@ int fast_solve(pps_handle *h, const double *dA, double *dp, double maxError, int
 // singular case projects g mean-free either way (the null space of S, and
 // of M, is the constants).
 int fast_adjoint(pps_handle *h, const double *dA, const double *dg, double *dlam, double rtol,
-                 int maxit, int *it, hipStream_t s) {
+                 int maxit, int *it, hipStream_t s, const FacesK *faces) {
     PPS_TRY(fast_ensure(h));
     FastState *f = h->fast;
     const bool mixed = h->opt.method == PPS_MG_MIXED;
@@ -3257,8 +3297,9 @@ int fast_adjoint(pps_handle *h, const double *dA, const double *dg, double *dlam
     const long long n = d.cells();
     // rebuild only if the operator changed; the fold of wall ghosts into
     // the right-hand side is irrelevant here (the adjoint ghosts are zero)
-    if (f->built_for != dA || f->built_mixed != (mixed ? 1 : 0)) {
-        PPS_TRY(fast_setup(h, dA, dlam, mixed, s));
+    const double *op = faces ? faces->cf : dA;
+    if (f->built_for != op || f->built_mixed != (mixed ? 1 : 0)) {
+        PPS_TRY(fast_setup(h, dA, dlam, mixed, s, nullptr, false, faces));
     }
     Vecs V = vecs(f);
     k_interior<<<cgrid3(d.nx, d.ny, d.nz), kB3, 0, s>>>(packk(d), dg, V.b);

@@ -203,15 +203,24 @@ struct ProjK {
     int lx, ly, lz;
 };
 
-// rows: assemble the system from the fields instead of reading dA (which
-// may then be null; norm 2 only)
+// Face values as the rows of the system (the map of pps_faces_to_A_device):
+// cf = three planes of face values (+x, +y, +z face of every cell),
+// wx / wy / wz the per-plane weights, b the interior right-hand side (null:
+// zero, for an adjoint that only needs the operator).
+struct FacesK {
+    const double *cf, *wx, *wy, *wz, *b;
+};
+
+// rows: assemble the system from the fields instead of reading dA; faces:
+// from face values (dA may then be null; norm 2 only)
 struct RowK;
 int fast_solve(pps_handle *h, const double *dA, double *dp, double maxError,
                int maxit, int norm, int *it, double *tres2, hipStream_t s,
                bool relative = false, const ProjK *proj = nullptr,
-               const RowK *rows = nullptr);
+               const RowK *rows = nullptr, const FacesK *faces = nullptr);
 int fast_adjoint(pps_handle *h, const double *dA, const double *dg,
-                 double *dlam, double rtol, int maxit, int *it, hipStream_t s);
+                 double *dlam, double rtol, int maxit, int *it, hipStream_t s,
+                 const FacesK *faces = nullptr);
 void fast_destroy(FastState *f);
 // forget the built operator (the next solve sets it up again)
 void fast_invalidate(FastState *f);
