@@ -1725,6 +1725,23 @@ k_axpy_d(long long n, double *__restrict__ x, const TD *__restrict__ d, const do
         x[c] = __fma_rn(alpha, (double)d[c], x[c]);
 }
 
+// The residual of x = 0: r = b and its FP32 copy, partial sums of r^2 (0)
+// and of b^2 (1), which are the same.
+template <class TB>
+__global__ void __launch_bounds__(256)
+k_resid_zero(long long n, const double *__restrict__ b, double *__restrict__ r, TB *__restrict__ bt,
+             double *__restrict__ part) {
+    double rr = 0.0;
+    for (long long c = blockIdx.x * 256LL + threadIdx.x; c < n; c += (long long)gridDim.x * 256) {
+        const double v = b[c];
+        r[c] = v;
+        if (bt) bt[c] = (TB)v;
+        rr += v * v;
+    }
+    block_partial(rr, part);
+    block_partial(rr, part + kRedBlocks);
+}
+
 // refinement step: x += z - zmean ; partial sums of z
 template <class TZ>
 __global__ void __launch_bounds__(256)
@@ -3311,20 +3328,22 @@ int fast_adjoint(pps_handle *h, const double *dA, const double *dg, double *dlam
         k_sub_mean<<<nb, 256, 0, s>>>(n, V.b, part, nb);
     }
     k_fill<<<nblk(n), 256, 0, s>>>(V.x, n, 0.0);
-    // absolute tolerance from the right-hand side norm
+    // from x = 0 the residual is g itself: r, its FP32 copy and r.r = g.g
+    // (the absolute tolerance) without a stencil pass; the outer iteration
+    // starts from them
     double *part = f->red.as<double>();
     double *S = part + 4 * kRedBlocks;
-    const OpK O = opk(f);
-    const ZGrid zg = zgrid(d.nx, d.ny, d.nz);
-    if (f->exact) k_outer_resid<float, true><<<zg.grid, dim3(64, 4, 1), 0, s>>>(O, zg.zc, V.b, V.x, V.r, nullptr, part);
-    else k_outer_resid<float, false><<<zg.grid, dim3(64, 4, 1), 0, s>>>(O, zg.zc, V.b, V.x, V.r, nullptr, part);
-    k_scal_rr<<<1, 1024, 0, s>>>(part, zg.blocks, 1, S);
-    PPS_HIP(hipMemcpyAsync(h->pinned, S + S_BB, sizeof(double), hipMemcpyDeviceToHost, s));
-    PPS_HIP(hipStreamSynchronize(s));
-    const double tol = rtol * std::sqrt(h->pinned[0] / (double)n);
-    f->have_r = 0;   // the outer iteration recomputes r (and its FP32 copy) for g
+    const int nbr = nblk(n);
+    float *b32 = mixed ? f->b32.as<float>() : nullptr;
+    k_resid_zero<float><<<nbr, 256, 0, s>>>(n, V.b, V.r, b32, part);
+    k_scal_rr<<<1, 1024, 0, s>>>(part, nbr, 1, S, h->mbox, ++h->mseq);
+    PPS_HIP(hipGetLastError());
+    PPS_TRY(wait_mailbox(h, s));
+    const double gg = h->mbox->bb;
+    const double tol = rtol * std::sqrt(gg / (double)n);
+    f->have_r = 1;
     h->hist.clear();
-    if (h->pinned[0] == 0.0) {
+    if (gg == 0.0) {
         // g is zero (after the mean-free projection): lam = 0 exactly
         PPS_HIP(hipMemsetAsync(dlam, 0, sizeof(double) * d.padded(), s));
         PPS_HIP(hipStreamSynchronize(s));
