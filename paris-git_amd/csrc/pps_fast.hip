@@ -1749,6 +1749,16 @@ k_resid_zero(long long n, const double *__restrict__ b, double *__restrict__ r, 
     block_partial(rr, part + kRedBlocks);
 }
 
+// the outer iteration's scalars at its start: first iteration, no pending
+// x update, not stopped
+__global__ void __launch_bounds__(64) k_outer_start(double *__restrict__ S) {
+    if (threadIdx.x == 0) {
+        S[S_FIRST] = 1.0;
+        S[S_ALPHA] = 0.0;
+        S[S_STOP] = 0.0;
+    }
+}
+
 // refinement step: x += z - zmean ; partial sums of z
 template <class TZ>
 __global__ void __launch_bounds__(256)
@@ -3081,9 +3091,7 @@ static int outer_iterate(pps_handle *h, Hier<T> &H, bool mixed, double tol_abs, 
         k_scal_rr<<<1, 1024, 0, s>>>(part, zg.blocks, 1, S, h->mbox, ++h->mseq);
     }
     f->have_r = 0;
-    k_fill<<<1, 64, 0, s>>>(S + S_FIRST, 1, 1.0);
-    k_fill<<<1, 64, 0, s>>>(S + S_ALPHA, 1, 0.0);   // no pending x update
-    k_fill<<<1, 64, 0, s>>>(S + S_STOP, 1, 0.0);
+    k_outer_start<<<1, 64, 0, s>>>(S);
     PPS_HIP(hipGetLastError());
     // r.r of the setup pass or of the residual above
     PPS_TRY(wait_mailbox(h, s));
