@@ -295,7 +295,7 @@ int pps_faces_to_A_device(const pps_grid *grid, const double *cface,
  * forming A: the rows (the same products and sums, bit for bit) are taken
  * from the face values inside the fast solve's setup pass.  Same p, it and
  * tres2 as that pair of calls; b interior (nx*ny*nz).  Fast methods only,
- * opt.check_setup 0 (else PPS_ERR_UNSUPPORTED).  The hierarchy stays known
+ * opt.check_setup 0, no cell widths (else PPS_ERR_UNSUPPORTED).  The hierarchy stays known
  * by cface's address for pps_adjoint_faces_device.  The system is
  * SetupPoisson's (modules.f90:2737-2743) with face values c_f; the solve
  * replaces NewSolver (newsolver.f90:34-57) on it. */
@@ -307,7 +307,8 @@ int pps_solve_faces_device(pps_handle *h, const double *cface,
 /* pps_adjoint_device for the system of face values cface (weights w): the
  * hierarchy of the last pps_solve_faces_device on the same cface address
  * and method is reused, else set up from the face values.  Same lam and it
- * as pps_adjoint_device on the A that pps_faces_to_A_device forms. */
+ * as pps_adjoint_device on the A that pps_faces_to_A_device forms.  No cell
+ * widths (else PPS_ERR_UNSUPPORTED). */
 int pps_adjoint_faces_device(pps_handle *h, const double *cface,
                              const double *wx, const double *wy,
                              const double *wz, const double *dg,

@@ -320,6 +320,7 @@ class Solver:
         o.tail_cells = int(tail_cells)
         o.transfer = TRANSFERS[transfer] if isinstance(transfer, str) else int(transfer)
         self.options = o
+        self.has_cell_widths = False
         h = ctypes.c_void_p()
         _check(_lib().pps_init(ctypes.byref(grid), ctypes.byref(o), ctypes.byref(h)), "pps_init")
         self._h = h
@@ -351,11 +352,13 @@ class Solver:
         volume, which symmetrises SetupPoisson's system."""
         if dx is None and dy is None and dz is None:
             _check(_lib().pps_set_cell_widths(self._h, None, None, None), "pps_set_cell_widths")
+            self.has_cell_widths = False
             return
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (dx, dy, dz)]
         g = self.grid
         ptrs = [_ptr(a, m, nm) for a, m, nm in zip(arrs, (g.nx, g.ny, g.nz), ("dx", "dy", "dz"))]
         _check(_lib().pps_set_cell_widths(self._h, *ptrs), "pps_set_cell_widths")
+        self.has_cell_widths = True
 
     # -- reference signatures on host arrays ---------------------------
     def new_solver(self, A, p, maxError, beta=1.2, maxit=50, norm=2):
@@ -802,7 +805,7 @@ def _autograd():
         forward(cface (3,nz,ny,nx), b (nz,ny,nx), solver, weights, tol, maxit)
         returns the interior pressure (nz,ny,nx).  With the fast methods the
         8-slot system is never formed: the solve takes its rows from the
-        face values.  backward solves the adjoint M lam = grad_p on the same
+        face values (without cell widths on the solver).  backward solves the adjoint M lam = grad_p on the same
         hierarchy (M is symmetric) and forms grad_c = -w (lam_L - lam_R)
         (p_L - p_R) with the fused face kernel; grad_b = lam.  The forward
         pressure is kept, not recomputed.
@@ -816,7 +819,7 @@ def _autograd():
             p = torch.zeros(grid.shape_p, dtype=torch.float64, device=b.device)
             bn = float(torch.sqrt((bc ** 2).mean()))
             tol_abs = tol * bn if bn > 0 else tol
-            if solver.options.method in (METHODS["fp64"], METHODS["mixed"]):
+            if solver.options.method in (METHODS["fp64"], METHODS["mixed"]) and not solver.has_cell_widths:
                 solver.solve_faces(cf, weights, bc, p, tol_abs, maxit, allow_maxit=False)
                 A = None
             else:

@@ -490,7 +490,10 @@ extern "C" int pps_solve_faces_device(pps_handle *h, const double *cface, const 
     if (!h || !cface || !wx || !wy || !wz || !b || !dp || !it || !tres2 || maxit < 1)
         return PPS_ERR_BAD_ARG;
     const int method = h->opt.method;
-    if ((method != PPS_MG_FP64 && method != PPS_MG_MIXED) || h->opt.check_setup) return PPS_ERR_UNSUPPORTED;
+    // the face-value system is symmetric as it stands: cell widths (the row
+    // weights of SetupPoisson's stretched system) do not apply to it
+    if ((method != PPS_MG_FP64 && method != PPS_MG_MIXED) || h->opt.check_setup || h->have_widths)
+        return PPS_ERR_UNSUPPORTED;
     (void)hipSetDevice(h->device);
     hipStream_t s = static_cast<hipStream_t>(stream);   // NULL: the default stream
     h->setup_ms = 0.0;
@@ -507,6 +510,7 @@ extern "C" int pps_adjoint_faces_device(pps_handle *h, const double *cface, cons
                                         double *dlam, double rtol, int maxit, int *it, void *stream) {
     if (!h || !cface || !wx || !wy || !wz || !dg || !dlam || !it || !(rtol > 0.0) || maxit < 1)
         return PPS_ERR_BAD_ARG;
+    if (h->have_widths) return PPS_ERR_UNSUPPORTED;
     (void)hipSetDevice(h->device);
     hipStream_t s = static_cast<hipStream_t>(stream);
     const int saved = h->opt.method;

@@ -98,3 +98,13 @@ def test_solve_faces_refusals(pps):
         with pytest.raises(pps.PPSError) as e:
             s.solve_faces(cf, w, b, p, 1e-8)
         assert e.value.code == pps.ERR_UNSUPPORTED
+    # cell widths weight SetupPoisson's rows; the face-value system is
+    # symmetric already and takes none
+    import numpy as np
+    with pps.Solver(grid, method="mixed") as s:
+        s.set_cell_widths(np.full(16, 1 / 16), np.full(16, 1 / 16), np.full(16, 1 / 16))
+        for call in (lambda: s.solve_faces(cf, w, b, p, 1e-8),
+                     lambda: s.adjoint_faces(cf, w, p, torch.empty_like(p))):
+            with pytest.raises(pps.PPSError) as e:
+                call()
+            assert e.value.code == pps.ERR_UNSUPPORTED
