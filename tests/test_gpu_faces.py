@@ -5,6 +5,7 @@ faces_to_A followed by pps_solve_device: p with its ghosts, the iteration
 count and tres2 bit for bit; the adjoint from face values
 (pps_adjoint_faces_device) against pps_adjoint_device on that A, on the
 hierarchy of the solve and set up afresh."""
+import numpy as np
 import pytest
 
 from helpers import faces_from_rho
@@ -100,7 +101,6 @@ def test_solve_faces_refusals(pps):
         assert e.value.code == pps.ERR_UNSUPPORTED
     # cell widths weight SetupPoisson's rows; the face-value system is
     # symmetric already and takes none
-    import numpy as np
     with pps.Solver(grid, method="mixed") as s:
         s.set_cell_widths(np.full(16, 1 / 16), np.full(16, 1 / 16), np.full(16, 1 / 16))
         for call in (lambda: s.solve_faces(cf, w, b, p, 1e-8),
