@@ -286,23 +286,12 @@ k_rb_sweep(LevelK<T> L, const T *__restrict__ b, const T *__restrict__ xin, T *_
     // one row of nx values from src into LDS row lrow of buf: each lane of a
     // wavefront moves 4 consecutive values (tpr is a multiple of 64, so a
     // wavefront lies in one thread row and its LDS base is uniform)
+    static_assert(!LOP || PX == 2, "the LDS-staged sweep runs two pairs a thread");
     auto dma_row = [&](T *buf, int lrow, const T *src) {
         const int t0 = threadIdx.x & ~63;
-        if constexpr (PX == 2) {
-            __builtin_amdgcn_global_load_lds(
-                (const void *)(src + 4 * threadIdx.x),
-                (__attribute__((address_space(3))) void *)(buf + lrow * nx + 4 * t0), 16, 0, 0);
-        } else {
-            // one pair a thread: the wavefront's 128 values in two passes of
-            // one value a lane
-            const int l = threadIdx.x & 63;
-            __builtin_amdgcn_global_load_lds(
-                (const void *)(src + 2 * t0 + l),
-                (__attribute__((address_space(3))) void *)(buf + lrow * nx + 2 * t0), 4, 0, 0);
-            __builtin_amdgcn_global_load_lds(
-                (const void *)(src + 2 * t0 + 64 + l),
-                (__attribute__((address_space(3))) void *)(buf + lrow * nx + 2 * t0 + 64), 4, 0, 0);
-        }
+        __builtin_amdgcn_global_load_lds(
+            (const void *)(src + 4 * threadIdx.x),
+            (__attribute__((address_space(3))) void *)(buf + lrow * nx + 4 * t0), 16, 0, 0);
     };
     auto dma_op = [&](T *buf, int k) {
         const long long pk = sz * wrapz(k);
@@ -2490,6 +2479,9 @@ static void rb_pass(const Level<T> &L, int l, const T *b, int colour, bool first
 // (128^3: 6 planes, 220 blocks, 18 us against 30 us for 640 blocks of 2
 // planes; 256^3 FP32 with two pairs per thread: 20 planes, 247 blocks,
 // 109 us against 124 us for one pair per thread).
+// Thread rows: at most 8 on levels of at most 32 pairs a row (64^3, 32^3 at
+// 256^3), whose small blocks then run several to a CU and overlap their
+// plane steps: the 256^3 mixed solve 7.46 -> 7.35 ms (32 and 30 rows before).
 struct SweepShape {
     int px, tpr, rb, oy, zc, ytiles;
     bool lop;   // couplings staged through LDS (k_rb_sweep<..., LOP>)
@@ -2502,11 +2494,14 @@ static SweepShape sweep_shape1(int nx, int ny, int nz, int max_rows, int planes,
     SweepShape sh;
     const int np = nx / 2;
     sh.px = (sizeof(T) == 4 && dgi && np % 128 == 0) ? 2 : 1;
-    // LOP: a wavefront must lie in one thread row (tpr a multiple of 64)
-    sh.lop = lop && dgi && sizeof(T) == 4 && (sh.px == 2 || np % 64 == 0);
+    // LOP: two pairs a thread (the one-pair form on 128-wide levels, with
+    // the couplings staged the same way, ran slower than the register form:
+    // the 256^3 mixed solve 7.46 against 7.49 ms)
+    sh.lop = lop && dgi && sizeof(T) == 4 && sh.px == 2;
     sh.tpr = np / sh.px;
     int rb = (sh.lop ? kLopThreads : 1024) / sh.tpr;
     if (max_rows > 0 && rb > max_rows) rb = max_rows;
+    if (max_rows <= 0 && np <= 32 && rb > 8) rb = 8;
     const int lds_rows = (int)((65536 / ((long long)np * sizeof(T)) - 8) / 6);
     if (rb > lds_rows) rb = lds_rows;
     if (rb > ny + 2) rb = ny + 2;
@@ -2630,8 +2625,7 @@ static void sweep_launch2(const SweepShape &sh, const Level<T> &L, int l, const 
     if constexpr (sizeof(T) == 4 && DGI) {
         // couplings staged through LDS
         if (sh.lop) {
-            if (sh.px == 2) sweep_lop<T, FIRST, DGI, PROL, 2>(sh, K, L, l, b, c1, ec, dot, s, ev, skip);
-            else sweep_lop<T, FIRST, DGI, PROL, 1>(sh, K, L, l, b, c1, ec, dot, s, ev, skip);
+            sweep_lop<T, FIRST, DGI, PROL, 2>(sh, K, L, l, b, c1, ec, dot, s, ev, skip);
             return;
         }
         // two pairs per thread, the LDS buffers not fitting (no run-ahead
