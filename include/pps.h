@@ -259,7 +259,10 @@ int pps_last_history(pps_handle *h, double *hist, int cap);
 
 /* Adjoint of the last-solved system (no reference counterpart).  With
  * M = diag(A7) - offdiag(A1..A6), solves M^T lam = g (g projected mean-free
- * when M is singular) on the same hierarchy, to relative residual rtol.
+ * when M is singular) to relative residual rtol, on the hierarchy of the
+ * last solve when dA is that solve's A (same address and method; the
+ * caller keeps its contents unchanged in between), else on one set up
+ * from dA.
  * M^T = M when M is symmetric; with cell widths set (pps_set_cell_widths)
  * the volume-weighted rows W M are symmetric and lam = W (W M)^-1 g.
  * dg, dlam: padded like p. */
@@ -305,8 +308,9 @@ int pps_solve_faces_device(pps_handle *h, const double *cface,
                            double maxError, int maxit, int *it,
                            double *tres2, void *stream);
 /* pps_adjoint_device for the system of face values cface (weights w): the
- * hierarchy of the last pps_solve_faces_device on the same cface address
- * and method is reused, else set up from the face values.  Same lam and it
+ * hierarchy of the last pps_solve_faces_device on the same cface, wx, wy,
+ * wz addresses and method is reused (the caller keeps their contents
+ * unchanged in between), else set up from the face values.  Same lam and it
  * as pps_adjoint_device on the A that pps_faces_to_A_device forms.  No cell
  * widths (else PPS_ERR_UNSUPPORTED). */
 int pps_adjoint_faces_device(pps_handle *h, const double *cface,

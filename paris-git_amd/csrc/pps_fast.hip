@@ -2284,6 +2284,7 @@ struct FastState {
     int singular = 1;
     int exact = 0;   // every A7 is the coupling sum of outer_diag, bit for bit
     const double *built_for = nullptr;
+    const double *built_w[3] = {nullptr, nullptr, nullptr};   // face weights (face-value systems)
     int built_mixed = 0;   // precision of the hierarchy last built for built_for
     Hier<double> h64;
     Hier<float> h32;
@@ -3049,6 +3050,9 @@ static int fast_setup(pps_handle *h, const double *dA, const double *dp, bool mi
     // an operator assembled from fields is not kept for a later adjoint; one
     // from face values is known by their address
     f->built_for = rows ? nullptr : (faces ? faces->cf : dA);
+    f->built_w[0] = faces ? faces->wx : nullptr;
+    f->built_w[1] = faces ? faces->wy : nullptr;
+    f->built_w[2] = faces ? faces->wz : nullptr;
     f->built_mixed = mixed ? 1 : 0;
     f->op_in_A = own ? 0 : 1;
     return PPS_OK;
@@ -3326,8 +3330,13 @@ int fast_adjoint(pps_handle *h, const double *dA, const double *dg, double *dlam
     const long long n = d.cells();
     // rebuild only if the operator changed; the fold of wall ghosts into
     // the right-hand side is irrelevant here (the adjoint ghosts are zero)
+    // (a face-value system is known by the addresses of its values and of
+    // its weights)
     const double *op = faces ? faces->cf : dA;
-    if (f->built_for != op || f->built_mixed != (mixed ? 1 : 0)) {
+    const bool same_w = faces ? (f->built_w[0] == faces->wx && f->built_w[1] == faces->wy &&
+                                 f->built_w[2] == faces->wz)
+                              : f->built_w[0] == nullptr;
+    if (f->built_for != op || !same_w || f->built_mixed != (mixed ? 1 : 0)) {
         PPS_TRY(fast_setup(h, dA, dlam, mixed, s, nullptr, false, faces));
     }
     Vecs V = vecs(f);

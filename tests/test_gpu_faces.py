@@ -92,6 +92,30 @@ def test_adjoint_faces_equals_adjoint_on_A(pps, shape, bc, method):
     assert it2 == it_ref and torch.equal(lam2, lam_ref)
 
 
+def test_adjoint_faces_with_other_weights_sets_up_afresh(pps):
+    """The hierarchy of a face-value solve is known by the addresses of the
+    face values and of the weights: the same values with other weights is
+    another system."""
+    _say("adjoint_faces, other weights")
+    grid, cf, w, b = _system(pps, (32, 32, 32), (0,) * 6, seed=8)
+    w2 = tuple(wi * 1.5 for wi in w)
+    A2 = pps.faces_to_A(grid, cf, w2, b)
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    g = torch.zeros(grid.shape_p, dtype=torch.float64, device="cuda")
+    grid.interior(g).copy_(torch.randn(grid.shape_A[1:], generator=gen, device="cuda", dtype=torch.float64))
+    brms = float(torch.sqrt((b ** 2).mean()))
+    with pps.Solver(grid, method="mixed") as s:
+        lam_ref = torch.empty_like(g)
+        it_ref = s.adjoint(A2, g, lam_ref, rtol=1e-11, maxit=60)
+    with pps.Solver(grid, method="mixed") as s:
+        p = torch.zeros(grid.shape_p, dtype=torch.float64, device="cuda")
+        s.solve_faces(cf, w, b, p, 1e-10 * brms, 60, allow_maxit=False)
+        lam = torch.empty_like(g)
+        it = s.adjoint_faces(cf, w2, g, lam, rtol=1e-11, maxit=60)
+    _say(f"   it {it} / {it_ref}")
+    assert it == it_ref and torch.equal(lam, lam_ref)
+
+
 def test_solve_faces_refusals(pps):
     grid, cf, w, b = _system(pps, (16, 16, 16), (0,) * 6, seed=2)
     p = torch.zeros(grid.shape_p, dtype=torch.float64, device="cuda")
