@@ -1434,6 +1434,11 @@ k_tail(TailK<T> K) {
 // FP64 outer iteration kernels.  Scalars live on the device:
 // ---------------------------------------------------------------------
 enum { S_RZ = 0, S_ZMEAN, S_BETA, S_DQ, S_ALPHA, S_RR, S_BB, S_FIRST, S_STOP, S_N };
+// alpha of the direction in ring slot j: S[S_AH + j] (j < kRingSlots;
+// the scalars take 64 doubles)
+constexpr int kRingSlots = 9;   // directions held for deferred x updates
+constexpr int S_AH = S_N;
+static_assert(S_AH + kRingSlots <= 64, "ring alphas fit the scalars");
 
 // 2.5-D launch shape for z-marching kernels: (64, 4) threads over (x, y),
 // each thread walks `zc` consecutive planes; at most kRedBlocks blocks so
@@ -1596,7 +1601,9 @@ k_dq_tiled(OpK L, TGrid g, const TZ *__restrict__ z, const TZ *__restrict__ dold
     const int k0 = ch * g.zc, k1 = min(k0 + g.zc, nz);
     const bool in = i < nx && j < ny;
     const double beta = S[S_BETA], zm = S[S_ZMEAN], ax = S[S_ALPHA];
-    const bool use_old = beta != 0.0, upd = ax != 0.0;
+    // x null: the previous direction is not applied here (the caller
+    // defers it)
+    const bool use_old = beta != 0.0, upd = x && ax != 0.0;
     const bool rd_old = use_old || upd;
     const long long sy = nx, sz = (long long)nx * ny;
     auto wrapn = [](int a, int n) { a %= n; return a < 0 ? a + n : a; };
@@ -1790,11 +1797,47 @@ k_scal_beta(const double *__restrict__ part, int nb, long long n, int singular,
 
 // the alpha step from k_dq's partials
 __global__ void __launch_bounds__(1024)
-k_scal_alpha(const double *__restrict__ part, int nb, double *__restrict__ S) {
+k_scal_alpha(const double *__restrict__ part, int nb, double *__restrict__ S, int slot) {
     const double dq = sum_partials(part, nb);
     if (threadIdx.x == 0) {
         S[S_DQ] = dq;
         S[S_ALPHA] = S[S_RZ] / dq;
+        S[S_AH + slot] = S[S_ALPHA];
+    }
+}
+
+// x += alpha_j d_j for the directions of ring slots first, first+1, ...
+// (count of them, modulo R, alphas S[S_AH + slot]) one after another in
+// that order, each an fma as k_dq_tiled's and k_check_tiled's update: x is
+// bit for bit what applying them one iteration at a time gives.  W cells a
+// thread (16-byte loads of the directions when W = 16 / sizeof(T); n a
+// multiple of W); the directions are read once (non-temporal).
+template <class T, int W>
+__global__ void __launch_bounds__(256)
+k_flush(long long n, double *__restrict__ x, const T *__restrict__ ring, const double *__restrict__ S,
+        int first, int count, int R) {
+    using Vt = T __attribute__((ext_vector_type(W)));
+    const long long nv = n / W;
+    for (long long e = blockIdx.x * 256LL + threadIdx.x; e < nv; e += (long long)gridDim.x * 256) {
+        Vt dv[kRingSlots];
+#pragma unroll
+        for (int m = 0; m < kRingSlots; ++m)
+            if (m < count)
+                dv[m] = __builtin_nontemporal_load(
+                    reinterpret_cast<const Vt *>(ring + (long long)((first + m) % R) * n) + e);
+        double v[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) v[w] = x[e * W + w];
+#pragma unroll
+        for (int m = 0; m < kRingSlots; ++m) {
+            if (m < count) {
+                const double al = S[S_AH + (first + m) % R];
+#pragma unroll
+                for (int w = 0; w < W; ++w) v[w] = __fma_rn(al, (double)dv[m][w], v[w]);
+            }
+        }
+#pragma unroll
+        for (int w = 0; w < W; ++w) x[e * W + w] = v[w];
     }
 }
 
@@ -2289,6 +2332,7 @@ struct FastState {
     Hier<double> h64;
     Hier<float> h32;
     DevBuf b32;      // FP32 right-hand side of the mixed V-cycle
+    DevBuf ring;     // the outer CG's directions, kRingSlots slots (deferred x updates)
     int mixed_built = 0, fp64_built = 0;
     // the outer operator is read from the caller's A (mixed mode: the FP64
     // compact operator is not written)
@@ -2313,6 +2357,7 @@ void fast_invalidate(FastState *f) {
 void fast_destroy(FastState *f) {
     if (!f) return;
     f->op.release(); f->vec.release(); f->red.release(); f->flags.release(); f->b32.release();
+    f->ring.release();
     for (auto &b : f->h64.bufs) b.release();
     for (auto &b : f->h32.bufs) b.release();
     f->h64.minv.release(); f->h64.work.release();
@@ -3128,7 +3173,32 @@ static int outer_iterate(pps_handle *h, Hier<T> &H, bool mixed, double tol_abs, 
     for (int a = 0; a < 3; ++a) cy.per[a] = f->d.per[a];
     double t2 = std::sqrt(rr0 / N);
     *last = t2;
-    int k = 1, cur = 0;
+    int k = 1;
+    // The direction d_k goes to ring slot k % R.  With R = kRingSlots the
+    // x += alpha_k d_k updates are deferred: the directions stay in the ring
+    // with their alphas (S[S_AH + slot]) and are applied together by
+    // k_flush, when the ring is full and after the last iteration, in
+    // iteration order and each as the same fma, so x is bit for bit that of
+    // one update per iteration, without x being read and written in every
+    // k_dq_tiled (256^3: 16 of its 60 bytes a cell).  The no-allocation
+    // solve keeps two slots (V.d), and k_dq_tiled applies the previous
+    // direction itself.
+    const bool lazy = cg && !f->arena;
+    const int R = lazy ? kRingSlots : 2;
+    if (lazy) PPS_TRY(f->ring.ensure(sizeof(T) * (size_t)R * (size_t)n));
+    auto dslot = [&](int kk) -> T * {
+        const int j = kk % R;
+        return lazy ? f->ring.as<T>() + (long long)j * n : (T *)(void *)V.d[j];
+    };
+    int p0 = 1;   // the first direction not yet applied to x
+    auto flush = [&](int upto) {   // apply d_p0 .. d_upto
+        constexpr int W = 16 / sizeof(T);
+        if (upto >= p0 && n % W == 0)
+            k_flush<T, W><<<nblk(n / W), 256, 0, s>>>(n, V.x, f->ring.as<T>(), S, p0 % R, upto - p0 + 1, R);
+        else if (upto >= p0)
+            k_flush<T, 1><<<nb, 256, 0, s>>>(n, V.x, f->ring.as<T>(), S, p0 % R, upto - p0 + 1, R);
+        p0 = upto + 1;
+    };
     if (t2 == 0.0) {   // exact solution: nothing to do
         if (f->coarse_pending) PPS_HIP(hipStreamSynchronize(h->side));
         f->coarse_pending = 0;
@@ -3159,17 +3229,20 @@ static int outer_iterate(pps_handle *h, Hier<T> &H, bool mixed, double tol_abs, 
                 nbz = nb;
             }
             k_scal_beta<<<1, 1024, 0, s>>>(part, nbz, n, f->singular, S);
-            T *dold = (T *)(void *)V.d[cur], *dnew = (T *)(void *)V.d[cur ^ 1];
+            // d_k's slot holds d_(k-R): applied first when still pending
+            if (lazy && k - p0 >= R) flush(k - 1);
+            T *dold = dslot(k - 1), *dnew = dslot(k);
+            double *xu = lazy ? nullptr : V.x;
             if (f->exact)
-                k_dq_tiled<T, true><<<tg.blocks, dim3(64, 16, 1), 0, s>>>(O, tg, z, dold, dnew, V.q, V.x, S,
+                k_dq_tiled<T, true><<<tg.blocks, dim3(64, 16, 1), 0, s>>>(O, tg, z, dold, dnew, V.q, xu, S,
                                                                           part);
             else
-                k_dq_tiled<T, false><<<tg.blocks, dim3(64, 16, 1), 0, s>>>(O, tg, z, dold, dnew, V.q, V.x, S,
+                k_dq_tiled<T, false><<<tg.blocks, dim3(64, 16, 1), 0, s>>>(O, tg, z, dold, dnew, V.q, xu, S,
                                                                            part);
-            k_scal_alpha<<<1, 1024, 0, s>>>(part, tg.blocks, S);
+            if (!lazy) p0 = k;   // d_(k-1) applied in k_dq_tiled
+            k_scal_alpha<<<1, 1024, 0, s>>>(part, tg.blocks, S, k % R);
             k_update<float><<<nb, 256, 0, s>>>(n, V.r, V.q, b32, S, part);
             k_scal_rr<<<1, 1024, 0, s>>>(part, nb, 0, S, h->mbox, ++h->mseq, nullptr, N, tol_abs);
-            cur ^= 1;
         } else {
             k_zsum<T><<<nb, 256, 0, s>>>(n, z, part);
             k_add_z<T><<<nb, 256, 0, s>>>(n, V.x, z, part, nb, f->singular);
@@ -3196,9 +3269,12 @@ static int outer_iterate(pps_handle *h, Hier<T> &H, bool mixed, double tol_abs, 
             break;
         }
     }
-    // the last iteration's x += alpha d
-    if (cg && pend) *pend = V.d[cur];
-    else if (cg) k_axpy_d<T><<<nb, 256, 0, s>>>(n, V.x, (const T *)(const void *)V.d[cur], S);
+    // the pending directions; the last one (alpha S[S_ALPHA]) left to the
+    // caller when asked
+    const int kl = k > maxit ? maxit : k;   // the last iteration run
+    if (cg && lazy) flush(pend ? kl - 1 : kl);
+    if (cg && pend) *pend = dslot(kl);
+    else if (cg && !lazy) k_axpy_d<T><<<nb, 256, 0, s>>>(n, V.x, dslot(kl), S);
     PPS_HIP(hipGetLastError());
     *it = k;
     return PPS_OK;
