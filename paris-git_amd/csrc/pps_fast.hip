@@ -1926,6 +1926,7 @@ struct CellAt {
     bool lastx, lasty, lastz;
 };
 struct RowsA {
+    static constexpr bool kDiagStored = true;
     const double *A;
     long long n;
     double a4n;   // the -z coupling of the next plane, carried down the march
@@ -1944,6 +1945,7 @@ struct RowsA {
     }
 };
 struct RowsF {
+    static constexpr bool kDiagStored = true;
     RowK R;
     int nx, ny, nz, ng;
     long long Py, Pz;
@@ -1987,12 +1989,17 @@ struct RowsF {
 // w c of the cell's six faces (0 through a wall), A7 their sum, A8 = b.
 // The system is symmetric by construction (a neighbour's lower coupling is
 // the same product as the row's upper one), so the check gets the row's
-// own value.
+// own value.  It is also singular, and every A7 is the coupling sum in
+// outer_diag's order, by construction: the diagonals are formed from the
+// couplings wherever they are used, and none is stored.  The -z coupling
+// of a row is the +z coupling of the row below, carried down the march.
 struct RowsC {
+    static constexpr bool kDiagStored = false;
     FacesK F;
     int nx, ny, nz, px, py, pz;
     double up[3];
-    __device__ void begin(long long) {}
+    bool have_dn;   // up[2] holds the -z coupling of the next row
+    __device__ void begin(long long) { have_dn = false; }
     __device__ int row(const CellAt &e, double a[8]) {
 #pragma clang fp contract(off)
         const long long n = (long long)nx * ny * nz, sy = nx, sz = (long long)nx * ny;
@@ -2003,8 +2010,10 @@ struct RowsC {
         a[1] = (i < nx - 1 || px) ? F.wx[i] * cx[c] : 0.0;
         a[2] = j > 0 ? F.wy[j - 1] * cy[c - sy] : (py ? F.wy[ny - 1] * cy[c + (long long)(ny - 1) * sy] : 0.0);
         a[3] = (j < ny - 1 || py) ? F.wy[j] * cy[c] : 0.0;
-        a[4] = k > 0 ? F.wz[k - 1] * cz[c - sz] : (pz ? F.wz[nz - 1] * cz[c + (long long)(nz - 1) * sz] : 0.0);
+        a[4] = have_dn ? up[2]
+                       : (k > 0 ? F.wz[k - 1] * cz[c - sz] : (pz ? F.wz[nz - 1] * cz[c + (long long)(nz - 1) * sz] : 0.0));
         a[5] = (k < nz - 1 || pz) ? F.wz[k] * cz[c] : 0.0;
+        have_dn = true;
         a[6] = ((((a[0] + a[1]) + a[2]) + a[3]) + a[4]) + a[5];
         a[7] = F.b ? F.b[c] : 0.0;
         up[0] = a[1];
@@ -2103,13 +2112,16 @@ k_setup(PackK d, int zc, Src src, const double *__restrict__ p,
                 const double vzm = (k == 0 && !d.pz) ? 0.0 : a[4];
                 if (a[6] != ((((vxm + vx) + vym) + vy) + vzm) + vz) f |= 8;
             }
-            if (cx) { cx[c] = vx; cy[c] = vy; cz[c] = vz; dg[c] = a[6]; }
+            if (cx) {
+                cx[c] = vx; cy[c] = vy; cz[c] = vz;
+                if (Src::kDiagStored) dg[c] = a[6];
+            }
             bo[c] = rhs;
             x[c] = pc;
             r[c] = rv;
             if (f32) {
                 f32[c] = (float)vx; f32[n + c] = (float)vy; f32[2 * n + c] = (float)vz;
-                f32[3 * n + c] = (float)a[6];
+                if (Src::kDiagStored) f32[3 * n + c] = (float)a[6];
                 r32[c] = (float)rv;
             }
             rr += rv * rv;
