@@ -1576,9 +1576,10 @@ k_dot_rz(long long n, const double *__restrict__ r, const TZ *__restrict__ z, do
 // d_new = (z - zmean) + beta d_old, rounded to TZ (the direction is
 // stored in the precision of z) ; q = M d_new from the rounded values, so
 // that x += alpha d and r -= alpha q change r by exactly -M (alpha d) in
-// FP64 ; partial sum d.q.  The previous iteration's x += alpha_prev d_old
-// (alpha_prev = S[S_ALPHA], 0 for none) is applied here, where d_old is
-// read anyway.  On LDS tiles: d_new of every plane (own cells plus a
+// FP64 ; partial sum d.q.  Given x, the previous iteration's
+// x += alpha_prev d_old (alpha_prev = S[S_ALPHA], 0 for none) is applied
+// here, where d_old is read anyway (the no-allocation solve; the others
+// pass no x and defer the updates to k_flush).  On LDS tiles: d_new of every plane (own cells plus a
 // one-cell halo ring, coordinates wrapped) is written to one of three LDS
 // planes; the in-plane neighbours of q = M d_new are read there, the
 // vertical ones from registers.  The raw z / d_old of plane s+2 and the
