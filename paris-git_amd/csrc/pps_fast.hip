@@ -2940,7 +2940,7 @@ static int fast_ensure(pps_handle *h) {
     const long long n = f->d.cells();
     Arena *a = f->arena;
     PPS_TRY(f->op.ensure(sizeof(double) * 4 * n, a));
-    PPS_TRY(f->vec.ensure(sizeof(double) * 7 * n, a));
+    PPS_TRY(f->vec.ensure(sizeof(double) * 5 * n, a));
     PPS_TRY(f->red.ensure(sizeof(double) * (4 * kRedBlocks + 64), a));
     PPS_TRY(f->flags.ensure(64, a));
     if (!h->pinned) PPS_HIP(hipHostMalloc((void **)&h->pinned, 16 * sizeof(double), 0));
@@ -2965,14 +2965,13 @@ static int fast_ensure(pps_handle *h) {
 }
 
 struct Vecs {
-    double *b, *x, *r, *d[2], *q, *z;
+    double *b, *x, *r, *q, *z;
 };
 static Vecs vecs(FastState *f) {
     const long long n = f->d.cells();
     double *v = f->vec.as<double>();
     Vecs V;
-    V.b = v; V.x = v + n; V.r = v + 2 * n; V.d[0] = v + 3 * n; V.d[1] = v + 4 * n;
-    V.q = v + 5 * n; V.z = v + 6 * n;
+    V.b = v; V.x = v + n; V.r = v + 2 * n; V.q = v + 3 * n; V.z = v + 4 * n;
     return V;
 }
 static OpK opk(FastState *f) {
@@ -3194,15 +3193,12 @@ static int outer_iterate(pps_handle *h, Hier<T> &H, bool mixed, double tol_abs, 
     // iteration order and each as the same fma, so x is bit for bit that of
     // one update per iteration, without x being read and written in every
     // k_dq_tiled (256^3: 16 of its 60 bytes a cell).  The no-allocation
-    // solve keeps two slots (V.d), and k_dq_tiled applies the previous
-    // direction itself.
+    // solve keeps two slots (taken last from its arena), and k_dq_tiled
+    // applies the previous direction itself.
     const bool lazy = cg && !f->arena;
     const int R = lazy ? kRingSlots : 2;
-    if (lazy) PPS_TRY(f->ring.ensure(sizeof(T) * (size_t)R * (size_t)n));
-    auto dslot = [&](int kk) -> T * {
-        const int j = kk % R;
-        return lazy ? f->ring.as<T>() + (long long)j * n : (T *)(void *)V.d[j];
-    };
+    if (cg) PPS_TRY(f->ring.ensure(sizeof(T) * (size_t)R * (size_t)n, f->arena));
+    auto dslot = [&](int kk) -> T * { return f->ring.as<T>() + (long long)(kk % R) * n; };
     int p0 = 1;   // the first direction not yet applied to x
     auto flush = [&](int upto) {   // apply d_p0 .. d_upto
         constexpr int W = 16 / sizeof(T);
@@ -3515,7 +3511,7 @@ static size_t noalloc_bytes(const Dims &d, const double lengths[3], bool mixed) 
     size_t tot = 0;
     auto take = [&](size_t b) { tot = ((tot + 255) & ~(size_t)255) + b; };
     take(sizeof(double) * 4 * n);
-    take(sizeof(double) * 7 * n);
+    take(sizeof(double) * 5 * n);
     take(sizeof(double) * (4 * (size_t)kRedBlocks + 64));
     take(64);
     const size_t S = mixed ? sizeof(float) : sizeof(double);
@@ -3529,6 +3525,7 @@ static size_t noalloc_bytes(const Dims &d, const double lengths[3], bool mixed) 
     take(S * nb * nb);
     take(sizeof(double) * nb * nb);
     if (mixed) take(sizeof(float) * n);
+    take(S * 2 * n);   // the two direction slots (outer_iterate)
     return tot;
 }
 
