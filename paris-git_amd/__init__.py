@@ -20,6 +20,7 @@ Nothing here computes on the CPU: every operation calls libpps, and a
 missing library raises instead of falling back.
 """
 import ctypes
+import math
 import os
 
 import numpy as np
@@ -817,7 +818,8 @@ def _autograd():
             cf = cface.detach().contiguous()
             bc = b.detach().contiguous()
             p = torch.zeros(grid.shape_p, dtype=torch.float64, device=b.device)
-            bn = float(torch.sqrt((bc ** 2).mean()))
+            # rms(b) in one pass over b (no squared temporary)
+            bn = float(torch.linalg.vector_norm(bc)) / math.sqrt(bc.numel())
             tol_abs = tol * bn if bn > 0 else tol
             if solver.options.method in (METHODS["fp64"], METHODS["mixed"]) and not solver.has_cell_widths:
                 solver.solve_faces(cf, weights, bc, p, tol_abs, maxit, allow_maxit=False)
