@@ -207,7 +207,7 @@ int pps_solve_fields_device(pps_handle *h, const double *u, const double *v,
  * holds from the library's first use until the process ends; pps_init
  * looks its address up.  Limits: methods PPS_MG_FP64 / PPS_MG_MIXED, norm 2,
  * opt.check_setup 0, maxit >= 1 (else PPS_ERR_UNSUPPORTED / PPS_ERR_BAD_ARG);
- * grids up to about 1.1 million cells (96^3 in either precision), larger
+ * grids up to about 1.1 million cells (104^3 mixed, 108^3 FP64), larger
  * ones return PPS_ERR_SCRATCH before any work is queued.  Every call on
  * one GPU shares the array and nothing carries over between calls; calls
  * that overlap in time on one GPU (several threads or streams) race.
